@@ -1,0 +1,179 @@
+"""Benchmark of the hot path on one MI355X.
+
+Workload (the 10 MHz plain-FP32 configuration at scale): 16 satellites on
+circular 300 km lunar orbits -> 120 baselines x 8640 steps (1 036 800
+visibilities), HEALPix Nside 128 sky (196 608 pixels, log-normal(0, 1.5) plus
+200 S^-2 point sources), synthetic.  One step = stage 1 (orbits, caps, uvw on
+the device) + vis-gen + recon, inputs resident in HBM.
+value = (vis-gen terms + recon terms) / step time, terms = N_vis x N_pix.
+
+Also reported: the FP64-phase path on the same workload, the relative error
+of both against the FP64 oracle on sampled outputs, per-kernel times from HIP
+events on the context stream, the dominant kernel's roofline fraction, and the
+FP32 OpenMP CPU restatement timed on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP32_TFLOPS = 157.3          # MI355X published FP32 vector/matrix peak
+TRANS_PEAK = 256 * 2.4e9 * 16     # v_sin/v_cos issue: 16 lanes/clk/CU (quarter rate)
+FLOPS_PER_TERM = 10               # 3 FMA phase + 2 FMA accumulate
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="c2")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--cpu-stride", type=int, default=128)
+    return ap.parse_args()
+
+
+def timed_steps(torch, step, steps, warmup):
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for _ in range(steps):
+        step()
+    ev1.record()
+    torch.cuda.synchronize()
+    return ev0.elapsed_time(ev1) / steps
+
+
+def kernel_avg_ms(ctx, step, names, reps):
+    ctx.timing(True)
+    acc = {n: 0.0 for n in names}
+    for _ in range(reps):
+        step()
+        for n in names:
+            acc[n] += ctx.last_ms(n)
+    ctx.timing(False)
+    return {n: acc[n] / reps for n in names}
+
+
+def cpu_baseline(g, wl, pos, uvw, lmn, I, V, stride):
+    import oracle as o
+    sel = np.arange(0, wl.n_vis, stride, dtype=np.int64)
+    psel = np.arange(0, wl.n_pix, stride, dtype=np.int64)
+    o.cpu_visgen_f32(pos, g.pairs, uvw, lmn, I, g.r_moon_km, vis_sel=sel[:8])  # warm
+    t0 = time.perf_counter()
+    o.cpu_visgen_f32(pos, g.pairs, uvw, lmn, I, g.r_moon_km, vis_sel=sel)
+    t1 = time.perf_counter()
+    o.cpu_recon_f32(pos, g.pairs, uvw, lmn, V, g.r_moon_km, pix_sel=psel)
+    t2 = time.perf_counter()
+    terms = len(sel) * wl.n_pix + len(psel) * wl.n_vis
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": round(terms / (t2 - t0) / 1e9, 4), "unit": "Gterms/s", "cores": cores,
+            "kind": "port",
+            "sample": f"every {stride}th visibility x all {wl.n_pix} pixels (vis-gen) + every "
+                      f"{stride}th pixel x all {wl.n_vis} visibilities (recon): {terms:.3e} terms "
+                      f"in {t2 - t0:.1f} s; FP32 OpenMP restatement of visscal/imagerecon"}
+
+
+def accuracy(g, wl, pos, uvw, lmn, I, V, F):
+    import oracle as o
+    sel = np.linspace(0, wl.n_vis - 1, 32).astype(np.int64)
+    Vo = o.visgen(pos, g.pairs, uvw, lmn, I, g.r_moon_km, vis_sel=sel)[0]
+    Vc = V[0, sel, 0].astype(np.float64) + 1j * V[0, sel, 1]
+    psel = np.linspace(0, wl.n_pix - 1, 8).astype(np.int64)
+    Vfull = V[0, :, 0].astype(np.float64) + 1j * V[0, :, 1]
+    Fo = o.recon(pos, g.pairs, uvw, lmn, Vfull, g.r_moon_km, pix_sel=psel)[0]
+    return {"V_rel_l2": float(np.linalg.norm(Vc - Vo) / np.linalg.norm(Vo)),
+            "F_rel_l2": float(np.linalg.norm(F[0, psel] - Fo) / np.linalg.norm(Fo)),
+            "sample": "32 visibilities x all pixels; 8 pixels x all visibilities"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print(json.dumps({"error": "this build measures one GPU only; the work of several GPUs "
+                                   "runs on one MI355X at full size (use --gpus 1)"}))
+        sys.exit(2)
+    import torch
+    import viss_recon_shade_amd as vrs
+    from viss_recon_shade_amd import workload as W
+
+    wl = W.CONFIGS[args.workload]
+    ctx = vrs.Context(0)
+    g = ctx.workload_geometry(wl)
+    pos, uvw = g.MIncline()
+    lmn = g.lmn()
+    I = W.make_sky(wl)
+    It = torch.from_numpy(I).cuda()
+    kp = vrs.knobs(precision="plain")
+    km = vrs.knobs(precision="mixed")
+    state = {}
+
+    def step_with(k):
+        def step():
+            g.MIncline(export=False)
+            V = g.visscal(It, k)
+            state["V"] = V
+            state["F"] = g.imagerecon(V, None, k)
+        return step
+
+    terms = 2 * wl.terms_per_direction
+    ms_plain = timed_steps(torch, step_with(kp), args.steps, args.warmup)
+    kt_plain = kernel_avg_ms(ctx, step_with(kp), ["orbit", "visgen", "recon"], max(2, args.steps // 2))
+    Vp, Fp = state["V"].cpu().numpy(), state["F"].cpu().numpy()
+    ms_mixed = timed_steps(torch, step_with(km), max(2, args.steps // 2), 1)
+    kt_mixed = kernel_avg_ms(ctx, step_with(km), ["visgen", "recon"], 2)
+    Vm, Fm = state["V"].cpu().numpy(), state["F"].cpu().numpy()
+
+    value = terms / (ms_plain * 1e-3) / 1e9
+    dom = max(("visgen", "recon"), key=lambda n: kt_plain[n])
+    dom_ms = kt_plain[dom]
+    achieved = FLOPS_PER_TERM * wl.terms_per_direction / (dom_ms * 1e-3) / 1e12
+    out = {
+        "metric": "Gterms/s (vis x pix) vis-gen + recon",
+        "value": round(value, 2),
+        "unit": "Gterms/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_plain, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{wl.name}: {wl.note}", "n_vis": wl.n_vis, "n_pix": wl.n_pix,
+                   "n_maps": wl.n_maps, "freq_hz": wl.freq_hz, "precision": "VRS_PLAIN_F32",
+                   "step": "orbit+uvw+caps, vis-gen, recon"},
+        "roofline": {"bound": "valu", "kernel": dom, "achieved": round(achieved, 2),
+                     "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                     "flops_per_term": FLOPS_PER_TERM,
+                     "trans_frac": round(2 * wl.terms_per_direction / (dom_ms * 1e-3) / TRANS_PEAK, 4),
+                     "kernel_ms": round(dom_ms, 3)},
+        "kernels_ms": {k: round(v, 3) for k, v in kt_plain.items()},
+        "mixed_f64phase": {"value": round(terms / (ms_mixed * 1e-3) / 1e9, 2),
+                           "ms_per_step": round(ms_mixed, 3),
+                           "kernels_ms": {k: round(v, 3) for k, v in kt_mixed.items()}},
+    }
+    if not args.no_accuracy:
+        out["accuracy"] = {"plain": accuracy(g, wl, pos, uvw, lmn, I, Vp, Fp),
+                           "mixed": accuracy(g, wl, pos, uvw, lmn, I, Vm, Fm)}
+    if not args.no_cpu:
+        Vc = Vp[0, :, 0].astype(np.float64) + 1j * Vp[0, :, 1]
+        out["cpu_baseline"] = cpu_baseline(g, wl, pos, uvw, lmn, I[0], Vc, args.cpu_stride)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    main()

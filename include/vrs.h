@@ -1,0 +1,186 @@
+/*
+ * vrs.h -- C-ABI of the MI355X-native viss_recon_shade hot path.
+ *
+ * The reference (SSshuishui/viss_recon_shade) has no library interface: its
+ * boundary is a set of CUDA kernels taking raw device pointers plus scalars,
+ * chained through text files.  Each entry point below names the reference
+ * code it replaces.  All types are C99; no C++ or torch types cross this line.
+ *
+ * Conventions
+ *  - Every call returns vrs_status; nothing calls exit().  vrs_last_error()
+ *    gives a message for the last failing call on that context (or on the
+ *    calling thread when ctx is NULL).
+ *  - Array arguments may be device pointers (hipMalloc / torch tensors) or
+ *    plain host pointers; the kind is detected.  Device arrays are used in
+ *    place and the call is stream-ordered on the context stream; host arrays
+ *    are staged and the call returns after the results are copied back.
+ *  - Complex arrays are complex64, interleaved (re, im) float pairs.
+ *  - A context is bound to one device and one stream and is not thread-safe;
+ *    use one context per thread.
+ */
+#ifndef VRS_H
+#define VRS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum vrs_status {
+    VRS_OK = 0,
+    VRS_E_ARG = -1,   /* invalid argument (checked on the host before any GPU work) */
+    VRS_E_OOM = -2,   /* device allocation failed */
+    VRS_E_HIP = -3,   /* a HIP runtime call or kernel launch failed */
+    VRS_E_IO = -4,    /* file could not be read or written */
+    VRS_E_NODEV = -5, /* no usable GPU */
+    VRS_E_STATE = -6  /* object used in the wrong state */
+} vrs_status;
+
+typedef enum vrs_precision {
+    VRS_PLAIN_F32 = 0,      /* reference arithmetic: phase, sin/cos, sums in FP32 */
+    VRS_MIXED_F64PHASE = 1, /* phase formed and range-reduced in FP64, FP32 sincos + sums */
+    VRS_MFMA_SPLIT16 = 2    /* FP64 phase; phase factors split hi+lo FP16, contracted on MFMA */
+} vrs_precision;
+
+typedef struct vrs_ctx vrs_ctx;
+typedef struct vrs_geom vrs_geom;
+
+/* Circular inclined orbits (e = 0, argument of perigee 0).
+ * Replaces the constants and sampling of MIncline_generate_uvw_float.cu:22-30,185-190. */
+typedef struct vrs_orbit_desc {
+    int32_t n_sat;
+    int32_t n_t;           /* time steps t_j = j * dt_s, j = 0 .. n_t-1 */
+    double dt_s;
+    double a_km;           /* orbit radius (R_moon + altitude) */
+    double incl_rad;
+    double mu_km3_s2;      /* lunar GM; mean motion sqrt(mu / a^3) */
+    const double *raan_rad; /* [n_sat], host memory */
+    const double *m0_rad;   /* [n_sat], host memory: mean anomaly at t = 0 */
+} vrs_orbit_desc;
+
+/* Everything that fixes the (visibility, pixel) term set. */
+typedef struct vrs_geom_desc {
+    vrs_orbit_desc orbit;
+    double freq_hz;        /* lambda = c / freq, c = 299792458 m/s */
+    double r_moon_km;      /* occulting sphere radius */
+    int32_t n_base;        /* 0: all pairs a < b in lexicographic order */
+    const int32_t *pairs;  /* [n_base][2] satellite indices (host), or NULL with n_base = 0 */
+    int32_t nside;         /* HEALPix RING sky, N_pix = 12 nside^2 */
+} vrs_geom_desc;
+
+typedef struct vrs_geom_info {
+    int64_t n_vis;         /* n_base * n_t; visibility index v = b * n_t + t */
+    int64_t n_pix;
+    int32_t n_sat, n_t, n_base, nside;
+    int32_t n_chunk;       /* pixel chunks the kernels classify against the shade caps */
+    int32_t chunk_pixels;
+    double lambda_m;
+} vrs_geom_info;
+
+/* Scalar knobs.  The reference's conventions are knob values, not code paths:
+ * reference preset = { 0.5, 1, 240, 0 (=1/N_vis), 1/count weights passed to recon };
+ * canonical preset = { 1, 0, 240, 0 }.                                         */
+typedef struct vrs_knobs {
+    double phase_scale;    /* s: 1/dl of vissGen10M_shade.cu:183-185 (reference 0.5) */
+    int32_t wfac;          /* 1: trailing exp(-2 pi i s w) on V (vissGen10M_shade.cu:143) and n in recon */
+    float t_occ;           /* intensity of shaded pixels in vis-gen (vissGen10M_shade.cu:132: 240) */
+    double kappa;          /* recon normalisation; 0 selects 1/N_vis (vissGen10M_shade.cu:156,173) */
+    int32_t precision;     /* vrs_precision */
+} vrs_knobs;
+
+typedef struct vrs_shade_stats {
+    int64_t shaded_pairs;  /* total (vis, pixel) pairs shaded */
+    int64_t near_limb;     /* pairs within 1e-6 R_moon of either satellite's limb */
+    int64_t fp64_tests;    /* pixel tests that the FP32 guard band sent to FP64 */
+} vrs_shade_stats;
+
+/* ---------------------------------------------------------------- context */
+/* Replaces the per-thread cudaSetDevice / CHECK pattern (vissGen10M_shade.cu:291-298, error.cuh:4-18). */
+vrs_status vrs_create(int device, vrs_ctx **out);
+void vrs_destroy(vrs_ctx *ctx);
+const char *vrs_last_error(const vrs_ctx *ctx);
+/* Run on `hip_stream` from now on; NULL selects the null (legacy default) stream. */
+vrs_status vrs_set_stream(vrs_ctx *ctx, void *hip_stream);
+vrs_status vrs_synchronize(vrs_ctx *ctx);
+int vrs_device_count(void);
+const char *vrs_version(void);
+void vrs_knobs_default(vrs_knobs *k);
+void vrs_knobs_reference(vrs_knobs *k);
+/* Per-kernel timing with HIP events on the context stream (off by default). */
+vrs_status vrs_timing_enable(vrs_ctx *ctx, int on);
+/* Milliseconds of the last launch of `kernel` ("orbit", "shade", "visgen", "recon", "mfma_visgen",
+ * "mfma_recon", "batch"); -1 when unknown.  Synchronizes the stream. */
+double vrs_timing_last_ms(vrs_ctx *ctx, const char *kernel);
+
+/* --------------------------------------------------------------- geometry */
+/* Stage 1 and the static sky: propagates the orbits on the GPU, forms uvw and the
+ * per-(satellite, t) shade caps, lays the HEALPix pixels out in spatially compact
+ * chunks.  Replaces MIncline() (MIncline_generate_uvw_float.cu:184-440) up to the
+ * uvw it writes, and the l,m,n inputs of vissGen (vissGen10M_shade.cu:199-248). */
+vrs_status vrs_geom_create(vrs_ctx *ctx, const vrs_geom_desc *desc, vrs_geom **out);
+void vrs_geom_destroy(vrs_geom *g);
+vrs_status vrs_geom_info_get(const vrs_geom *g, vrs_geom_info *out);
+/* Re-runs stage 1 on the device (calculateOrbit + computeUVW,
+ * MIncline_generate_uvw_float.cu:83-110,153-160) and optionally exports
+ * positions [n_sat][n_t][3] km and uvw [n_base][n_t][3] wavelengths (FP64). */
+vrs_status vrs_orbit_uvw(vrs_ctx *ctx, vrs_geom *g, double *sat_pos, double *uvw);
+/* Pixel direction cosines in RING order [n_pix][3] (FP64), as the kernels use them. */
+vrs_status vrs_geom_lmn(vrs_ctx *ctx, const vrs_geom *g, double *lmn);
+
+/* ------------------------------------------------------------------ shade */
+/* Stage 2: the occultation mask M[v][p] = occ(sat_a(t), p) | occ(sat_b(t), p),
+ * never materialised in full.  Replaces ShadeMatrix + save_Shade (shade10M.cu:73-127).
+ * count/hash: [n_vis] per-visibility shaded-pixel count and XOR of pix_hash(p)
+ * over shaded RING pixels p (either may be NULL).  bits: optional full bitmap
+ * [n_vis][ceil(n_pix/32)] (bit p%32 of word p/32), only when it fits max_bits_bytes. */
+vrs_status vrs_shade(vrs_ctx *ctx, const vrs_geom *g, int32_t *count, uint32_t *hash,
+                     uint32_t *bits, vrs_shade_stats *stats);
+uint32_t vrs_pix_hash(int64_t p);
+
+/* ----------------------------------------------------------- vis-gen / recon */
+/* Stage 3, visscal (vissGen10M_shade_optim1.cu:130-206, vissGen10M_shade.cu:108-145):
+ * V[map][v] = sum_p I'[map][p] exp(-2 pi i s (u l + v m + w (n-1)))  [* exp(-2 pi i s w) if wfac]
+ * with I' = t_occ where M[v][p], else I.  I: [n_maps][n_pix] float (RING order),
+ * V: [n_maps][n_vis] complex64. */
+vrs_status vrs_visgen(vrs_ctx *ctx, const vrs_geom *g, const vrs_knobs *k, int32_t n_maps,
+                      const float *I, float *V);
+/* Stage 4, imagerecon (vissGen10M_shade_optim1.cu:210-272, vissGen10M_shade.cu:149-175):
+ * F[map][p] = kappa sum_v w[v] (1 - M[v][p]) Re(V[map][v] exp(+2 pi i s psi)),
+ * psi = u l + v m + w (n - 1 + wfac).  w: [n_vis] float or NULL (all 1; the
+ * reference's 1/count weights, vissGen10M_shade.cu:364). F: [n_maps][n_pix] float. */
+vrs_status vrs_recon(vrs_ctx *ctx, const vrs_geom *g, const vrs_knobs *k, int32_t n_maps,
+                     const float *V, const float *w, float *F);
+/* Gradient of L(I) = sum_v |V(I)[v] - V_obs[v]|^2 given resid = V(I) - V_obs
+ * (complex64 [n_vis]): grad[p] = 2 sum_v (1 - M) Re(resid[v] exp(+2 pi i s psi)).
+ * The shaded back-projection; no reference counterpart (the reference has no
+ * gradient), built on the imagerecon kernel with kappa = 2 and unit weights. */
+vrs_status vrs_adjoint(vrs_ctx *ctx, const vrs_geom *g, const vrs_knobs *k, const float *resid,
+                       float *grad);
+
+/* ------------------------------------------------------- host-side helpers */
+/* Synthetic inputs (no reference counterpart; the reference reads MATLAB files).
+ * These run on the host and need no GPU. */
+/* SplitMix64 stream; normals by Box-Muller (cos branch first). */
+vrs_status vrs_rng_normals(uint64_t seed, int64_t n, double *out);
+vrs_status vrs_rng_uniforms(uint64_t seed, int64_t n, double *out);
+/* I_p = exp(sigma z_p), z ~ N(0,1) from `seed`; then n_src point sources at
+ * distinct pixels drawn from the same stream, flux from p(S) ~ S^-2 on
+ * [s_min, s_max] by inverse CDF, added to I. */
+vrs_status vrs_sky_lognormal(uint64_t seed, double sigma, int64_t n_pix, int32_t n_src,
+                             double s_min, double s_max, float *out);
+/* RAAN spread uniformly (2 pi k / n_sat); along-track offsets d_k ~ U(0, max_offset_km)
+ * from `seed`, as mean-anomaly offsets 2 asin(d_k / (2 a)) (MIncline_generate_uvw_float.cu:51-58). */
+vrs_status vrs_orbit_offsets(int32_t n_sat, uint64_t seed, double a_km, double max_offset_km,
+                             double *raan_rad, double *m0_rad);
+/* HEALPix RING pix2vec [12 nside^2][3] (Gorski et al. 2005). */
+vrs_status vrs_healpix_lmn(int32_t nside, double *lmn);
+/* NESTED -> RING index map [12 nside^2]. */
+vrs_status vrs_healpix_nest2ring(int32_t nside, int64_t *ring_of_nest);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VRS_H */

@@ -1,0 +1,307 @@
+"""Pins the FP64 oracle (oracle/oracle.c) with closed forms, invariants,
+published test vectors and identities that a dropped term, a wrong sign,
+a swapped index or a transposed operand would break.  CPU only."""
+import math
+
+import numpy as np
+import pytest
+
+R = 1737.4
+A = R + 300.0
+MU = 4902.793455
+INCL = math.radians(30.0)
+
+
+# ----------------------------------------------------------------- RNG
+def test_splitmix64_published_vectors(oracle):
+    # SplitMix64 reference outputs (Steele, Lea, Flood 2014; Vigna's splitmix64.c)
+    assert list(oracle.splitmix64(1234567, 5)) == [
+        6457827717110365317, 3203168211198807973, 9817491932198370423,
+        4593380528125082431, 16408922859458223821]
+    assert [hex(x) for x in oracle.splitmix64(0, 3)] == [
+        "0xe220a8397b1dcdaf", "0x6e789e6aa1b965f4", "0x6c45d188009454f"]
+
+
+def test_box_muller_moments(oracle):
+    z = oracle.normals(12345, 400000)
+    assert abs(z.mean()) < 5e-3
+    assert abs(z.var() - 1.0) < 1e-2
+    assert abs((z ** 3).mean()) < 2e-2
+    assert abs((z ** 4).mean() - 3.0) < 5e-2
+    # pairs come from one (u1, u2): z0^2 + z1^2 = -2 ln(1 - u1)
+    u = oracle.splitmix64(12345, 2).astype(np.float64)
+    u1 = np.floor(u[0] / 2 ** 11) * 2.0 ** -53
+    assert abs(z[0] ** 2 + z[1] ** 2 + 2 * math.log(1 - u1)) < 1e-12
+    u2 = np.floor(u[1] / 2 ** 11) * 2.0 ** -53
+    assert abs(math.atan2(z[1], z[0]) % (2 * math.pi) - 2 * math.pi * u2) < 1e-12
+
+
+# ------------------------------------------------------------- HEALPix
+@pytest.mark.parametrize("nside", [1, 2, 4, 8, 32])
+def test_healpix_ring_matches_nested_path(oracle, nside):
+    ring = oracle.healpix_ring(nside)
+    nest = oracle.healpix_nest(nside)
+    n2r = oracle.nest2ring(nside)
+    assert sorted(n2r) == list(range(12 * nside * nside))  # bijection
+    assert np.abs(ring[n2r] - nest).max() < 1e-15           # two independent formulas agree
+
+
+@pytest.mark.parametrize("nside", [1, 4, 16, 64])
+def test_healpix_invariants(oracle, nside):
+    v = oracle.healpix_ring(nside)
+    npix = 12 * nside * nside
+    assert v.shape == (npix, 3)
+    assert np.abs(np.linalg.norm(v, axis=1) - 1).max() < 1e-15
+    assert np.abs(v.sum(0)).max() < 1e-9 * npix             # symmetric sphere
+    # ring z values: 1 - i^2/(3 nside^2) in the caps, 4/3 - 2i/(3 nside) in the belt
+    z = np.unique(np.round(v[:, 2], 12))
+    expect = [1 - i * i / (3 * nside ** 2) for i in range(1, nside)]
+    expect += [4 / 3 - 2 * i / (3 * nside) for i in range(nside, 3 * nside + 1)]
+    expect += [-1 + i * i / (3 * nside ** 2) for i in range(1, nside)]
+    assert np.allclose(np.sort(z), np.sort(np.round(expect, 12)), atol=1e-12)
+    # north/south mirror: p <-> npix-1-p mirrors z
+    assert np.abs(v[:, 2] + v[::-1, 2]).max() < 1e-15
+    # equal area: mean of z^2 tends to 1/3
+    assert abs((v[:, 2] ** 2).mean() - 1 / 3) < 0.1 / nside ** 2 + 1e-12
+
+
+def test_healpix_known_pixels(oracle):
+    v = oracle.healpix_ring(1)
+    # pixel 0 of Nside 1: theta = acos(2/3), phi = pi/4
+    assert np.allclose(v[0], [math.sqrt(5 / 9) * math.cos(math.pi / 4),
+                              math.sqrt(5 / 9) * math.sin(math.pi / 4), 2 / 3], atol=1e-15)
+    v4 = oracle.healpix_ring(4)
+    # first equatorial ring (i = nside) starts at phi = pi/(4 nside)... shifted by s = 1/2
+    p = 2 * 4 * 3  # ncap
+    assert abs(math.atan2(v4[p, 1], v4[p, 0]) - 0.5 * math.pi / 8) < 1e-14
+
+
+# ------------------------------------------------------------- orbits
+def test_orbit_closed_forms(oracle):
+    n = 4
+    raan = np.arange(n) * 2 * math.pi / n
+    m0 = np.array([0.0, 0.1, 0.2, 0.3])
+    period = 2 * math.pi * math.sqrt(A ** 3 / MU)
+    assert abs(period - 8252.3) < 0.1
+    pos = oracle.orbit_positions(2, period, A, INCL, MU, raan, m0)
+    # |r| = a, one full period returns to the start
+    assert np.abs(np.linalg.norm(pos, axis=2) - A).max() < 1e-9
+    assert np.abs(pos[:, 1] - pos[:, 0]).max() < 1e-8
+    # t = 0, m0 = 0: satellite at the ascending node a (cos O, sin O, 0)
+    assert np.allclose(pos[0, 0], [A, 0, 0], atol=1e-12)
+    # orbit plane normal (sin O sin i, -cos O sin i, cos i) is orthogonal to r
+    pos = oracle.orbit_positions(50, 97.0, A, INCL, MU, raan, m0)
+    for k in range(n):
+        h = np.array([math.sin(raan[k]) * math.sin(INCL), -math.cos(raan[k]) * math.sin(INCL),
+                      math.cos(INCL)])
+        assert np.abs(pos[k] @ h).max() < 1e-9
+        assert np.abs(pos[k][:, 2]).max() <= A * math.sin(INCL) + 1e-9
+    # mean motion: angle swept in dt is sqrt(mu/a^3) dt
+    c = (pos[0, 1] @ pos[0, 0]) / A ** 2
+    assert abs(math.acos(c) - math.sqrt(MU / A ** 3) * 97.0) < 1e-12
+
+
+def test_uvw_difference_and_antisymmetry(oracle):
+    raan = np.array([0.0, 2.0, 4.0])
+    pos = oracle.orbit_positions(10, 5.0, A, INCL, MU, raan, np.zeros(3))
+    lam = 299792458.0 / 1e7
+    uvw = oracle.uvw(pos, [(0, 1), (1, 0), (0, 2)], lam)
+    assert np.array_equal(uvw[0], -uvw[1])
+    assert np.allclose(uvw[2], (pos[2] - pos[0]) * 1e3 / lam, rtol=1e-15, atol=0)
+
+
+# ---------------------------------------------------------------- mask
+def test_mask_closed_form_axis(oracle):
+    # satellite on +z at |r| = A: shaded iff s_z <= -sqrt(1 - R^2/A^2)
+    r = np.array([0.0, 0.0, A])
+    c = math.sqrt(1 - (R / A) ** 2)
+    for sz in [-1.0, -0.9, -c - 1e-9, -c + 1e-9, -0.3, 0.0, 0.7, 1.0]:
+        s = np.array([math.sqrt(1 - sz * sz), 0.0, sz])
+        m, _ = oracle.occ_one(r, s, R)
+        assert m == (sz <= -c), sz
+    # exactly tangent ray: from (-3, 4, 0) along +x grazes the R = 4 sphere at (0, 4, 0);
+    # the closed sphere counts it as a hit and it lies on the limb
+    m, near = oracle.occ_one(np.array([-3.0, 4.0, 0.0]), np.array([1.0, 0, 0]), 4.0)
+    assert m and near
+    # just outside and just inside the limb
+    assert not oracle.occ_one(np.array([-3.0, 4.0 + 1e-9, 0.0]), np.array([1.0, 0, 0]), 4.0)[0]
+    assert oracle.occ_one(np.array([-3.0, 4.0 - 1e-9, 0.0]), np.array([1.0, 0, 0]), 4.0)[0]
+    # pointing away from the sphere is never shaded
+    assert not oracle.occ_one(np.array([-3.0, 4.0, 0.0]), np.array([-1.0, 0, 0]), 4.0)[0]
+
+
+def test_mask_cap_area(oracle):
+    nside = 64
+    lmn = oracle.healpix_ring(nside)
+    pos = np.array([[[0.0, 0.0, A]], [[A, 0.0, 0.0]]])  # two satellites, one step
+    cnt, h, near = oracle.mask_stats(pos, np.array([[0, 1]], np.int32), lmn, R)
+    c = math.sqrt(1 - (R / A) ** 2)
+    frac1 = (1 - c) / 2
+    # two caps, centres 90 deg apart with half-angle 58.5 deg: union by direct count
+    sa = (lmn[:, 2] <= -c)
+    sb = (lmn[:, 0] <= -c)
+    assert cnt[0] == np.count_nonzero(sa | sb)
+    assert abs(sa.mean() - frac1) < 3e-3
+    # hash = XOR of pix_hash over shaded pixels
+    hh = 0
+    for p in np.nonzero(sa | sb)[0]:
+        hh ^= oracle.pix_hash(p)
+    assert h[0] == hh
+    bits = oracle.mask_bits(pos, np.array([[0, 1]], np.int32), lmn, R)
+    unpacked = np.unpackbits(bits.view(np.uint8), bitorder="little")[:len(lmn)]
+    assert np.array_equal(unpacked.astype(bool), sa | sb)
+
+
+# ------------------------------------------------------------- vis-gen
+def _tiny_geom(oracle, n_t=6, freq=3e6):
+    raan = np.array([0.0, 2.1, 4.2])
+    m0 = np.array([0.0, 0.01, 0.02])
+    pos = oracle.orbit_positions(n_t, 40.0, A, INCL, MU, raan, m0)
+    pairs = np.array([(0, 1), (0, 2), (1, 2)], np.int32)
+    uvw = oracle.uvw(pos, pairs, 299792458.0 / freq)
+    return pos, pairs, uvw
+
+
+def test_visgen_pole_source(oracle):
+    # a source at s = +z has phi = w (1 - 1) = 0 for every baseline: V = I
+    pos, pairs, uvw = _tiny_geom(oracle)
+    lmn = np.array([[0.0, 0.0, 1.0]])
+    V = oracle.visgen(pos, pairs, uvw, lmn, np.array([3.5], np.float32), R, t_occ=0.0)
+    cnt, _, _ = oracle.mask_stats(pos, pairs, lmn, R)
+    expect = np.where(cnt == 0, 3.5, 0.0)
+    assert np.abs(V[0] - expect).max() < 1e-13
+    # with the trailing factor the net phase is w n: V = I exp(-2 pi i s w)
+    V2 = oracle.visgen(pos, pairs, uvw, lmn, np.array([3.5], np.float32), R, s=0.5, wfac=1,
+                       t_occ=0.0)
+    w = uvw.reshape(-1, 3)[:, 2]
+    assert np.abs(V2[0] - expect * np.exp(-2j * math.pi * 0.5 * w)).max() < 1e-9
+
+
+def test_visgen_hermitian_and_linear(oracle):
+    pos, pairs, uvw = _tiny_geom(oracle)
+    lmn = oracle.healpix_ring(4)
+    rng = np.random.default_rng(0)
+    # dyadic intensities keep 2 I1 - 3 I2 exact in float32
+    I1 = (rng.integers(0, 1024, len(lmn)) / 256).astype(np.float32)
+    I2 = (rng.integers(0, 1024, len(lmn)) / 256).astype(np.float32)
+    V1 = oracle.visgen(pos, pairs, uvw, lmn, I1, R)
+    # swapping a baseline's satellites negates uvw; the mask is symmetric: V -> conj V
+    rp = pairs[:, ::-1].copy()
+    Vr = oracle.visgen(pos, rp, oracle.uvw(pos, rp, 299792458.0 / 3e6), lmn, I1, R)
+    assert np.abs(Vr - np.conj(V1)).max() < 1e-9 * np.abs(V1).max()
+    # linear in I once shading is off (t_occ = 0 makes shaded pixels drop out)
+    a = oracle.visgen(pos, pairs, uvw, lmn, np.stack([I1, I2, 2 * I1 - 3 * I2]), R, t_occ=0.0)
+    assert np.abs(a[2] - (2 * a[0] - 3 * a[1])).max() < 1e-10 * np.abs(a).max()
+
+
+def test_visgen_rotation_invariance(oracle):
+    # rotating the sky, the satellites and hence uvw about z leaves V unchanged
+    # (catches transposed or swapped l, m, n / u, v, w operands)
+    pos, pairs, uvw = _tiny_geom(oracle)
+    lmn = oracle.healpix_ring(4)
+    I = np.random.default_rng(1).random(len(lmn)).astype(np.float32)
+    g = 0.7
+    Rz = np.array([[math.cos(g), -math.sin(g), 0], [math.sin(g), math.cos(g), 0], [0, 0, 1]])
+    V0 = oracle.visgen(pos, pairs, uvw, lmn, I, R)
+    V1 = oracle.visgen(pos @ Rz.T, pairs, uvw @ Rz.T, lmn @ Rz.T, I, R)
+    assert np.abs(V1 - V0).max() < 1e-9 * np.abs(V0).max()
+    # a rotation about x is different (not a symmetry of the w (n - 1) term)
+    Rx = np.array([[1, 0, 0], [0, math.cos(g), -math.sin(g)], [0, math.sin(g), math.cos(g)]])
+    V2 = oracle.visgen(pos @ Rx.T, pairs, uvw @ Rx.T, lmn @ Rx.T, I, R)
+    assert np.abs(V2 - V0).max() > 1e-3 * np.abs(V0).max()
+
+
+def test_visgen_brute_force_numpy(oracle):
+    pos, pairs, uvw = _tiny_geom(oracle, n_t=4)
+    lmn = oracle.healpix_ring(2)
+    I = np.random.default_rng(2).random(len(lmn)).astype(np.float32)
+    V = oracle.visgen(pos, pairs, uvw, lmn, I, R, s=0.5, t_occ=240.0)
+    flat = uvw.reshape(-1, 3)
+    for v in range(len(flat)):
+        b, t = divmod(v, pos.shape[1])
+        acc = 0j
+        for p in range(len(lmn)):
+            sh = any(oracle.occ_one(pos[k, t], lmn[p], R)[0] for k in pairs[b])
+            ip = 240.0 if sh else float(I[p])
+            ph = 0.5 * (flat[v] @ (lmn[p] - [0, 0, 1]))
+            acc += ip * np.exp(-2j * math.pi * ph)
+        assert abs(acc - V[0, v]) < 1e-9 * abs(acc) + 1e-9
+
+
+# --------------------------------------------------------------- recon
+def test_recon_is_adjoint_of_visgen(oracle):
+    # <A I, y> = <I, A^H y> with A the shaded forward operator (t_occ = 0)
+    pos, pairs, uvw = _tiny_geom(oracle)
+    lmn = oracle.healpix_ring(4)
+    rng = np.random.default_rng(3)
+    I = rng.random(len(lmn)).astype(np.float32)
+    n_vis = len(pairs) * pos.shape[1]
+    y = rng.standard_normal(n_vis) + 1j * rng.standard_normal(n_vis)
+    for s, wfac in [(1.0, 0), (0.5, 1)]:
+        AI = oracle.visgen(pos, pairs, uvw, lmn, I, R, s=s, wfac=wfac, t_occ=0.0)[0]
+        AHy = oracle.recon(pos, pairs, uvw, lmn, y, R, s=s, wfac=wfac, kappa=1.0)[0]
+        lhs = np.real(np.vdot(AI, y))
+        rhs = float(I.astype(np.float64) @ AHy)
+        assert abs(lhs - rhs) < 1e-10 * abs(lhs)
+
+
+def test_recon_weights_kappa_and_selection(oracle):
+    pos, pairs, uvw = _tiny_geom(oracle)
+    lmn = oracle.healpix_ring(4)
+    rng = np.random.default_rng(4)
+    n_vis = len(pairs) * pos.shape[1]
+    V = rng.standard_normal(n_vis) + 1j * rng.standard_normal(n_vis)
+    w = rng.random(n_vis).astype(np.float32)
+    F = oracle.recon(pos, pairs, uvw, lmn, V, R)                     # kappa = 1/N_vis
+    F1 = oracle.recon(pos, pairs, uvw, lmn, V, R, kappa=1.0)
+    assert np.allclose(F * n_vis, F1, rtol=1e-12)
+    Fw = oracle.recon(pos, pairs, uvw, lmn, V * w, R)
+    Fw2 = oracle.recon(pos, pairs, uvw, lmn, V, R, weights=w)
+    assert np.allclose(Fw, Fw2, rtol=1e-12, atol=1e-15)
+    sel = np.array([5, 0, 100], np.int64)
+    assert np.allclose(oracle.recon(pos, pairs, uvw, lmn, V, R, pix_sel=sel)[0], F[0, sel])
+
+
+def test_recon_single_fringe(oracle):
+    # one unshaded visibility V = 1 at uvw: F_p = kappa cos(2 pi s psi_p)
+    pos = np.array([[[0, 0, A]], [[0, 0, A]]], np.float64)  # both above the north pole
+    pos[1, 0, 0] += 10.0
+    pairs = np.array([[0, 1]], np.int32)
+    uvw = np.array([[[12.25, -3.5, 2.0]]])
+    lmn = oracle.healpix_ring(2)
+    F = oracle.recon(pos, pairs, uvw, lmn, np.array([1.0 + 0j]), R, kappa=1.0)[0]
+    cnt_mask = np.array([oracle.occ_one(pos[0, 0], s, R)[0] or oracle.occ_one(pos[1, 0], s, R)[0]
+                         for s in lmn])
+    psi = lmn @ uvw[0, 0] - uvw[0, 0, 2]
+    expect = np.where(cnt_mask, 0.0, np.cos(2 * math.pi * psi))
+    assert np.abs(F - expect).max() < 1e-12
+
+
+# -------------------------------------------------- FP32 baseline error model
+def test_cpu_f32_baseline_tracks_oracle(oracle):
+    pos, pairs, uvw = _tiny_geom(oracle, n_t=8, freq=1e6)
+    lmn = oracle.healpix_ring(8)
+    I = np.exp(np.random.default_rng(5).standard_normal(len(lmn))).astype(np.float32)
+    V = oracle.visgen(pos, pairs, uvw, lmn, I, R)[0]
+    Vf = oracle.cpu_visgen_f32(pos, pairs, uvw, lmn, I, R)
+    err = np.linalg.norm(Vf - V) / np.linalg.norm(V)
+    assert err < 2e-2, err
+    F = oracle.recon(pos, pairs, uvw, lmn, V, R)[0]
+    Ff = oracle.cpu_recon_f32(pos, pairs, uvw, lmn, V, R)
+    assert np.linalg.norm(Ff - F) / np.linalg.norm(F) < 2e-2
+
+
+def test_shade_box_reference_formula(oracle):
+    # baseline along +x: theta = atan2(0, u) = 0 -> antipode index RES/2 in theta,
+    # phi = asin(0) -> RES/2 in phi; half-widths from r0 = atan(R/(|uvw| + R))
+    res = 2094
+    u = np.array([5000.0], np.float32)
+    z = np.zeros(1, np.float32)
+    out = oracle.shade_box(u, z, z, res)[0]
+    dth = 2 * math.pi / res
+    r0 = math.atan(1737.74 / (5000 + 1737.74))
+    tR = round(r0 / dth)
+    pR = tR // 2
+    ti = round(math.pi / dth)
+    pi_ = round((math.pi / 2) / (math.pi / res))
+    assert list(out) == [ti - tR, ti + tR, pi_ - pR, pi_ + pR]

@@ -1,0 +1,243 @@
+"""Python mirror of the hot-path interface over the C-ABI.
+
+Names follow the reference's stages: MIncline (orbits and uvw,
+MIncline_generate_uvw_float.cu:184), ShadeMatrix (the occultation mask,
+shade10M.cu:73), visscal (vis-gen, vissGen10M_shade.cu:108) and imagerecon
+(recon, vissGen10M_shade.cu:149), as methods of Geometry.  Arrays may be
+numpy arrays (host) or torch CUDA tensors (device, used in place on torch's
+current stream); outputs follow the input's kind.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from . import workload as W
+from ._lib import Knobs, VrsError  # noqa: F401  (re-exported)
+
+try:
+    import torch
+except ImportError:  # torch is optional for host-only use
+    torch = None
+
+
+def knobs(preset="canonical", **over):
+    """Scalar knobs: 'canonical' (s = 1, n - 1, T_occ = 240, kappa = 1/N_vis)
+    or 'reference' (s = 0.5, trailing exp(-2 pi i s w), T_occ = 240)."""
+    lib = _lib.load()
+    k = Knobs()
+    (lib.vrs_knobs_reference if preset == "reference" else lib.vrs_knobs_default)(C.byref(k))
+    for name, val in over.items():
+        if name == "precision" and isinstance(val, str):
+            val = {"plain": 0, "f32": 0, "mixed": 1, "f64phase": 1, "mfma": 2}[val]
+        setattr(k, name, val)
+    return k
+
+
+def _is_torch(x):
+    return torch is not None and isinstance(x, torch.Tensor)
+
+
+def _ptr(x):
+    if _is_torch(x):
+        if not x.is_contiguous():
+            raise ValueError("tensor must be contiguous")
+        return C.c_void_p(x.data_ptr())
+    if not x.flags["C_CONTIGUOUS"]:
+        raise ValueError("array must be C-contiguous")
+    return C.c_void_p(x.ctypes.data)
+
+
+def _empty_like_kind(ref, shape, dtype):
+    if _is_torch(ref):
+        tdt = {np.float32: torch.float32, np.float64: torch.float64, np.int32: torch.int32,
+               np.uint32: torch.int32}[dtype]
+        return torch.empty(shape, dtype=tdt, device=ref.device)
+    return np.empty(shape, dtype)
+
+
+class Context:
+    """One device + one stream (vrs_ctx)."""
+
+    def __init__(self, device=0):
+        self._lib = _lib.load()
+        h = C.c_void_p()
+        _lib.check(self._lib.vrs_create(device, C.byref(h)))
+        self.handle = h
+        self.device = device
+        self._stream = None
+
+    def _sync_stream(self, *arrays):
+        """Run on torch's current stream when any argument is a torch tensor."""
+        if any(_is_torch(a) for a in arrays if a is not None):
+            s = torch.cuda.current_stream(self.device).cuda_stream
+            if s != self._stream:
+                self.check(self._lib.vrs_set_stream(self.handle, C.c_void_p(s)))
+                self._stream = s
+
+    def check(self, status):
+        _lib.check(status, self.handle)
+
+    def synchronize(self):
+        self.check(self._lib.vrs_synchronize(self.handle))
+
+    def timing(self, on=True):
+        self.check(self._lib.vrs_timing_enable(self.handle, 1 if on else 0))
+
+    def last_ms(self, kernel):
+        return self._lib.vrs_timing_last_ms(self.handle, kernel.encode())
+
+    def geometry(self, n_sat, n_t, freq_hz, nside, raan=None, m0=None, dt_s=W.DT_S,
+                 a_km=W.A_KM, incl_rad=W.INCL_RAD, mu=W.MU_MOON, r_moon_km=W.R_MOON_KM,
+                 pairs=None):
+        return Geometry(self, n_sat, n_t, freq_hz, nside, raan, m0, dt_s, a_km, incl_rad, mu,
+                        r_moon_km, pairs)
+
+    def workload_geometry(self, wl):
+        return self.geometry(wl.n_sat, wl.n_t, wl.freq_hz, wl.nside)
+
+    def close(self):
+        if self.handle:
+            self._lib.vrs_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Geometry:
+    """Orbits, uvw, shade caps and the chunked HEALPix sky on one device (vrs_geom)."""
+
+    def __init__(self, ctx, n_sat, n_t, freq_hz, nside, raan, m0, dt_s, a_km, incl_rad, mu,
+                 r_moon_km, pairs):
+        self.ctx = ctx
+        lib = ctx._lib
+        if raan is None or m0 is None:
+            r0, mm = W.orbit_params(n_sat, a_km=a_km)
+            raan = r0 if raan is None else raan
+            m0 = mm if m0 is None else m0
+        self.raan = np.ascontiguousarray(raan, np.float64)
+        self.m0 = np.ascontiguousarray(m0, np.float64)
+        d = _lib.GeomDesc()
+        d.orbit.n_sat = n_sat
+        d.orbit.n_t = n_t
+        d.orbit.dt_s = dt_s
+        d.orbit.a_km = a_km
+        d.orbit.incl_rad = incl_rad
+        d.orbit.mu_km3_s2 = mu
+        d.orbit.raan_rad = self.raan.ctypes.data_as(C.POINTER(C.c_double))
+        d.orbit.m0_rad = self.m0.ctypes.data_as(C.POINTER(C.c_double))
+        d.freq_hz = freq_hz
+        d.r_moon_km = r_moon_km
+        if pairs is not None:
+            self._pairs = np.ascontiguousarray(pairs, np.int32)
+            d.n_base = len(self._pairs)
+            d.pairs = self._pairs.ctypes.data_as(C.POINTER(C.c_int32))
+        else:
+            self._pairs = W.all_pairs(n_sat)
+            d.n_base = 0
+        d.nside = nside
+        h = C.c_void_p()
+        ctx.check(lib.vrs_geom_create(ctx.handle, C.byref(d), C.byref(h)))
+        self.handle = h
+        info = _lib.GeomInfo()
+        ctx.check(lib.vrs_geom_info_get(h, C.byref(info)))
+        self.n_vis, self.n_pix = info.n_vis, info.n_pix
+        self.n_sat, self.n_t, self.n_base = info.n_sat, info.n_t, info.n_base
+        self.nside, self.n_chunk, self.lambda_m = info.nside, info.n_chunk, info.lambda_m
+        self.r_moon_km = r_moon_km
+        self.pairs = self._pairs
+
+    # ------------------------------------------------------------- stage 1
+    def MIncline(self, export=True):
+        """Re-run orbit propagation + uvw on the device; returns FP64
+        (positions [n_sat][n_t][3] km, uvw [n_base][n_t][3] wavelengths)."""
+        lib = self.ctx._lib
+        if not export:
+            self.ctx.check(lib.vrs_orbit_uvw(self.ctx.handle, self.handle, None, None))
+            return None
+        pos = np.empty((self.n_sat, self.n_t, 3), np.float64)
+        uvw = np.empty((self.n_base, self.n_t, 3), np.float64)
+        self.ctx.check(lib.vrs_orbit_uvw(self.ctx.handle, self.handle, _ptr(pos), _ptr(uvw)))
+        return pos, uvw
+
+    orbit_uvw = MIncline
+
+    def lmn(self):
+        out = np.empty((self.n_pix, 3), np.float64)
+        self.ctx.check(self.ctx._lib.vrs_geom_lmn(self.ctx.handle, self.handle, _ptr(out)))
+        return out
+
+    # ------------------------------------------------------------- stage 2
+    def ShadeMatrix(self, bits=False):
+        """Per-visibility shaded-pixel counts and hashes (and optionally the bitmap)."""
+        lib = self.ctx._lib
+        cnt = np.empty(self.n_vis, np.int32)
+        h = np.empty(self.n_vis, np.uint32)
+        b = np.empty((self.n_vis, (self.n_pix + 31) // 32), np.uint32) if bits else None
+        st = _lib.ShadeStats()
+        self.ctx.check(lib.vrs_shade(self.ctx.handle, self.handle, _ptr(cnt), _ptr(h),
+                                     _ptr(b) if b is not None else None, C.byref(st)))
+        out = {"count": cnt, "hash": h, "shaded_pairs": st.shaded_pairs,
+               "near_limb": st.near_limb, "fp64_tests": st.fp64_tests}
+        if bits:
+            out["bits"] = b
+        return out
+
+    shade = ShadeMatrix
+
+    # ------------------------------------------------------------- stage 3
+    def visscal(self, I, k=None):
+        """V[map][v] (complex64 as [..., 2] float32) from I[map][pix] float32."""
+        k = k or knobs()
+        n_maps = 1 if I.ndim == 1 else I.shape[0]
+        if I.shape[-1] != self.n_pix:
+            raise ValueError(f"I has {I.shape[-1]} pixels, geometry has {self.n_pix}")
+        V = _empty_like_kind(I, (n_maps, self.n_vis, 2), np.float32)
+        self.ctx._sync_stream(I)
+        self.ctx.check(self.ctx._lib.vrs_visgen(self.ctx.handle, self.handle, C.byref(k), n_maps,
+                                                _ptr(I), _ptr(V)))
+        return V
+
+    visgen = visscal
+
+    # ------------------------------------------------------------- stage 4
+    def imagerecon(self, V, w=None, k=None):
+        """F[map][pix] float32 from V[map][v] (complex64 as [..., 2] float32)."""
+        k = k or knobs()
+        n_maps = 1 if V.ndim == 2 else V.shape[0]
+        if V.shape[-2] != self.n_vis or V.shape[-1] != 2:
+            raise ValueError(f"V must be [maps][{self.n_vis}][2]")
+        F = _empty_like_kind(V, (n_maps, self.n_pix), np.float32)
+        self.ctx._sync_stream(V, w)
+        self.ctx.check(self.ctx._lib.vrs_recon(self.ctx.handle, self.handle, C.byref(k), n_maps,
+                                               _ptr(V), _ptr(w) if w is not None else None,
+                                               _ptr(F)))
+        return F
+
+    recon = imagerecon
+
+    def adjoint(self, resid, k=None):
+        """Gradient of sum |V(I) - V_obs|^2 given resid = V(I) - V_obs ([n_vis][2])."""
+        k = k or knobs()
+        if resid.shape[-2:] != (self.n_vis, 2):
+            raise ValueError(f"resid must be [{self.n_vis}][2]")
+        g = _empty_like_kind(resid, (self.n_pix,), np.float32)
+        self.ctx._sync_stream(resid)
+        self.ctx.check(self.ctx._lib.vrs_adjoint(self.ctx.handle, self.handle, C.byref(k),
+                                                 _ptr(resid), _ptr(g)))
+        return g
+
+    def close(self):
+        if self.handle:
+            self.ctx._lib.vrs_geom_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,765 @@
+// C-ABI of the hot path: argument validation, device memory, dispatch.
+// Every argument is validated on the host before any HIP call; errors are
+// returned as vrs_status with a message (the reference prints and exit(1)s,
+// error.cuh:4-18, or prints and continues, vissGen10M_shade_optim2.cu:208-211).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "host_sky.hpp"
+#include "kernels.hpp"
+#include "vrs.h"
+#include "vrs_internal.hpp"
+
+using namespace vrs;
+
+namespace {
+thread_local std::string g_thread_err;
+constexpr double kC = 299792458.0;
+}  // namespace
+
+struct vrs_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    int n_cu = 0;
+    std::string err;
+    bool timing = false;
+    std::map<std::string, std::pair<hipEvent_t, hipEvent_t>> events;
+    // grow-only device workspaces
+    void *ws[4] = {nullptr, nullptr, nullptr, nullptr};
+    size_t ws_bytes[4] = {0, 0, 0, 0};
+    unsigned long long *d_tot = nullptr;
+};
+
+struct vrs_geom {
+    vrs_ctx *ctx = nullptr;
+    int32_t n_sat = 0, n_t = 0, n_base = 0, nside = 0, n_chunk = 0;
+    int64_t n_vis = 0, n_pix = 0, n_pad = 0;
+    double lambda_m = 0, r_moon = 0;
+    OrbitArgs orb{};
+    std::vector<int32_t> pairs;
+    std::vector<double> lmn_ring;  // [n_pix][3] host copy, RING order
+    // device
+    double *d_orbit = nullptr;  // raan[n_sat], m0[n_sat]
+    double4 *d_pos = nullptr;
+    CapRec *d_cap = nullptr;
+    double4 *d_uvw64 = nullptr;
+    float4 *d_uvw32 = nullptr;
+    int2 *d_pairs = nullptr;
+    float4 *d_pix32 = nullptr;
+    double4 *d_pix64 = nullptr;
+    int32_t *d_ring = nullptr;
+    ChunkRec *d_chunk = nullptr;
+    GeomDev dev() const {
+        GeomDev g;
+        g.n_sat = n_sat; g.n_t = n_t; g.n_base = n_base; g.n_chunk = n_chunk;
+        g.n_vis = n_vis; g.n_pad = n_pad;
+        g.pos = d_pos; g.cap = d_cap; g.uvw64 = d_uvw64; g.uvw32 = d_uvw32; g.pairs = d_pairs;
+        g.pix32 = d_pix32; g.pix64 = d_pix64; g.ring = d_ring; g.chunk = d_chunk;
+        const double e = kNearLimbRel * r_moon;
+        g.limb.R2 = r_moon * r_moon;
+        g.limb.lo2 = (r_moon - e) * (r_moon - e);
+        g.limb.hi2 = (r_moon + e) * (r_moon + e);
+        return g;
+    }
+};
+
+namespace {
+
+vrs_status fail(vrs_ctx *ctx, vrs_status st, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (ctx) ctx->err = buf;
+    g_thread_err = buf;
+    return st;
+}
+
+#define VRS_HIP(ctx, call)                                                                  \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return fail((ctx), e_ == hipErrorOutOfMemory ? VRS_E_OOM : VRS_E_HIP,           \
+                        "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__,    \
+                        __LINE__);                                                          \
+    } while (0)
+
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev) {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        if (prev != dev) (void)hipSetDevice(dev);
+    }
+    ~DeviceGuard() {
+        int cur;
+        if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+    }
+};
+
+bool is_device_ptr(const void *p) {
+    hipPointerAttribute_t attr;
+    hipError_t e = hipPointerGetAttributes(&attr, p);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged ||
+           attr.type == hipMemoryTypeUnified;
+}
+
+vrs_status ws_get(vrs_ctx *ctx, int slot, size_t bytes, void **out) {
+    if (ctx->ws_bytes[slot] < bytes) {
+        if (ctx->ws[slot]) {
+            VRS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            VRS_HIP(ctx, hipFree(ctx->ws[slot]));
+            ctx->ws[slot] = nullptr;
+            ctx->ws_bytes[slot] = 0;
+        }
+        size_t b = std::max(bytes, (size_t)256);
+        VRS_HIP(ctx, hipMalloc(&ctx->ws[slot], b));
+        ctx->ws_bytes[slot] = b;
+    }
+    *out = ctx->ws[slot];
+    return VRS_OK;
+}
+
+// A caller array that may live on the host: used in place when it is device
+// memory, otherwise staged through a private device buffer.
+struct Staged {
+    vrs_ctx *ctx;
+    const void *user;
+    void *dev = nullptr;
+    size_t bytes;
+    bool host = false;
+    bool out;
+    Staged(vrs_ctx *c, const void *u, size_t b, bool is_out) : ctx(c), user(u), bytes(b), out(is_out) {}
+    vrs_status prepare() {
+        if (!user) return VRS_OK;
+        if (is_device_ptr(user)) {
+            dev = const_cast<void *>(user);
+            return VRS_OK;
+        }
+        host = true;
+        VRS_HIP(ctx, hipMalloc(&dev, std::max(bytes, (size_t)4)));
+        if (!out) VRS_HIP(ctx, hipMemcpyAsync(dev, user, bytes, hipMemcpyHostToDevice, ctx->stream));
+        return VRS_OK;
+    }
+    vrs_status finish() {
+        if (!host || !dev) return VRS_OK;
+        if (out)
+            VRS_HIP(ctx, hipMemcpyAsync(const_cast<void *>(user), dev, bytes,
+                                        hipMemcpyDeviceToHost, ctx->stream));
+        return VRS_OK;
+    }
+    ~Staged() {
+        if (host && dev) {
+            (void)hipStreamSynchronize(ctx->stream);
+            (void)hipFree(dev);
+        }
+    }
+};
+
+struct Timer {
+    vrs_ctx *ctx;
+    std::pair<hipEvent_t, hipEvent_t> *ev = nullptr;
+    Timer(vrs_ctx *c, const char *name) : ctx(c) {
+        if (!ctx->timing) return;
+        auto it = ctx->events.find(name);
+        if (it == ctx->events.end()) {
+            hipEvent_t a, b;
+            if (hipEventCreate(&a) != hipSuccess) return;
+            if (hipEventCreate(&b) != hipSuccess) { (void)hipEventDestroy(a); return; }
+            it = ctx->events.emplace(name, std::make_pair(a, b)).first;
+        }
+        ev = &it->second;
+        (void)hipEventRecord(ev->first, ctx->stream);
+    }
+    void stop() {
+        if (ev) (void)hipEventRecord(ev->second, ctx->stream);
+        ev = nullptr;
+    }
+    ~Timer() { stop(); }
+};
+
+int persistent_grid(vrs_ctx *ctx, const void *kernel, int64_t n_items) {
+    int per_cu = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, 0) != hipSuccess ||
+        per_cu < 1)
+        per_cu = 1;
+    int64_t cap = (int64_t)per_cu * ctx->n_cu;
+    int64_t grid = std::min<int64_t>(n_items, cap);
+    if (grid >= 8) grid -= grid % 8;  // whole rounds over the 8 XCDs
+    return (int)std::max<int64_t>(grid, 1);
+}
+
+vrs_status check_ctx(vrs_ctx *ctx) {
+    if (!ctx) return fail(nullptr, VRS_E_ARG, "ctx is NULL");
+    return VRS_OK;
+}
+
+vrs_status check_knobs(vrs_ctx *ctx, const vrs_knobs *k) {
+    if (!k) return fail(ctx, VRS_E_ARG, "knobs is NULL");
+    if (!std::isfinite(k->phase_scale) || k->phase_scale == 0.0)
+        return fail(ctx, VRS_E_ARG, "phase_scale must be finite and non-zero (got %g)", k->phase_scale);
+    if (k->wfac != 0 && k->wfac != 1) return fail(ctx, VRS_E_ARG, "wfac must be 0 or 1");
+    if (!std::isfinite(k->t_occ)) return fail(ctx, VRS_E_ARG, "t_occ must be finite");
+    if (!std::isfinite(k->kappa) || k->kappa < 0.0)
+        return fail(ctx, VRS_E_ARG, "kappa must be finite and >= 0 (0 selects 1/N_vis)");
+    if (k->precision < VRS_PLAIN_F32 || k->precision > VRS_MFMA_SPLIT16)
+        return fail(ctx, VRS_E_ARG, "unknown precision %d", k->precision);
+    return VRS_OK;
+}
+
+PhaseKnobs phase_knobs(const vrs_knobs *k) {
+    PhaseKnobs p;
+    p.s = k->phase_scale;
+    p.wfac = k->wfac;
+    p.t_occ = k->t_occ;
+    return p;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *vrs_version(void) { return "vrs 0.1.0 (gfx950)"; }
+
+void vrs_knobs_default(vrs_knobs *k) {
+    if (!k) return;
+    k->phase_scale = 1.0;
+    k->wfac = 0;
+    k->t_occ = 240.0f;
+    k->kappa = 0.0;
+    k->precision = VRS_PLAIN_F32;
+}
+
+void vrs_knobs_reference(vrs_knobs *k) {
+    if (!k) return;
+    k->phase_scale = 0.5;  // dl = 2*RES/(RES-1) == 2 in integer arithmetic
+    k->wfac = 1;
+    k->t_occ = 240.0f;
+    k->kappa = 0.0;
+    k->precision = VRS_PLAIN_F32;
+}
+
+int vrs_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return n;
+}
+
+const char *vrs_last_error(const vrs_ctx *ctx) {
+    return ctx ? ctx->err.c_str() : g_thread_err.c_str();
+}
+
+vrs_status vrs_create(int device, vrs_ctx **out) {
+    if (!out) return fail(nullptr, VRS_E_ARG, "out is NULL");
+    *out = nullptr;
+    int n = vrs_device_count();
+    if (n <= 0) return fail(nullptr, VRS_E_NODEV, "no HIP device available");
+    if (device < 0 || device >= n)
+        return fail(nullptr, VRS_E_ARG, "device %d out of range [0, %d)", device, n);
+    DeviceGuard dg(device);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess)
+        return fail(nullptr, VRS_E_HIP, "hipGetDeviceProperties failed");
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(nullptr, VRS_E_NODEV, "device %d is %s; this library is built for gfx950",
+                    device, prop.gcnArchName);
+    vrs_ctx *c = new vrs_ctx;
+    c->device = device;
+    c->n_cu = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return fail(nullptr, VRS_E_HIP, "hipStreamCreate failed");
+    }
+    c->own_stream = true;
+    if (hipMalloc(&c->d_tot, 4 * sizeof(unsigned long long)) != hipSuccess) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return fail(nullptr, VRS_E_OOM, "hipMalloc of counters failed");
+    }
+    *out = c;
+    return VRS_OK;
+}
+
+void vrs_destroy(vrs_ctx *ctx) {
+    if (!ctx) return;
+    DeviceGuard dg(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto &kv : ctx->events) {
+        (void)hipEventDestroy(kv.second.first);
+        (void)hipEventDestroy(kv.second.second);
+    }
+    for (int i = 0; i < 4; ++i)
+        if (ctx->ws[i]) (void)hipFree(ctx->ws[i]);
+    if (ctx->d_tot) (void)hipFree(ctx->d_tot);
+    if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+vrs_status vrs_set_stream(vrs_ctx *ctx, void *stream) {
+    if (vrs_status s = check_ctx(ctx)) return s;
+    DeviceGuard dg(ctx->device);
+    VRS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (ctx->own_stream) {
+        VRS_HIP(ctx, hipStreamDestroy(ctx->stream));
+        ctx->own_stream = false;
+    }
+    // NULL selects the device's null stream (the legacy default stream)
+    ctx->stream = (hipStream_t)stream;
+    return VRS_OK;
+}
+
+vrs_status vrs_synchronize(vrs_ctx *ctx) {
+    if (vrs_status s = check_ctx(ctx)) return s;
+    DeviceGuard dg(ctx->device);
+    VRS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return VRS_OK;
+}
+
+vrs_status vrs_timing_enable(vrs_ctx *ctx, int on) {
+    if (vrs_status s = check_ctx(ctx)) return s;
+    ctx->timing = on != 0;
+    return VRS_OK;
+}
+
+double vrs_timing_last_ms(vrs_ctx *ctx, const char *kernel) {
+    if (!ctx || !kernel) return -1.0;
+    auto it = ctx->events.find(kernel);
+    if (it == ctx->events.end()) return -1.0;
+    DeviceGuard dg(ctx->device);
+    if (hipEventSynchronize(it->second.second) != hipSuccess) return -1.0;
+    float ms = -1.0f;
+    if (hipEventElapsedTime(&ms, it->second.first, it->second.second) != hipSuccess) return -1.0;
+    return ms;
+}
+
+uint32_t vrs_pix_hash(int64_t p) { return pix_hash_u64((uint64_t)p); }
+
+// ---------------------------------------------------------------- host helpers
+vrs_status vrs_rng_normals(uint64_t seed, int64_t n, double *out) {
+    if (n < 0 || (n > 0 && !out)) return fail(nullptr, VRS_E_ARG, "bad n or out");
+    rng_normals(seed, n, out);
+    return VRS_OK;
+}
+
+vrs_status vrs_rng_uniforms(uint64_t seed, int64_t n, double *out) {
+    if (n < 0 || (n > 0 && !out)) return fail(nullptr, VRS_E_ARG, "bad n or out");
+    rng_uniforms(seed, n, out);
+    return VRS_OK;
+}
+
+vrs_status vrs_sky_lognormal(uint64_t seed, double sigma, int64_t n_pix, int32_t n_src,
+                             double s_min, double s_max, float *out) {
+    if (n_pix <= 0 || !out) return fail(nullptr, VRS_E_ARG, "n_pix must be > 0 and out non-NULL");
+    if (!std::isfinite(sigma) || sigma < 0) return fail(nullptr, VRS_E_ARG, "sigma must be >= 0");
+    if (n_src < 0 || n_src > n_pix) return fail(nullptr, VRS_E_ARG, "n_src out of range");
+    if (n_src > 0 && !(s_min > 0 && s_max > s_min))
+        return fail(nullptr, VRS_E_ARG, "need 0 < s_min < s_max");
+    sky_lognormal(seed, sigma, n_pix, n_src, s_min, s_max, out);
+    return VRS_OK;
+}
+
+vrs_status vrs_orbit_offsets(int32_t n_sat, uint64_t seed, double a_km, double max_offset_km,
+                             double *raan, double *m0) {
+    if (n_sat <= 0 || !raan || !m0) return fail(nullptr, VRS_E_ARG, "bad n_sat or outputs");
+    if (!(a_km > 0) || !(max_offset_km >= 0) || max_offset_km > 2 * a_km)
+        return fail(nullptr, VRS_E_ARG, "need a_km > 0 and 0 <= max_offset_km <= 2 a_km");
+    orbit_offsets(n_sat, seed, a_km, max_offset_km, raan, m0);
+    return VRS_OK;
+}
+
+vrs_status vrs_healpix_lmn(int32_t nside, double *lmn) {
+    if (nside < 1 || nside > 8192 || !lmn) return fail(nullptr, VRS_E_ARG, "bad nside or lmn");
+    const int64_t n = 12LL * nside * nside;
+    for (int64_t p = 0; p < n; ++p) healpix_ring_vec(nside, p, lmn + 3 * p);
+    return VRS_OK;
+}
+
+vrs_status vrs_healpix_nest2ring(int32_t nside, int64_t *out) {
+    if (nside < 1 || nside > 8192 || (nside & (nside - 1)) || !out)
+        return fail(nullptr, VRS_E_ARG, "nside must be a power of two in [1, 8192]");
+    const int64_t n = 12LL * nside * nside;
+    for (int64_t p = 0; p < n; ++p) out[p] = healpix_nest2ring(nside, p);
+    return VRS_OK;
+}
+
+// ---------------------------------------------------------------- geometry
+void vrs_geom_destroy(vrs_geom *g) {
+    if (!g) return;
+    DeviceGuard dg(g->ctx ? g->ctx->device : 0);
+    if (g->ctx) (void)hipStreamSynchronize(g->ctx->stream);
+    void *ptrs[] = {g->d_orbit, g->d_pos,  g->d_cap,  g->d_uvw64, g->d_uvw32, g->d_pairs,
+                    g->d_pix32, g->d_pix64, g->d_ring, g->d_chunk};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    delete g;
+}
+
+static vrs_status geom_build(vrs_ctx *ctx, const vrs_geom_desc *d, vrs_geom *g) {
+    const vrs_orbit_desc &o = d->orbit;
+    g->n_sat = o.n_sat;
+    g->n_t = o.n_t;
+    g->nside = d->nside;
+    g->r_moon = d->r_moon_km;
+    g->lambda_m = kC / d->freq_hz;
+    if (d->n_base == 0) {
+        for (int a = 0; a < o.n_sat; ++a)
+            for (int b = a + 1; b < o.n_sat; ++b) {
+                g->pairs.push_back(a);
+                g->pairs.push_back(b);
+            }
+    } else {
+        g->pairs.assign(d->pairs, d->pairs + 2 * d->n_base);
+    }
+    g->n_base = (int32_t)(g->pairs.size() / 2);
+    g->n_vis = (int64_t)g->n_base * g->n_t;
+    g->n_pix = 12LL * d->nside * d->nside;
+    g->n_chunk = (int32_t)((g->n_pix + kChunk - 1) / kChunk);
+    g->n_pad = (int64_t)g->n_chunk * kChunk;
+
+    // pixels in NESTED (chunk) order
+    std::vector<float4> pix32(g->n_pad, make_float4(0, 0, 0, 0));
+    std::vector<double4> pix64(g->n_pad, make_double4(0, 0, 0, 0));
+    std::vector<int32_t> ring(g->n_pad, -1);
+    g->lmn_ring.resize(3 * g->n_pix);
+    for (int64_t p = 0; p < g->n_pix; ++p) healpix_ring_vec(d->nside, p, &g->lmn_ring[3 * p]);
+    for (int64_t q = 0; q < g->n_pix; ++q) {
+        const int64_t r = healpix_nest2ring(d->nside, q);
+        const double *v = &g->lmn_ring[3 * r];
+        pix64[q] = make_double4(v[0], v[1], v[2], 0.0);
+        pix32[q] = make_float4((float)v[0], (float)v[1], (float)v[2], 0.0f);
+        ring[q] = (int32_t)r;
+    }
+    std::vector<ChunkRec> chunks(g->n_chunk);
+    for (int32_t c = 0; c < g->n_chunk; ++c) {
+        double sx = 0, sy = 0, sz = 0;
+        int nv = 0;
+        uint32_t h = 0;
+        for (int i = 0; i < kChunk; ++i) {
+            const int64_t q = (int64_t)c * kChunk + i;
+            if (ring[q] < 0) continue;
+            sx += pix64[q].x; sy += pix64[q].y; sz += pix64[q].z;
+            ++nv;
+            h ^= pix_hash_u64((uint64_t)ring[q]);
+        }
+        const double nrm = std::sqrt(sx * sx + sy * sy + sz * sz);
+        ChunkRec &ch = chunks[c];
+        double rho = M_PI;
+        if (nrm > 1e-6 * nv) {
+            sx /= nrm; sy /= nrm; sz /= nrm;
+            double maxchord = 0;
+            for (int i = 0; i < kChunk; ++i) {
+                const int64_t q = (int64_t)c * kChunk + i;
+                if (ring[q] < 0) continue;
+                const double dx = pix64[q].x - sx, dy = pix64[q].y - sy, dz = pix64[q].z - sz;
+                maxchord = std::max(maxchord, std::sqrt(dx * dx + dy * dy + dz * dz));
+            }
+            rho = 2.0 * std::asin(std::min(1.0, 0.5 * maxchord));
+            rho = rho * (1.0 + 1e-6) + 1e-6;
+        } else {
+            sx = 0; sy = 0; sz = 1;
+        }
+        ch.gx = (float)sx; ch.gy = (float)sy; ch.gz = (float)sz;
+        ch.crho = (float)std::cos(rho);
+        ch.srho = (float)std::sin(rho);
+        ch.flags = (rho > 0.5 || nv < kChunk) ? kChunkAlwaysStraddle : 0;
+        ch.n_valid = nv;
+        ch.hash_all = h;
+    }
+
+    hipStream_t st = ctx->stream;
+    const int64_t nst = (int64_t)g->n_sat * g->n_t;
+    VRS_HIP(ctx, hipMalloc(&g->d_orbit, 2 * sizeof(double) * g->n_sat));
+    VRS_HIP(ctx, hipMalloc(&g->d_pos, sizeof(double4) * nst));
+    VRS_HIP(ctx, hipMalloc(&g->d_cap, sizeof(CapRec) * nst));
+    VRS_HIP(ctx, hipMalloc(&g->d_uvw64, sizeof(double4) * std::max<int64_t>(g->n_vis, 1)));
+    VRS_HIP(ctx, hipMalloc(&g->d_uvw32, sizeof(float4) * std::max<int64_t>(g->n_vis, 1)));
+    VRS_HIP(ctx, hipMalloc(&g->d_pairs, sizeof(int2) * g->n_base));
+    VRS_HIP(ctx, hipMalloc(&g->d_pix32, sizeof(float4) * g->n_pad));
+    VRS_HIP(ctx, hipMalloc(&g->d_pix64, sizeof(double4) * g->n_pad));
+    VRS_HIP(ctx, hipMalloc(&g->d_ring, sizeof(int32_t) * g->n_pad));
+    VRS_HIP(ctx, hipMalloc(&g->d_chunk, sizeof(ChunkRec) * g->n_chunk));
+
+    std::vector<double> orbit(2 * g->n_sat);
+    for (int k = 0; k < g->n_sat; ++k) {
+        orbit[k] = o.raan_rad[k];
+        orbit[g->n_sat + k] = o.m0_rad[k];
+    }
+    VRS_HIP(ctx, hipMemcpyAsync(g->d_orbit, orbit.data(), sizeof(double) * orbit.size(),
+                                hipMemcpyHostToDevice, st));
+    VRS_HIP(ctx, hipMemcpyAsync(g->d_pairs, g->pairs.data(), sizeof(int32_t) * g->pairs.size(),
+                                hipMemcpyHostToDevice, st));
+    VRS_HIP(ctx, hipMemcpyAsync(g->d_pix32, pix32.data(), sizeof(float4) * g->n_pad,
+                                hipMemcpyHostToDevice, st));
+    VRS_HIP(ctx, hipMemcpyAsync(g->d_pix64, pix64.data(), sizeof(double4) * g->n_pad,
+                                hipMemcpyHostToDevice, st));
+    VRS_HIP(ctx, hipMemcpyAsync(g->d_ring, ring.data(), sizeof(int32_t) * g->n_pad,
+                                hipMemcpyHostToDevice, st));
+    VRS_HIP(ctx, hipMemcpyAsync(g->d_chunk, chunks.data(), sizeof(ChunkRec) * g->n_chunk,
+                                hipMemcpyHostToDevice, st));
+
+    OrbitArgs &a = g->orb;
+    a.n_sat = g->n_sat;
+    a.n_t = g->n_t;
+    a.dt = o.dt_s;
+    a.a = o.a_km;
+    a.cos_i = std::cos(o.incl_rad);
+    a.sin_i = std::sin(o.incl_rad);
+    a.n_mot = std::sqrt(o.mu_km3_s2 / (o.a_km * o.a_km * o.a_km));
+    a.r_moon = d->r_moon_km;
+    a.inv_lambda_km = 1e3 / g->lambda_m;
+    a.raan = g->d_orbit;
+    a.m0 = g->d_orbit + g->n_sat;
+    // the host-side copies above are pageable; finish them before the vectors go away
+    VRS_HIP(ctx, hipStreamSynchronize(st));
+    return VRS_OK;
+}
+
+vrs_status vrs_geom_create(vrs_ctx *ctx, const vrs_geom_desc *d, vrs_geom **out) {
+    if (vrs_status s = check_ctx(ctx)) return s;
+    if (!d || !out) return fail(ctx, VRS_E_ARG, "desc and out must be non-NULL");
+    *out = nullptr;
+    const vrs_orbit_desc &o = d->orbit;
+    if (o.n_sat < 2 || o.n_sat > 4096) return fail(ctx, VRS_E_ARG, "n_sat must be in [2, 4096]");
+    if (o.n_t < 1) return fail(ctx, VRS_E_ARG, "n_t must be >= 1");
+    if (!(o.dt_s > 0) || !std::isfinite(o.dt_s)) return fail(ctx, VRS_E_ARG, "dt_s must be > 0");
+    if (!(o.mu_km3_s2 > 0)) return fail(ctx, VRS_E_ARG, "mu must be > 0");
+    if (!o.raan_rad || !o.m0_rad) return fail(ctx, VRS_E_ARG, "raan_rad and m0_rad are required");
+    for (int k = 0; k < o.n_sat; ++k)
+        if (!std::isfinite(o.raan_rad[k]) || !std::isfinite(o.m0_rad[k]))
+            return fail(ctx, VRS_E_ARG, "orbit angles of satellite %d are not finite", k);
+    if (!(d->r_moon_km > 0)) return fail(ctx, VRS_E_ARG, "r_moon_km must be > 0");
+    if (!(o.a_km > d->r_moon_km * (1.0 + 1e-3)))
+        return fail(ctx, VRS_E_ARG, "orbit radius %.3f km must exceed the Moon radius %.3f km",
+                    o.a_km, d->r_moon_km);
+    if (!(d->freq_hz > 0) || !std::isfinite(d->freq_hz))
+        return fail(ctx, VRS_E_ARG, "freq_hz must be > 0");
+    if (d->nside < 1 || d->nside > 4096 || (d->nside & (d->nside - 1)))
+        return fail(ctx, VRS_E_ARG, "nside must be a power of two in [1, 4096]");
+    if (d->n_base < 0) return fail(ctx, VRS_E_ARG, "n_base must be >= 0");
+    if (d->n_base > 0) {
+        if (!d->pairs) return fail(ctx, VRS_E_ARG, "pairs required when n_base > 0");
+        for (int b = 0; b < d->n_base; ++b) {
+            int pa = d->pairs[2 * b], pb = d->pairs[2 * b + 1];
+            if (pa < 0 || pb < 0 || pa >= o.n_sat || pb >= o.n_sat || pa == pb)
+                return fail(ctx, VRS_E_ARG, "baseline %d has invalid satellites (%d, %d)", b, pa, pb);
+        }
+    }
+    const int64_t nb = d->n_base ? d->n_base : (int64_t)o.n_sat * (o.n_sat - 1) / 2;
+    if (nb * (int64_t)o.n_t > (int64_t)1 << 31)
+        return fail(ctx, VRS_E_ARG, "n_base * n_t = %lld exceeds 2^31", (long long)(nb * o.n_t));
+    if ((int64_t)o.n_sat * o.n_t > (int64_t)1 << 31)
+        return fail(ctx, VRS_E_ARG, "n_sat * n_t exceeds 2^31");
+    DeviceGuard dg(ctx->device);
+    vrs_geom *g = new vrs_geom;
+    g->ctx = ctx;
+    vrs_status s = geom_build(ctx, d, g);
+    if (s == VRS_OK) s = vrs_orbit_uvw(ctx, g, nullptr, nullptr);
+    if (s == VRS_OK) {
+        VRS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        *out = g;
+    } else {
+        vrs_geom_destroy(g);
+    }
+    return s;
+}
+
+vrs_status vrs_geom_info_get(const vrs_geom *g, vrs_geom_info *out) {
+    if (!g || !out) return fail(nullptr, VRS_E_ARG, "geom and out must be non-NULL");
+    out->n_vis = g->n_vis;
+    out->n_pix = g->n_pix;
+    out->n_sat = g->n_sat;
+    out->n_t = g->n_t;
+    out->n_base = g->n_base;
+    out->nside = g->nside;
+    out->n_chunk = g->n_chunk;
+    out->chunk_pixels = kChunk;
+    out->lambda_m = g->lambda_m;
+    return VRS_OK;
+}
+
+vrs_status vrs_orbit_uvw(vrs_ctx *ctx, vrs_geom *g, double *sat_pos, double *uvw) {
+    if (vrs_status s = check_ctx(ctx)) return s;
+    if (!g || g->ctx != ctx) return fail(ctx, VRS_E_ARG, "geometry does not belong to this context");
+    DeviceGuard dg(ctx->device);
+    {
+        Timer tm(ctx, "orbit");
+        VRS_HIP(ctx, launch_orbit(g->orb, g->dev(), g->d_pos, g->d_cap, g->d_uvw64, g->d_uvw32,
+                                  ctx->stream));
+    }
+    // strided exports (x, y, z) out of the padded 4-wide device records
+    if (sat_pos)
+        VRS_HIP(ctx, hipMemcpy2DAsync(sat_pos, 3 * sizeof(double), g->d_pos, sizeof(double4),
+                                      3 * sizeof(double), (size_t)g->n_sat * g->n_t,
+                                      hipMemcpyDefault, ctx->stream));
+    if (uvw)
+        VRS_HIP(ctx, hipMemcpy2DAsync(uvw, 3 * sizeof(double), g->d_uvw64, sizeof(double4),
+                                      3 * sizeof(double), (size_t)g->n_vis, hipMemcpyDefault,
+                                      ctx->stream));
+    if (sat_pos || uvw) VRS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return VRS_OK;
+}
+
+vrs_status vrs_geom_lmn(vrs_ctx *ctx, const vrs_geom *g, double *lmn) {
+    if (!g || !lmn) return fail(ctx, VRS_E_ARG, "geom and lmn must be non-NULL");
+    if (is_device_ptr(lmn)) {
+        DeviceGuard dg(g->ctx->device);
+        VRS_HIP(ctx, hipMemcpy(lmn, g->lmn_ring.data(), sizeof(double) * g->lmn_ring.size(),
+                               hipMemcpyHostToDevice));
+    } else {
+        std::memcpy(lmn, g->lmn_ring.data(), sizeof(double) * g->lmn_ring.size());
+    }
+    return VRS_OK;
+}
+
+// ---------------------------------------------------------------- shade
+vrs_status vrs_shade(vrs_ctx *ctx, const vrs_geom *g, int32_t *count, uint32_t *hash,
+                     uint32_t *bits, vrs_shade_stats *stats) {
+    if (vrs_status s = check_ctx(ctx)) return s;
+    if (!g || g->ctx != ctx) return fail(ctx, VRS_E_ARG, "geometry does not belong to this context");
+    const int64_t n_words = (g->n_pix + 31) / 32;
+    if (bits && g->n_vis * n_words * 4 > ((int64_t)1 << 32))
+        return fail(ctx, VRS_E_ARG, "shade bitmap of %lld bytes exceeds 4 GiB",
+                    (long long)(g->n_vis * n_words * 4));
+    DeviceGuard dg(ctx->device);
+    Staged sc(ctx, count, sizeof(int32_t) * g->n_vis, true);
+    Staged sh(ctx, hash, sizeof(uint32_t) * g->n_vis, true);
+    Staged sb(ctx, bits, sizeof(uint32_t) * g->n_vis * n_words, true);
+    if (vrs_status s = sc.prepare()) return s;
+    if (vrs_status s = sh.prepare()) return s;
+    if (vrs_status s = sb.prepare()) return s;
+    if (bits) VRS_HIP(ctx, hipMemsetAsync(sb.dev, 0, sb.bytes, ctx->stream));
+    VRS_HIP(ctx, hipMemsetAsync(ctx->d_tot, 0, 4 * sizeof(unsigned long long), ctx->stream));
+    const GeomDev gd = g->dev();
+    const int grid = persistent_grid(ctx, shade_kernel_ptr(), (g->n_vis + 3) / 4);
+    {
+        Timer tm(ctx, "shade");
+        VRS_HIP(ctx, launch_shade(gd, (int32_t *)sc.dev, (uint32_t *)sh.dev, (uint32_t *)sb.dev,
+                                  n_words, ctx->d_tot, grid, ctx->stream));
+    }
+    if (vrs_status s = sc.finish()) return s;
+    if (vrs_status s = sh.finish()) return s;
+    if (vrs_status s = sb.finish()) return s;
+    if (stats) {
+        unsigned long long tot[4];
+        VRS_HIP(ctx, hipMemcpyAsync(tot, ctx->d_tot, sizeof tot, hipMemcpyDeviceToHost, ctx->stream));
+        VRS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        stats->shaded_pairs = (int64_t)tot[0];
+        stats->near_limb = (int64_t)tot[1];
+        stats->fp64_tests = (int64_t)tot[2];
+    }
+    return VRS_OK;
+}
+
+// ---------------------------------------------------------------- vis-gen / recon
+static vrs_status check_geom(vrs_ctx *ctx, const vrs_geom *g) {
+    if (!g) return fail(ctx, VRS_E_ARG, "geometry is NULL");
+    if (g->ctx != ctx) return fail(ctx, VRS_E_ARG, "geometry does not belong to this context");
+    return VRS_OK;
+}
+
+vrs_status vrs_visgen(vrs_ctx *ctx, const vrs_geom *g, const vrs_knobs *k, int32_t n_maps,
+                      const float *I, float *V) {
+    if (vrs_status s = check_ctx(ctx)) return s;
+    if (vrs_status s = check_geom(ctx, g)) return s;
+    if (vrs_status s = check_knobs(ctx, k)) return s;
+    if (n_maps < 1 || n_maps > 65536) return fail(ctx, VRS_E_ARG, "n_maps must be in [1, 65536]");
+    if (!I || !V) return fail(ctx, VRS_E_ARG, "I and V must be non-NULL");
+    if (k->precision == VRS_MFMA_SPLIT16)
+        return fail(ctx, VRS_E_ARG, "VRS_MFMA_SPLIT16 vis-gen is not available in this build");
+    DeviceGuard dg(ctx->device);
+    Staged sI(ctx, I, sizeof(float) * n_maps * g->n_pix, false);
+    Staged sV(ctx, V, sizeof(float) * 2 * n_maps * g->n_vis, true);
+    if (vrs_status s = sI.prepare()) return s;
+    if (vrs_status s = sV.prepare()) return s;
+    void *ich;
+    if (vrs_status s = ws_get(ctx, 0, sizeof(float) * n_maps * g->n_pad, &ich)) return s;
+    const GeomDev gd = g->dev();
+    VRS_HIP(ctx, launch_to_chunk_order(gd, n_maps, g->n_pix, (const float *)sI.dev, (float *)ich,
+                                       ctx->stream));
+    const bool f64 = k->precision == VRS_MIXED_F64PHASE;
+    const int64_t n_tiles = ((g->n_vis + visgen_tile_vis() - 1) / visgen_tile_vis()) * n_maps;
+    const int grid = persistent_grid(ctx, visgen_kernel_ptr(f64), n_tiles);
+    {
+        Timer tm(ctx, "visgen");
+        VRS_HIP(ctx, launch_visgen(gd, phase_knobs(k), f64, n_maps, (const float *)ich,
+                                   (float2 *)sV.dev, grid, ctx->stream));
+    }
+    if (vrs_status s = sV.finish()) return s;
+    return VRS_OK;
+}
+
+static vrs_status recon_common(vrs_ctx *ctx, const vrs_geom *g, const vrs_knobs *k,
+                               int32_t n_maps, const float *V, const float *w, float *F,
+                               double kappa, const char *timer_name) {
+    DeviceGuard dg(ctx->device);
+    Staged sV(ctx, V, sizeof(float) * 2 * n_maps * g->n_vis, false);
+    Staged sw(ctx, w, sizeof(float) * g->n_vis, false);
+    Staged sF(ctx, F, sizeof(float) * n_maps * g->n_pix, true);
+    if (vrs_status s = sV.prepare()) return s;
+    if (vrs_status s = sw.prepare()) return s;
+    if (vrs_status s = sF.prepare()) return s;
+    // segments of the visibility range: enough work items to fill the chip,
+    // a multiple of 8 so that each XCD keeps to its own segments
+    const int64_t units = (int64_t)n_maps * g->n_chunk;
+    int64_t n_seg = (4096 + units - 1) / units;
+    n_seg = std::min<int64_t>(n_seg, std::max<int64_t>(1, g->n_vis / 256));
+    if (n_seg > 8) n_seg = (n_seg + 7) / 8 * 8;
+    n_seg = std::max<int64_t>(1, std::min<int64_t>(n_seg, 1024));
+    void *part;
+    if (vrs_status s = ws_get(ctx, 1, sizeof(float) * n_seg * n_maps * g->n_pad, &part)) return s;
+    const GeomDev gd = g->dev();
+    const bool f64 = k->precision == VRS_MIXED_F64PHASE;
+    const int grid = persistent_grid(ctx, recon_kernel_ptr(f64), units * n_seg);
+    {
+        Timer tm(ctx, timer_name);
+        VRS_HIP(ctx, launch_recon(gd, phase_knobs(k), f64, n_maps, (const float2 *)sV.dev,
+                                  (const float *)sw.dev, (float)kappa, (int32_t)n_seg,
+                                  (float *)part, grid, ctx->stream));
+    }
+    VRS_HIP(ctx, launch_recon_finalize(gd, n_maps, (int32_t)n_seg, g->n_pix, (const float *)part,
+                                       (float *)sF.dev, ctx->stream));
+    if (vrs_status s = sF.finish()) return s;
+    return VRS_OK;
+}
+
+vrs_status vrs_recon(vrs_ctx *ctx, const vrs_geom *g, const vrs_knobs *k, int32_t n_maps,
+                     const float *V, const float *w, float *F) {
+    if (vrs_status s = check_ctx(ctx)) return s;
+    if (vrs_status s = check_geom(ctx, g)) return s;
+    if (vrs_status s = check_knobs(ctx, k)) return s;
+    if (n_maps < 1 || n_maps > 65536) return fail(ctx, VRS_E_ARG, "n_maps must be in [1, 65536]");
+    if (!V || !F) return fail(ctx, VRS_E_ARG, "V and F must be non-NULL");
+    if (k->precision == VRS_MFMA_SPLIT16)
+        return fail(ctx, VRS_E_ARG, "VRS_MFMA_SPLIT16 recon is not available in this build");
+    const double kappa = k->kappa > 0 ? k->kappa : 1.0 / (double)g->n_vis;
+    return recon_common(ctx, g, k, n_maps, V, w, F, kappa, "recon");
+}
+
+vrs_status vrs_adjoint(vrs_ctx *ctx, const vrs_geom *g, const vrs_knobs *k, const float *resid,
+                       float *grad) {
+    if (vrs_status s = check_ctx(ctx)) return s;
+    if (vrs_status s = check_geom(ctx, g)) return s;
+    if (vrs_status s = check_knobs(ctx, k)) return s;
+    if (!resid || !grad) return fail(ctx, VRS_E_ARG, "resid and grad must be non-NULL");
+    vrs_knobs kk = *k;
+    if (kk.precision == VRS_MFMA_SPLIT16) kk.precision = VRS_MIXED_F64PHASE;
+    return recon_common(ctx, g, &kk, 1, resid, nullptr, grad, 2.0, "adjoint");
+}
+
+}  // extern "C"

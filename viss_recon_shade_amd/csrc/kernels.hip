@@ -1,0 +1,633 @@
+// gfx950 kernels of the four hot-path stages.
+//
+// Work layout (all kernels): lanes own pixels of one spatially compact chunk
+// (kPixPerLane pixels per lane, kChunk per chunk, coalesced float4 loads);
+// the visibility side is wave-uniform and comes from LDS tiles.  For every
+// (visibility, chunk) pair the chunk's bounding cap is classified against the
+// two satellites' shade caps: wholly outside (no mask work), wholly inside
+// (shaded: t_occ in vis-gen, skipped in recon) or straddling (per-pixel FP32
+// test; pixels inside the FP32 guard band are decided by the exact FP64
+// predicate, the same sequence of fma()s as the FP64 oracle).
+//
+// Phase: revolutions.  phi = u l + v m + w n + wo with wo = -w (n - 1
+// convention) or 0 (trailing exp(-2 pi i s w) folded in); fract() reduces it
+// to [0, 1) and v_sin_f32 / v_cos_f32, which take revolutions, finish it.
+// The FP64-phase variant forms phi and the fract in FP64 and only the reduced
+// value is rounded to FP32.
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+
+namespace vrs {
+
+namespace {
+
+__device__ __forceinline__ float wave_sum(float x) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, kWave);
+    return x;
+}
+__device__ __forceinline__ int wave_sum_i(int x) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, kWave);
+    return x;
+}
+__device__ __forceinline__ uint32_t wave_xor(uint32_t x) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x ^= (uint32_t)__shfl_xor((int)x, o, kWave);
+    return x;
+}
+
+// Classify a chunk's bounding cap against the caps of satellites a and b.
+__device__ __forceinline__ int classify(const ChunkRec &ch, const CapRec &a, const CapRec &b) {
+    if (ch.flags & kChunkAlwaysStraddle) return kStraddle;
+    const float xa = fmaf(ch.gz, a.dz, fmaf(ch.gy, a.dy, ch.gx * a.dx));
+    const float xb = fmaf(ch.gz, b.dz, fmaf(ch.gy, b.dy, ch.gx * b.dx));
+    const float ma = a.g + kChunkMargin, mb = b.g + kChunkMargin;
+    // all pixels within angle alpha - margin  <=  x >= cos(alpha - rho) (+ margin)
+    const bool in_a = (xa >= fmaf(a.c, ch.crho, a.sa * ch.srho) + ma) && (a.c < ch.crho);
+    const bool in_b = (xb >= fmaf(b.c, ch.crho, b.sa * ch.srho) + mb) && (b.c < ch.crho);
+    // all pixels beyond angle alpha + margin  <=  x < cos(alpha + rho) (- margin)
+    const bool out_a = xa < fmaf(a.c, ch.crho, -a.sa * ch.srho) - ma;
+    const bool out_b = xb < fmaf(b.c, ch.crho, -b.sa * ch.srho) - mb;
+    return (in_a || in_b) ? kIn : ((out_a && out_b) ? kOut : kStraddle);
+}
+
+// FP32 cap test; flags pixels inside the guard band.
+__device__ __forceinline__ bool occ32(float sx, float sy, float sz, const CapRec &a, bool &unc) {
+    const float x = fmaf(sz, a.dz, fmaf(sy, a.dy, sx * a.dx));
+    unc = unc || (fabsf(x - a.c) <= a.g);
+    return x >= a.c;
+}
+
+// Exact predicate, identical operation sequence to the FP64 oracle:
+// d = r.s < 0 and |r|^2 - d^2 <= R^2; near-limb when |r|^2 - d^2 is within
+// ((R - e)^2, (R + e)^2), e = 1e-6 R.
+__device__ __forceinline__ bool occ64(const double4 r, double sx, double sy, double sz,
+                                      const LimbConsts &L, int &near) {
+    const double d = fma(r.z, sz, fma(r.y, sy, r.x * sx));
+    if (!(d < 0.0)) return false;
+    const double q = fma(-d, d, r.w);
+    if (q > L.lo2 && q < L.hi2) near = 1;
+    return q <= L.R2;
+}
+
+__device__ __forceinline__ bool occ_pair64(const double4 *__restrict__ pos,
+                                        const double4 *__restrict__ pix64, int64_t pi, int ia,
+                                        int ib, LimbConsts L, int *near) {
+    const double4 s = pix64[pi];
+    int na = 0, nb = 0;
+    const bool m = occ64(pos[ia], s.x, s.y, s.z, L, na) | occ64(pos[ib], s.x, s.y, s.z, L, nb);
+    *near = na | nb;
+    return m;
+}
+
+__device__ __forceinline__ float fract_rev(float x) { return __builtin_amdgcn_fractf(x); }
+__device__ __forceinline__ float fract_rev(double x) { return (float)__builtin_amdgcn_fract(x); }
+
+// ------------------------------------------------------------------ stage 1
+__global__ void __launch_bounds__(256) orbit_kernel(OrbitArgs a, double4 *__restrict__ pos,
+                                                    CapRec *__restrict__ cap) {
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)a.n_sat * a.n_t) return;
+    const int k = (int)(idx / a.n_t);
+    const int j = (int)(idx % a.n_t);
+    // circular orbit: E = M = nu (calculateOrbit, MIncline_generate_uvw_float.cu:93-104)
+    const double w = a.m0[k] + a.n_mot * ((double)j * a.dt);
+    const double cw = cos(w), sw = sin(w), cO = cos(a.raan[k]), sO = sin(a.raan[k]);
+    const double x = a.a * (cw * cO - sw * a.cos_i * sO);
+    const double y = a.a * (cw * sO + sw * a.cos_i * cO);
+    const double z = a.a * (sw * a.sin_i);
+    const double r2 = fma(z, z, fma(y, y, x * x));
+    pos[idx] = make_double4(x, y, z, r2);
+    const double rn = sqrt(r2);
+    const double R = a.r_moon;
+    const double c = sqrt(fmax(r2 - R * R, 0.0)) / rn;
+    CapRec cr;
+    cr.dx = (float)(-x / rn);
+    cr.dy = (float)(-y / rn);
+    cr.dz = (float)(-z / rn);
+    cr.c = (float)c;
+    cr.sa = (float)(R / rn);
+    // near-limb band |delta| < 1e-6 R maps to |dot - c| < 1e-6 R^2 / (|r|^2 c)
+    const double band = kNearLimbRel * R * R / (r2 * fmax(c, 1e-12));
+    cr.g = (float)fmax((double)kGuardMin, 2.0 * band + 1e-6);
+    cr.pad0 = cr.pad1 = 0.0f;
+    cap[idx] = cr;
+}
+
+__global__ void __launch_bounds__(256) uvw_kernel(GeomDev g, double inv_lambda_km,
+                                                  double4 *__restrict__ uvw64,
+                                                  float4 *__restrict__ uvw32) {
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= g.n_vis) return;
+    const int b = (int)(v / g.n_t), t = (int)(v % g.n_t);
+    const int2 pr = g.pairs[b];
+    const double4 ra = g.pos[(int64_t)pr.x * g.n_t + t], rb = g.pos[(int64_t)pr.y * g.n_t + t];
+    // computeUVW without ceil (MIncline_generate_uvw_float.cu:156-158)
+    const double u = (rb.x - ra.x) * inv_lambda_km, vv = (rb.y - ra.y) * inv_lambda_km,
+                 w = (rb.z - ra.z) * inv_lambda_km;
+    uvw64[v] = make_double4(u, vv, w, 0.0);
+    uvw32[v] = make_float4((float)u, (float)vv, (float)w, 0.0f);
+}
+
+// ------------------------------------------------------------------ stage 2
+// One wave per visibility; lanes classify 64 chunks at a time, then sweep the
+// straddling (and, for the bitmap, the shaded) chunks pixel by pixel.
+__global__ void __launch_bounds__(kBlock) shade_kernel(GeomDev g, int32_t *__restrict__ count,
+                                                       uint32_t *__restrict__ hash,
+                                                       uint32_t *__restrict__ bits,
+                                                       int64_t n_words,
+                                                       unsigned long long *__restrict__ tot) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t wave0 = (int64_t)blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    const int64_t nwaves = (int64_t)gridDim.x * kWavesPerBlock;
+    unsigned long long t_shaded = 0, t_near = 0, t_f64 = 0;
+    for (int64_t v = wave0; v < g.n_vis; v += nwaves) {
+        const int b = (int)(v / g.n_t), t = (int)(v % g.n_t);
+        const int2 pr = g.pairs[b];
+        const int ia = pr.x * g.n_t + t, ib = pr.y * g.n_t + t;
+        const CapRec A = g.cap[ia], B = g.cap[ib];
+        int cnt = 0, near = 0, f64 = 0;
+        uint32_t h = 0;
+        for (int c0 = 0; c0 < g.n_chunk; c0 += kWave) {
+            const int c = c0 + lane;
+            int cls = kOut;
+            if (c < g.n_chunk) {
+                const ChunkRec ch = g.chunk[c];
+                cls = classify(ch, A, B);
+                if (cls == kIn) {
+                    cnt += ch.n_valid;
+                    h ^= ch.hash_all;
+                }
+            }
+            uint64_t todo = __ballot(cls == kStraddle);
+            if (bits) todo |= __ballot(cls == kIn);
+            while (todo) {
+                const int cs = __ffsll((unsigned long long)todo) - 1;
+                todo &= todo - 1;
+                const int cc = c0 + cs;
+                const int whole = __shfl(cls, cs, kWave) == kIn;
+#pragma unroll
+                for (int j = 0; j < kPixPerLane; ++j) {
+                    const int64_t pi = (int64_t)cc * kChunk + j * kWave + lane;
+                    const int rp = g.ring[pi];
+                    if (rp < 0) continue;
+                    bool m;
+                    if (whole) {
+                        m = true;  // counted above
+                    } else {
+                        const float4 s = g.pix32[pi];
+                        bool unc = false;
+                        m = occ32(s.x, s.y, s.z, A, unc) | occ32(s.x, s.y, s.z, B, unc);
+                        if (unc) {
+                            int nl = 0;
+                            m = occ_pair64(g.pos, g.pix64, pi, ia, ib, g.limb, &nl);
+                            near += nl;
+                            ++f64;
+                        }
+                        if (m) {
+                            ++cnt;
+                            h ^= pix_hash_u64((uint64_t)rp);
+                        }
+                    }
+                    if (m && bits)
+                        atomicOr(&bits[v * n_words + (rp >> 5)], 1u << (rp & 31));
+                }
+            }
+        }
+        cnt = wave_sum_i(cnt);
+        near = wave_sum_i(near);
+        f64 = wave_sum_i(f64);
+        h = wave_xor(h);
+        if (lane == 0) {
+            if (count) count[v] = cnt;
+            if (hash) hash[v] = h;
+            t_shaded += (unsigned long long)cnt;
+            t_near += (unsigned long long)near;
+            t_f64 += (unsigned long long)f64;
+        }
+    }
+    if (lane == 0 && tot) {
+        atomicAdd(&tot[0], t_shaded);
+        atomicAdd(&tot[1], t_near);
+        atomicAdd(&tot[2], t_f64);
+    }
+}
+
+__global__ void __launch_bounds__(256) to_chunk_order_kernel(GeomDev g, int32_t n_maps,
+                                                             int64_t n_pix,
+                                                             const float *__restrict__ I,
+                                                             float *__restrict__ Ich) {
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)n_maps * g.n_pad) return;
+    const int64_t m = idx / g.n_pad, pi = idx % g.n_pad;
+    const int r = g.ring[pi];
+    Ich[idx] = (r >= 0) ? I[m * n_pix + r] : 0.0f;
+}
+
+// ------------------------------------------------------------------ stage 3
+constexpr int kVisTile = 16;  // visibilities per vis-gen tile (accumulators per lane: 2x)
+
+template <bool F64> struct PhaseT { using T = float; };
+template <> struct PhaseT<true> { using T = double; };
+
+template <bool F64> struct VisEnt {
+    typename PhaseT<F64>::T u, v, w, wo;
+    CapRec a, b;
+    int32_t ia, ib, pad0, pad1;
+};
+
+template <bool F64>
+__device__ __forceinline__ void fill_vis_ent(const GeomDev &g, const PhaseKnobs &k, int64_t v,
+                                             VisEnt<F64> &e) {
+    using T = typename PhaseT<F64>::T;
+    if (v < g.n_vis) {
+        const int b = (int)(v / g.n_t), t = (int)(v % g.n_t);
+        const int2 pr = g.pairs[b];
+        e.ia = pr.x * g.n_t + t;
+        e.ib = pr.y * g.n_t + t;
+        e.a = g.cap[e.ia];
+        e.b = g.cap[e.ib];
+        if (F64) {
+            const double4 q = g.uvw64[v];
+            e.u = (T)(q.x * k.s);
+            e.v = (T)(q.y * k.s);
+            e.w = (T)(q.z * k.s);
+        } else {
+            const float4 q = g.uvw32[v];
+            const float s = (float)k.s;
+            e.u = (T)(q.x * s);
+            e.v = (T)(q.y * s);
+            e.w = (T)(q.z * s);
+        }
+        e.wo = k.wfac ? (T)0 : -e.w;
+    } else {
+        // tail: no shading (c = 2 is never reached), zero phase; never written out
+        e.ia = e.ib = 0;
+        CapRec none = {0.f, 0.f, 0.f, 2.0f, 0.f, 0.f, 0.f, 0.f};
+        e.a = none;
+        e.b = none;
+        e.u = e.v = e.w = e.wo = (T)0;
+    }
+}
+
+template <bool F64>
+__global__ void __launch_bounds__(kBlock) visgen_kernel(GeomDev g, PhaseKnobs k, int32_t n_maps,
+                                                        const float *__restrict__ Ich,
+                                                        float2 *__restrict__ V) {
+    using T = typename PhaseT<F64>::T;
+    __shared__ VisEnt<F64> ents[kVisTile];
+    __shared__ float red[kWavesPerBlock][kVisTile][2];
+    __shared__ float corr[kVisTile][2];  // FP64 guard-band corrections
+    constexpr uint32_t kTileMask = (kVisTile == 32) ? 0xffffffffu : ((1u << kVisTile) - 1u);
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x >> 6;
+    const int64_t n_vtiles = (g.n_vis + kVisTile - 1) / kVisTile;
+    const int64_t n_tiles = n_vtiles * n_maps;
+    const float t_occ = k.t_occ;
+
+    for (int64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+        const int64_t map = tile / n_vtiles;
+        const int64_t v0 = (tile % n_vtiles) * kVisTile;
+        const float *__restrict__ Im = Ich + map * g.n_pad;
+        if (threadIdx.x < kVisTile) fill_vis_ent<F64>(g, k, v0 + threadIdx.x, ents[threadIdx.x]);
+        if (threadIdx.x < 2 * kVisTile) (&corr[0][0])[threadIdx.x] = 0.0f;
+        __syncthreads();
+
+        float are[kVisTile], aim[kVisTile];
+#pragma unroll
+        for (int q = 0; q < kVisTile; ++q) are[q] = aim[q] = 0.0f;
+
+        for (int c = wave; c < g.n_chunk; c += kWavesPerBlock) {
+            // The tile entries are invariant over this loop; keep the compiler
+            // from hoisting all of them into registers (it would need ~450).
+            asm volatile("" ::: "memory");
+            const ChunkRec ch = g.chunk[c];
+            // lane-parallel classification of the tile's visibilities against
+            // this chunk; the results become wave-uniform bit masks
+            uint32_t in_m, out_m;
+            {
+                const int q = lane & (kVisTile - 1);
+                const int cls = classify(ch, ents[q].a, ents[q].b);
+                in_m = (uint32_t)__ballot(cls == kIn) & kTileMask;
+                out_m = (uint32_t)__ballot(cls == kOut) & kTileMask;
+            }
+            T l[kPixPerLane], m[kPixPerLane], n[kPixPerLane];
+            float I[kPixPerLane];
+#pragma unroll
+            for (int j = 0; j < kPixPerLane; ++j) {
+                const int64_t pi = (int64_t)c * kChunk + j * kWave + lane;
+                if (F64) {
+                    const double4 p = g.pix64[pi];
+                    l[j] = (T)p.x; m[j] = (T)p.y; n[j] = (T)p.z;
+                } else {
+                    const float4 p = g.pix32[pi];
+                    l[j] = (T)p.x; m[j] = (T)p.y; n[j] = (T)p.z;
+                }
+                I[j] = Im[pi];
+            }
+            // Wholly shaded or wholly clear pairs run here; straddling pairs
+            // are skipped and done pixel by pixel below.
+            const uint32_t str_m = ~(in_m | out_m) & kTileMask;
+#pragma unroll
+            for (int q = 0; q < kVisTile; ++q) {
+                if ((str_m >> q) & 1u) continue;
+                const bool sh = (in_m >> q) & 1u;
+                const T u = ents[q].u, vv = ents[q].v, w = ents[q].w, wo = ents[q].wo;
+#pragma unroll
+                for (int j = 0; j < kPixPerLane; ++j) {
+                    const float ie = sh ? t_occ : I[j];
+                    const T ph = fma(u, l[j], fma(vv, m[j], fma(w, n[j], wo)));
+                    const float r = fract_rev(ph);
+                    const float sn = __builtin_amdgcn_sinf(r);
+                    const float cs = __builtin_amdgcn_cosf(r);
+                    are[q] = fmaf(ie, cs, are[q]);
+                    aim[q] = fmaf(ie, sn, aim[q]);
+                }
+            }
+            // Straddling pairs: FP32 cap test per pixel; pixels inside the
+            // guard band take the exact FP64 predicate.  The wave's partial
+            // sum goes to the tile's LDS accumulator.
+            uint32_t todo = str_m;
+            while (todo) {
+                const int q = __ffs(todo) - 1;
+                todo &= todo - 1;
+                const CapRec A = ents[q].a, B = ents[q].b;
+                const T u = ents[q].u, vv = ents[q].v, w = ents[q].w, wo = ents[q].wo;
+                float dre = 0.0f, dim = 0.0f;
+#pragma unroll
+                for (int j = 0; j < kPixPerLane; ++j) {
+                    const float sx = (float)l[j], sy = (float)m[j], sz = (float)n[j];
+                    bool unc = false;
+                    bool sh = occ32(sx, sy, sz, A, unc) | occ32(sx, sy, sz, B, unc);
+                    if (unc) {
+                        int nl;
+                        sh = occ_pair64(g.pos, g.pix64, (int64_t)c * kChunk + j * kWave + lane,
+                                        ents[q].ia, ents[q].ib, g.limb, &nl);
+                    }
+                    const float ie = sh ? t_occ : I[j];
+                    const T ph = fma(u, l[j], fma(vv, m[j], fma(w, n[j], wo)));
+                    const float r = fract_rev(ph);
+                    dre = fmaf(ie, __builtin_amdgcn_cosf(r), dre);
+                    dim = fmaf(ie, __builtin_amdgcn_sinf(r), dim);
+                }
+                dre = wave_sum(dre);
+                dim = wave_sum(dim);
+                if (lane == 0) {
+                    atomicAdd(&corr[q][0], dre);
+                    atomicAdd(&corr[q][1], dim);
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < kVisTile; ++q) {
+            const float sr = wave_sum(are[q]);
+            const float si = wave_sum(aim[q]);
+            if (lane == 0) {
+                red[wave][q][0] = sr;
+                red[wave][q][1] = si;
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x < kVisTile) {
+            const int q = threadIdx.x;
+            const int64_t v = v0 + q;
+            if (v < g.n_vis) {
+                float sr = corr[q][0], si = corr[q][1];
+#pragma unroll
+                for (int w = 0; w < kWavesPerBlock; ++w) {
+                    sr += red[w][q][0];
+                    si += red[w][q][1];
+                }
+                // exp(-i theta): V = sum I (cos - i sin)
+                V[map * g.n_vis + v] = make_float2(sr, -si);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------ stage 4
+constexpr int kReconStage = 64;  // visibilities staged in LDS per step
+
+template <bool F64> struct RecEnt {
+    typename PhaseT<F64>::T u, v, w, wo;
+    CapRec a, b;
+    float vr, vi;
+    int32_t ia, ib;
+};
+
+template <bool F64>
+__global__ void __launch_bounds__(kBlock) recon_kernel(GeomDev g, PhaseKnobs k, int32_t n_maps,
+                                                       const float2 *__restrict__ V,
+                                                       const float *__restrict__ wgt,
+                                                       float kappa, int32_t n_seg,
+                                                       float *__restrict__ part) {
+    using T = typename PhaseT<F64>::T;
+    __shared__ RecEnt<F64> ents[kReconStage];
+    __shared__ float red[kWavesPerBlock][kChunk];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x >> 6;
+    const int64_t seg_len = (g.n_vis + n_seg - 1) / n_seg;
+    const int64_t n_items = (int64_t)n_maps * n_seg * g.n_chunk;
+
+    for (int64_t item = blockIdx.x; item < n_items; item += gridDim.x) {
+        // segment varies fastest so that blocks on one XCD share visibility data
+        const int64_t seg = item % n_seg;
+        const int64_t rest = item / n_seg;
+        const int c = (int)(rest % g.n_chunk);
+        const int64_t map = rest / g.n_chunk;
+        const ChunkRec ch = g.chunk[c];
+        T l[kPixPerLane], m[kPixPerLane], n[kPixPerLane];
+        float sx[kPixPerLane], sy[kPixPerLane], sz[kPixPerLane], acc[kPixPerLane];
+#pragma unroll
+        for (int j = 0; j < kPixPerLane; ++j) {
+            const int64_t pi = (int64_t)c * kChunk + j * kWave + lane;
+            if (F64) {
+                const double4 p = g.pix64[pi];
+                l[j] = (T)p.x; m[j] = (T)p.y; n[j] = (T)p.z;
+                sx[j] = (float)p.x; sy[j] = (float)p.y; sz[j] = (float)p.z;
+            } else {
+                const float4 p = g.pix32[pi];
+                l[j] = (T)p.x; m[j] = (T)p.y; n[j] = (T)p.z;
+                sx[j] = p.x; sy[j] = p.y; sz[j] = p.z;
+            }
+            acc[j] = 0.0f;
+        }
+        const int64_t vb = seg * seg_len;
+        const int64_t ve = (vb + seg_len < g.n_vis) ? vb + seg_len : g.n_vis;
+        for (int64_t vt = vb; vt < ve; vt += kReconStage) {
+            if (threadIdx.x < kReconStage) {
+                const int64_t v = vt + threadIdx.x;
+                RecEnt<F64> &e = ents[threadIdx.x];
+                if (v < ve) {
+                    const int b = (int)(v / g.n_t), t = (int)(v % g.n_t);
+                    const int2 pr = g.pairs[b];
+                    e.ia = pr.x * g.n_t + t;
+                    e.ib = pr.y * g.n_t + t;
+                    e.a = g.cap[e.ia];
+                    e.b = g.cap[e.ib];
+                    if (F64) {
+                        const double4 q = g.uvw64[v];
+                        e.u = (T)(q.x * k.s); e.v = (T)(q.y * k.s); e.w = (T)(q.z * k.s);
+                    } else {
+                        const float4 q = g.uvw32[v];
+                        const float s = (float)k.s;
+                        e.u = (T)(q.x * s); e.v = (T)(q.y * s); e.w = (T)(q.z * s);
+                    }
+                    e.wo = k.wfac ? (T)0 : -e.w;
+                    const float2 vv = V[map * g.n_vis + v];
+                    const float sc = kappa * (wgt ? wgt[v] : 1.0f);
+                    e.vr = vv.x * sc;
+                    e.vi = vv.y * sc;
+                } else {
+                    e.ia = e.ib = 0;
+                    CapRec none = {0.f, 0.f, 0.f, 2.0f, 0.f, 0.f, 0.f, 0.f};
+                    e.a = none; e.b = none;
+                    e.u = e.v = e.w = e.wo = (T)0;
+                    e.vr = e.vi = 0.0f;
+                }
+            }
+            __syncthreads();
+            const int nstage = (int)((ve - vt < kReconStage) ? ve - vt : kReconStage);
+#pragma unroll 2
+            for (int q = wave; q < nstage; q += kWavesPerBlock) {
+                const RecEnt<F64> &e = ents[q];
+                const CapRec A = e.a, B = e.b;
+                const int cls = __builtin_amdgcn_readfirstlane(classify(ch, A, B));
+                if (cls == kIn) continue;  // shaded for every pixel of the chunk
+                float vr[kPixPerLane], vi[kPixPerLane];
+                if (cls == kStraddle) {
+#pragma unroll
+                    for (int j = 0; j < kPixPerLane; ++j) {
+                        bool unc = false;
+                        bool sh = occ32(sx[j], sy[j], sz[j], A, unc) |
+                                  occ32(sx[j], sy[j], sz[j], B, unc);
+                        if (unc) {
+                            int nl;
+                            sh = occ_pair64(g.pos, g.pix64,
+                                            (int64_t)c * kChunk + j * kWave + lane, e.ia, e.ib,
+                                            g.limb, &nl);
+                        }
+                        vr[j] = sh ? 0.0f : e.vr;
+                        vi[j] = sh ? 0.0f : e.vi;
+                    }
+                } else {
+#pragma unroll
+                    for (int j = 0; j < kPixPerLane; ++j) {
+                        vr[j] = e.vr;
+                        vi[j] = e.vi;
+                    }
+                }
+                const T u = e.u, vv = e.v, w = e.w, wo = e.wo;
+#pragma unroll
+                for (int j = 0; j < kPixPerLane; ++j) {
+                    const T ph = fma(u, l[j], fma(vv, m[j], fma(w, n[j], wo)));
+                    const float r = fract_rev(ph);
+                    const float sn = __builtin_amdgcn_sinf(r);
+                    const float cs = __builtin_amdgcn_cosf(r);
+                    // Re(V exp(+i theta)) = Vr cos - Vi sin
+                    acc[j] = fmaf(vr[j], cs, acc[j]);
+                    acc[j] = fmaf(-vi[j], sn, acc[j]);
+                }
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int j = 0; j < kPixPerLane; ++j) red[wave][j * kWave + lane] = acc[j];
+        __syncthreads();
+        {
+            const int p = threadIdx.x;  // kBlock == kChunk
+            const float s = red[0][p] + red[1][p] + red[2][p] + red[3][p];
+            part[(map * n_seg + seg) * g.n_pad + (int64_t)c * kChunk + p] = s;
+        }
+        __syncthreads();
+    }
+}
+static_assert(kBlock == kChunk, "recon reduction assumes one thread per chunk pixel");
+
+__global__ void __launch_bounds__(256) recon_finalize_kernel(GeomDev g, int32_t n_maps,
+                                                             int32_t n_seg, int64_t n_pix,
+                                                             const float *__restrict__ part,
+                                                             float *__restrict__ F) {
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)n_maps * g.n_pad) return;
+    const int64_t map = idx / g.n_pad, pi = idx % g.n_pad;
+    const int r = g.ring[pi];
+    if (r < 0) return;
+    float s = 0.0f;
+    for (int q = 0; q < n_seg; ++q) s += part[(map * n_seg + q) * g.n_pad + pi];
+    F[map * n_pix + r] = s;
+}
+
+inline unsigned blocks_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+}  // namespace
+
+hipError_t launch_orbit(const OrbitArgs &a, const GeomDev &g, double4 *pos, CapRec *cap,
+                        double4 *uvw64, float4 *uvw32, hipStream_t st) {
+    const int64_t n = (int64_t)a.n_sat * a.n_t;
+    hipLaunchKernelGGL(orbit_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, st, a, pos, cap);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(uvw_kernel, dim3(blocks_for(g.n_vis, 256)), dim3(256), 0, st, g,
+                       a.inv_lambda_km, uvw64, uvw32);
+    return hipGetLastError();
+}
+
+hipError_t launch_shade(const GeomDev &g, int32_t *count, uint32_t *hash, uint32_t *bits,
+                        int64_t n_words, unsigned long long *tot, int grid, hipStream_t st) {
+    hipLaunchKernelGGL(shade_kernel, dim3(grid), dim3(kBlock), 0, st, g, count, hash, bits,
+                       n_words, tot);
+    return hipGetLastError();
+}
+
+hipError_t launch_to_chunk_order(const GeomDev &g, int32_t n_maps, int64_t n_pix, const float *I,
+                                 float *Ich, hipStream_t st) {
+    hipLaunchKernelGGL(to_chunk_order_kernel, dim3(blocks_for((int64_t)n_maps * g.n_pad, 256)),
+                       dim3(256), 0, st, g, n_maps, n_pix, I, Ich);
+    return hipGetLastError();
+}
+
+hipError_t launch_visgen(const GeomDev &g, const PhaseKnobs &k, bool f64_phase, int32_t n_maps,
+                         const float *Ich, float2 *V, int grid, hipStream_t st) {
+    if (f64_phase)
+        hipLaunchKernelGGL(visgen_kernel<true>, dim3(grid), dim3(kBlock), 0, st, g, k, n_maps,
+                           Ich, V);
+    else
+        hipLaunchKernelGGL(visgen_kernel<false>, dim3(grid), dim3(kBlock), 0, st, g, k, n_maps,
+                           Ich, V);
+    return hipGetLastError();
+}
+
+hipError_t launch_recon(const GeomDev &g, const PhaseKnobs &k, bool f64_phase, int32_t n_maps,
+                        const float2 *V, const float *w, float kappa, int32_t n_seg, float *part,
+                        int grid, hipStream_t st) {
+    if (f64_phase)
+        hipLaunchKernelGGL(recon_kernel<true>, dim3(grid), dim3(kBlock), 0, st, g, k, n_maps, V,
+                           w, kappa, n_seg, part);
+    else
+        hipLaunchKernelGGL(recon_kernel<false>, dim3(grid), dim3(kBlock), 0, st, g, k, n_maps, V,
+                           w, kappa, n_seg, part);
+    return hipGetLastError();
+}
+
+hipError_t launch_recon_finalize(const GeomDev &g, int32_t n_maps, int32_t n_seg, int64_t n_pix,
+                                 const float *part, float *F, hipStream_t st) {
+    hipLaunchKernelGGL(recon_finalize_kernel, dim3(blocks_for((int64_t)n_maps * g.n_pad, 256)),
+                       dim3(256), 0, st, g, n_maps, n_seg, n_pix, part, F);
+    return hipGetLastError();
+}
+
+int visgen_tile_vis() { return kVisTile; }
+int recon_stage_vis() { return kReconStage; }
+const void *visgen_kernel_ptr(bool f64) {
+    return f64 ? (const void *)visgen_kernel<true> : (const void *)visgen_kernel<false>;
+}
+const void *recon_kernel_ptr(bool f64) {
+    return f64 ? (const void *)recon_kernel<true> : (const void *)recon_kernel<false>;
+}
+const void *shade_kernel_ptr() { return (const void *)shade_kernel; }
+
+}  // namespace vrs

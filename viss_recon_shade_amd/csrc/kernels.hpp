@@ -1,0 +1,70 @@
+// Launch interface of the gfx950 kernels (kernels.hip).  Host code in capi.hip
+// validates every argument before calling these.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "vrs_internal.hpp"
+
+namespace vrs {
+
+// Device view of one geometry; passed to kernels by value.
+struct GeomDev {
+    int32_t n_sat, n_t, n_base, n_chunk;
+    int64_t n_vis, n_pad;
+    const double4 *pos;    // [n_sat * n_t]  (x, y, z, |r|^2)
+    const CapRec *cap;     // [n_sat * n_t]
+    const double4 *uvw64;  // [n_vis]
+    const float4 *uvw32;   // [n_vis]
+    const int2 *pairs;     // [n_base]
+    const float4 *pix32;   // [n_pad]
+    const double4 *pix64;  // [n_pad]
+    const int32_t *ring;   // [n_pad]
+    const ChunkRec *chunk; // [n_chunk]
+    LimbConsts limb;
+};
+
+struct OrbitArgs {
+    int32_t n_sat, n_t;
+    double dt, a, cos_i, sin_i, n_mot, r_moon, inv_lambda_km;
+    const double *raan, *m0;  // device [n_sat]
+};
+
+struct PhaseKnobs {
+    double s;       // phase scale
+    int32_t wfac;   // 1: use n (trailing exp(-2 pi i s w) folded in), 0: n - 1
+    float t_occ;
+};
+
+// Stage 1: positions, caps, uvw.
+hipError_t launch_orbit(const OrbitArgs &a, const GeomDev &g, double4 *pos, CapRec *cap,
+                        double4 *uvw64, float4 *uvw32, hipStream_t st);
+
+// Stage 2: per-visibility shade statistics.  bits may be null; tot = [shaded, near, fp64].
+hipError_t launch_shade(const GeomDev &g, int32_t *count, uint32_t *hash, uint32_t *bits,
+                        int64_t n_words, unsigned long long *tot, int grid, hipStream_t st);
+
+// Gather RING-ordered maps into chunk order (zero padding).
+hipError_t launch_to_chunk_order(const GeomDev &g, int32_t n_maps, int64_t n_pix,
+                                 const float *I, float *Ich, hipStream_t st);
+
+// Stage 3.  Ich: [n_maps][n_pad]; V: [n_maps][n_vis] complex64.
+hipError_t launch_visgen(const GeomDev &g, const PhaseKnobs &k, bool f64_phase, int32_t n_maps,
+                         const float *Ich, float2 *V, int grid, hipStream_t st);
+
+// Stage 4.  V: [n_maps][n_vis] complex64; w: [n_vis] or null; part: [n_maps][n_seg][n_pad].
+hipError_t launch_recon(const GeomDev &g, const PhaseKnobs &k, bool f64_phase, int32_t n_maps,
+                        const float2 *V, const float *w, float kappa, int32_t n_seg,
+                        float *part, int grid, hipStream_t st);
+// F[map][ring[p]] = sum_seg part[map][seg][p].
+hipError_t launch_recon_finalize(const GeomDev &g, int32_t n_maps, int32_t n_seg, int64_t n_pix,
+                                 const float *part, float *F, hipStream_t st);
+
+int visgen_tile_vis();           // visibilities per vis-gen tile
+int recon_stage_vis();           // visibilities staged per recon LDS tile
+const void *visgen_kernel_ptr(bool f64_phase);
+const void *recon_kernel_ptr(bool f64_phase);
+const void *shade_kernel_ptr();
+
+}  // namespace vrs
