@@ -120,6 +120,7 @@ double vrs_timing_last_ms(vrs_ctx *ctx, const char *kernel);
  * chunks.  Replaces MIncline() (MIncline_generate_uvw_float.cu:184-440) up to the
  * uvw it writes, and the l,m,n inputs of vissGen (vissGen10M_shade.cu:199-248). */
 vrs_status vrs_geom_create(vrs_ctx *ctx, const vrs_geom_desc *desc, vrs_geom **out);
+/* Destroy a geometry; call before destroying the context that created it. */
 void vrs_geom_destroy(vrs_geom *g);
 vrs_status vrs_geom_info_get(const vrs_geom *g, vrs_geom_info *out);
 /* Re-runs stage 1 on the device (calculateOrbit + computeUVW,

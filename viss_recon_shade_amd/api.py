@@ -8,6 +8,7 @@ numpy arrays (host) or torch CUDA tensors (device, used in place on torch's
 current stream); outputs follow the input's kind.
 """
 import ctypes as C
+import weakref
 
 import numpy as np
 
@@ -66,6 +67,7 @@ class Context:
         self.handle = h
         self.device = device
         self._stream = None
+        self._geoms = weakref.WeakSet()
 
     def _sync_stream(self, *arrays):
         """Run on torch's current stream when any argument is a torch tensor."""
@@ -98,6 +100,8 @@ class Context:
 
     def close(self):
         if self.handle:
+            for g in list(self._geoms):   # geometries first: they live on this context
+                g.close()
             self._lib.vrs_destroy(self.handle)
             self.handle = None
 
@@ -143,6 +147,7 @@ class Geometry:
         h = C.c_void_p()
         ctx.check(lib.vrs_geom_create(ctx.handle, C.byref(d), C.byref(h)))
         self.handle = h
+        ctx._geoms.add(self)
         info = _lib.GeomInfo()
         ctx.check(lib.vrs_geom_info_get(h, C.byref(info)))
         self.n_vis, self.n_pix = info.n_vis, info.n_pix
@@ -233,7 +238,8 @@ class Geometry:
 
     def close(self):
         if self.handle:
-            self.ctx._lib.vrs_geom_destroy(self.handle)
+            if self.ctx.handle:
+                self.ctx._lib.vrs_geom_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

@@ -40,7 +40,8 @@ struct vrs_ctx {
 };
 
 struct vrs_geom {
-    vrs_ctx *ctx = nullptr;
+    vrs_ctx *ctx = nullptr;  // owning context; must outlive the geometry
+    int device = 0;
     int32_t n_sat = 0, n_t = 0, n_base = 0, nside = 0, n_chunk = 0;
     int64_t n_vis = 0, n_pix = 0, n_pad = 0;
     double lambda_m = 0, r_moon = 0;
@@ -401,8 +402,8 @@ vrs_status vrs_healpix_nest2ring(int32_t nside, int64_t *out) {
 // ---------------------------------------------------------------- geometry
 void vrs_geom_destroy(vrs_geom *g) {
     if (!g) return;
-    DeviceGuard dg(g->ctx ? g->ctx->device : 0);
-    if (g->ctx) (void)hipStreamSynchronize(g->ctx->stream);
+    DeviceGuard dg(g->device);
+    (void)hipDeviceSynchronize();
     void *ptrs[] = {g->d_orbit, g->d_pos,  g->d_cap,  g->d_uvw64, g->d_uvw32, g->d_pairs,
                     g->d_pix32, g->d_pix64, g->d_ring, g->d_chunk};
     for (void *p : ptrs)
@@ -568,6 +569,7 @@ vrs_status vrs_geom_create(vrs_ctx *ctx, const vrs_geom_desc *d, vrs_geom **out)
     DeviceGuard dg(ctx->device);
     vrs_geom *g = new vrs_geom;
     g->ctx = ctx;
+    g->device = ctx->device;
     vrs_status s = geom_build(ctx, d, g);
     if (s == VRS_OK) s = vrs_orbit_uvw(ctx, g, nullptr, nullptr);
     if (s == VRS_OK) {
@@ -716,8 +718,9 @@ static vrs_status recon_common(vrs_ctx *ctx, const vrs_geom *g, const vrs_knobs 
     if (vrs_status s = sF.prepare()) return s;
     // segments of the visibility range: enough work items to fill the chip,
     // a multiple of 8 so that each XCD keeps to its own segments
+    // (independent of n_maps, so that each map's sums are grouped the same way)
     const int64_t units = (int64_t)n_maps * g->n_chunk;
-    int64_t n_seg = (4096 + units - 1) / units;
+    int64_t n_seg = (4096 + g->n_chunk - 1) / g->n_chunk;
     n_seg = std::min<int64_t>(n_seg, std::max<int64_t>(1, g->n_vis / 256));
     if (n_seg > 8) n_seg = (n_seg + 7) / 8 * 8;
     n_seg = std::max<int64_t>(1, std::min<int64_t>(n_seg, 1024));

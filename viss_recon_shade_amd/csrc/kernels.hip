@@ -38,19 +38,24 @@ __device__ __forceinline__ uint32_t wave_xor(uint32_t x) {
     return x;
 }
 
-// Classify a chunk's bounding cap against the caps of satellites a and b.
-__device__ __forceinline__ int classify(const ChunkRec &ch, const CapRec &a, const CapRec &b) {
+// Classify a chunk's bounding cap against one satellite's shade cap:
+// kIn = every pixel is inside by more than the guard band, kOut = every pixel
+// is outside by more than the guard band, kStraddle otherwise.
+__device__ __forceinline__ int classify_cap(const ChunkRec &ch, const CapRec &a) {
     if (ch.flags & kChunkAlwaysStraddle) return kStraddle;
-    const float xa = fmaf(ch.gz, a.dz, fmaf(ch.gy, a.dy, ch.gx * a.dx));
-    const float xb = fmaf(ch.gz, b.dz, fmaf(ch.gy, b.dy, ch.gx * b.dx));
-    const float ma = a.g + kChunkMargin, mb = b.g + kChunkMargin;
-    // all pixels within angle alpha - margin  <=  x >= cos(alpha - rho) (+ margin)
-    const bool in_a = (xa >= fmaf(a.c, ch.crho, a.sa * ch.srho) + ma) && (a.c < ch.crho);
-    const bool in_b = (xb >= fmaf(b.c, ch.crho, b.sa * ch.srho) + mb) && (b.c < ch.crho);
-    // all pixels beyond angle alpha + margin  <=  x < cos(alpha + rho) (- margin)
-    const bool out_a = xa < fmaf(a.c, ch.crho, -a.sa * ch.srho) - ma;
-    const bool out_b = xb < fmaf(b.c, ch.crho, -b.sa * ch.srho) - mb;
-    return (in_a || in_b) ? kIn : ((out_a && out_b) ? kOut : kStraddle);
+    const float x = fmaf(ch.gz, a.dz, fmaf(ch.gy, a.dy, ch.gx * a.dx));
+    const float mg = a.g + kChunkMargin;
+    // all pixels within angle alpha - rho of the cap centre: x >= cos(alpha - rho)
+    if ((x >= fmaf(a.c, ch.crho, a.sa * ch.srho) + mg) && (a.c < ch.crho)) return kIn;
+    // all pixels beyond angle alpha + rho: x < cos(alpha + rho)
+    if (x < fmaf(a.c, ch.crho, -a.sa * ch.srho) - mg) return kOut;
+    return kStraddle;
+}
+
+// Classify a chunk against the pair: shaded by either satellite, by neither, or mixed.
+__device__ __forceinline__ int classify(const ChunkRec &ch, const CapRec &a, const CapRec &b) {
+    const int ca = classify_cap(ch, a), cb = classify_cap(ch, b);
+    return (ca == kIn || cb == kIn) ? kIn : ((ca == kOut && cb == kOut) ? kOut : kStraddle);
 }
 
 // FP32 cap test; flags pixels inside the guard band.
@@ -153,42 +158,49 @@ __global__ void __launch_bounds__(kBlock) shade_kernel(GeomDev g, int32_t *__res
         for (int c0 = 0; c0 < g.n_chunk; c0 += kWave) {
             const int c = c0 + lane;
             int cls = kOut;
+            bool limb = false;  // the chunk straddles either satellite's limb
             if (c < g.n_chunk) {
                 const ChunkRec ch = g.chunk[c];
-                cls = classify(ch, A, B);
+                const int ca = classify_cap(ch, A), cb = classify_cap(ch, B);
+                cls = (ca == kIn || cb == kIn) ? kIn : ((ca == kOut && cb == kOut) ? kOut : kStraddle);
+                limb = (ca == kStraddle) || (cb == kStraddle);
                 if (cls == kIn) {
                     cnt += ch.n_valid;
                     h ^= ch.hash_all;
                 }
             }
-            uint64_t todo = __ballot(cls == kStraddle);
+            uint64_t todo = __ballot(limb);
             if (bits) todo |= __ballot(cls == kIn);
             while (todo) {
                 const int cs = __ffsll((unsigned long long)todo) - 1;
                 todo &= todo - 1;
                 const int cc = c0 + cs;
-                const int whole = __shfl(cls, cs, kWave) == kIn;
+                const bool whole = __shfl(cls, cs, kWave) == kIn;
+                const bool test = __shfl((int)limb, cs, kWave) != 0;
 #pragma unroll
                 for (int j = 0; j < kPixPerLane; ++j) {
                     const int64_t pi = (int64_t)cc * kChunk + j * kWave + lane;
                     const int rp = g.ring[pi];
                     if (rp < 0) continue;
-                    bool m;
-                    if (whole) {
-                        m = true;  // counted above
-                    } else {
+                    bool m = whole;  // wholly shaded chunks were counted above
+                    if (test) {
+                        // near-limb pixels are counted for either satellite's limb,
+                        // also inside chunks that the other satellite shades wholly
                         const float4 s = g.pix32[pi];
                         bool unc = false;
-                        m = occ32(s.x, s.y, s.z, A, unc) | occ32(s.x, s.y, s.z, B, unc);
+                        bool m32 = occ32(s.x, s.y, s.z, A, unc) | occ32(s.x, s.y, s.z, B, unc);
                         if (unc) {
                             int nl = 0;
-                            m = occ_pair64(g.pos, g.pix64, pi, ia, ib, g.limb, &nl);
+                            m32 = occ_pair64(g.pos, g.pix64, pi, ia, ib, g.limb, &nl);
                             near += nl;
                             ++f64;
                         }
-                        if (m) {
-                            ++cnt;
-                            h ^= pix_hash_u64((uint64_t)rp);
+                        if (!whole) {
+                            m = m32;
+                            if (m) {
+                                ++cnt;
+                                h ^= pix_hash_u64((uint64_t)rp);
+                            }
                         }
                     }
                     if (m && bits)
@@ -279,7 +291,7 @@ __global__ void __launch_bounds__(kBlock) visgen_kernel(GeomDev g, PhaseKnobs k,
     using T = typename PhaseT<F64>::T;
     __shared__ VisEnt<F64> ents[kVisTile];
     __shared__ float red[kWavesPerBlock][kVisTile][2];
-    __shared__ float corr[kVisTile][2];  // FP64 guard-band corrections
+    __shared__ float corr[kWavesPerBlock][kVisTile][2];  // straddling pairs, per wave
     constexpr uint32_t kTileMask = (kVisTile == 32) ? 0xffffffffu : ((1u << kVisTile) - 1u);
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x >> 6;
@@ -292,7 +304,7 @@ __global__ void __launch_bounds__(kBlock) visgen_kernel(GeomDev g, PhaseKnobs k,
         const int64_t v0 = (tile % n_vtiles) * kVisTile;
         const float *__restrict__ Im = Ich + map * g.n_pad;
         if (threadIdx.x < kVisTile) fill_vis_ent<F64>(g, k, v0 + threadIdx.x, ents[threadIdx.x]);
-        if (threadIdx.x < 2 * kVisTile) (&corr[0][0])[threadIdx.x] = 0.0f;
+        if (threadIdx.x < kWavesPerBlock * kVisTile * 2) (&corr[0][0][0])[threadIdx.x] = 0.0f;
         __syncthreads();
 
         float are[kVisTile], aim[kVisTile];
@@ -374,9 +386,9 @@ __global__ void __launch_bounds__(kBlock) visgen_kernel(GeomDev g, PhaseKnobs k,
                 }
                 dre = wave_sum(dre);
                 dim = wave_sum(dim);
-                if (lane == 0) {
-                    atomicAdd(&corr[q][0], dre);
-                    atomicAdd(&corr[q][1], dim);
+                if (lane == 0) {  // only this wave touches corr[wave]: deterministic order
+                    corr[wave][q][0] += dre;
+                    corr[wave][q][1] += dim;
                 }
             }
         }
@@ -394,11 +406,11 @@ __global__ void __launch_bounds__(kBlock) visgen_kernel(GeomDev g, PhaseKnobs k,
             const int q = threadIdx.x;
             const int64_t v = v0 + q;
             if (v < g.n_vis) {
-                float sr = corr[q][0], si = corr[q][1];
+                float sr = 0.0f, si = 0.0f;
 #pragma unroll
                 for (int w = 0; w < kWavesPerBlock; ++w) {
-                    sr += red[w][q][0];
-                    si += red[w][q][1];
+                    sr += red[w][q][0] + corr[w][q][0];
+                    si += red[w][q][1] + corr[w][q][1];
                 }
                 // exp(-i theta): V = sum I (cos - i sin)
                 V[map * g.n_vis + v] = make_float2(sr, -si);
