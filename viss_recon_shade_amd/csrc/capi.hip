@@ -15,6 +15,7 @@
 
 #include "host_sky.hpp"
 #include "kernels.hpp"
+#include "mfma.hpp"
 #include "vrs.h"
 #include "vrs_internal.hpp"
 
@@ -192,9 +193,9 @@ struct Timer {
     ~Timer() { stop(); }
 };
 
-int persistent_grid(vrs_ctx *ctx, const void *kernel, int64_t n_items) {
+int persistent_grid(vrs_ctx *ctx, const void *kernel, int64_t n_items, int block = kBlock) {
     int per_cu = 1;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, 0) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0) != hipSuccess ||
         per_cu < 1)
         per_cu = 1;
     int64_t cap = (int64_t)per_cu * ctx->n_cu;
@@ -682,13 +683,26 @@ vrs_status vrs_visgen(vrs_ctx *ctx, const vrs_geom *g, const vrs_knobs *k, int32
     if (vrs_status s = check_knobs(ctx, k)) return s;
     if (n_maps < 1 || n_maps > 65536) return fail(ctx, VRS_E_ARG, "n_maps must be in [1, 65536]");
     if (!I || !V) return fail(ctx, VRS_E_ARG, "I and V must be non-NULL");
-    if (k->precision == VRS_MFMA_SPLIT16)
-        return fail(ctx, VRS_E_ARG, "VRS_MFMA_SPLIT16 vis-gen is not available in this build");
     DeviceGuard dg(ctx->device);
     Staged sI(ctx, I, sizeof(float) * n_maps * g->n_pix, false);
     Staged sV(ctx, V, sizeof(float) * 2 * n_maps * g->n_vis, true);
     if (vrs_status s = sI.prepare()) return s;
     if (vrs_status s = sV.prepare()) return s;
+    if (k->precision == VRS_MFMA_SPLIT16) {
+        void *a16, *small;
+        if (vrs_status s = ws_get(ctx, 0, mfma_a16_bytes_visgen(n_maps, g->n_pad), &a16)) return s;
+        if (vrs_status s = ws_get(ctx, 3, 8 * (size_t)n_maps + 256, &small)) return s;
+        const int64_t groups = (n_maps + 63) / 64;
+        const int64_t n_tiles = groups * ((g->n_vis + mfma_vis_per_block() - 1) / mfma_vis_per_block());
+        const int grid = persistent_grid(ctx, mfma_visgen_kernel_ptr(), n_tiles, mfma_block_threads());
+        Timer tm(ctx, "mfma_visgen");
+        VRS_HIP(ctx, launch_mfma_visgen(g->dev(), phase_knobs(k), n_maps, g->n_pix,
+                                        (const float *)sI.dev, (unsigned int *)small,
+                                        (float *)small + n_maps, a16, (float2 *)sV.dev, grid,
+                                        ctx->stream));
+        tm.stop();
+        return sV.finish();
+    }
     void *ich;
     if (vrs_status s = ws_get(ctx, 0, sizeof(float) * n_maps * g->n_pad, &ich)) return s;
     const GeomDev gd = g->dev();
@@ -716,6 +730,32 @@ static vrs_status recon_common(vrs_ctx *ctx, const vrs_geom *g, const vrs_knobs 
     if (vrs_status s = sV.prepare()) return s;
     if (vrs_status s = sw.prepare()) return s;
     if (vrs_status s = sF.prepare()) return s;
+    if (k->precision == VRS_MFMA_SPLIT16) {
+        int64_t n_vis_pad;
+        mfma_vis_pad(g->n_vis, &n_vis_pad);
+        const int64_t groups = (n_maps + 63) / 64;
+        const int64_t ptiles = g->n_pad / mfma_pix_per_block();
+        int64_t n_seg = (2048 + ptiles * groups - 1) / (ptiles * groups);
+        n_seg = std::max<int64_t>(1, std::min<int64_t>(n_seg, n_vis_pad / 1024));
+        if (n_seg > 8) n_seg = (n_seg + 7) / 8 * 8;
+        void *a16, *small, *part;
+        if (vrs_status s = ws_get(ctx, 2, mfma_a16_bytes_recon(n_maps, n_vis_pad), &a16)) return s;
+        if (vrs_status s = ws_get(ctx, 3, 8 * (size_t)n_maps + 256, &small)) return s;
+        if (vrs_status s = ws_get(ctx, 1, sizeof(float) * n_seg * n_maps * g->n_pad, &part)) return s;
+        const int grid = persistent_grid(ctx, mfma_recon_kernel_ptr(), ptiles * groups * n_seg,
+                                         mfma_block_threads());
+        const GeomDev gd = g->dev();
+        {
+            Timer tm(ctx, "mfma_recon");
+            VRS_HIP(ctx, launch_mfma_recon(gd, phase_knobs(k), n_maps, (const float2 *)sV.dev,
+                                           (const float *)sw.dev, (float)kappa, n_vis_pad,
+                                           (unsigned int *)small, (float *)small + n_maps, a16,
+                                           (int32_t)n_seg, (float *)part, grid, ctx->stream));
+        }
+        VRS_HIP(ctx, launch_recon_finalize(gd, n_maps, (int32_t)n_seg, g->n_pix,
+                                           (const float *)part, (float *)sF.dev, ctx->stream));
+        return sF.finish();
+    }
     // segments of the visibility range: enough work items to fill the chip,
     // a multiple of 8 so that each XCD keeps to its own segments
     // (independent of n_maps, so that each map's sums are grouped the same way)
@@ -748,8 +788,6 @@ vrs_status vrs_recon(vrs_ctx *ctx, const vrs_geom *g, const vrs_knobs *k, int32_
     if (vrs_status s = check_knobs(ctx, k)) return s;
     if (n_maps < 1 || n_maps > 65536) return fail(ctx, VRS_E_ARG, "n_maps must be in [1, 65536]");
     if (!V || !F) return fail(ctx, VRS_E_ARG, "V and F must be non-NULL");
-    if (k->precision == VRS_MFMA_SPLIT16)
-        return fail(ctx, VRS_E_ARG, "VRS_MFMA_SPLIT16 recon is not available in this build");
     const double kappa = k->kappa > 0 ? k->kappa : 1.0 / (double)g->n_vis;
     return recon_common(ctx, g, k, n_maps, V, w, F, kappa, "recon");
 }

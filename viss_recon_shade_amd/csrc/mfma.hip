@@ -1,0 +1,508 @@
+// Multi-map vis-gen and recon as real contractions on the gfx950 matrix cores.
+//
+// With n_maps sky maps the direct DFT is a GEMM whose middle operand, the
+// phase-factor matrix E[v][p] = exp(-2 pi i s phi_vp) with the shade mask
+// folded in, is generated on the fly:
+//   vis-gen  V[m][v] = sum_p I[m][p] E'[v][p] + t_occ sum_{p shaded} E[v][p]
+//            (E' = E with shaded entries zeroed; the second term is
+//             map-independent and accumulated on the VALU side)
+//   recon    F[m][p] = sum_v (Vr w kappa)[m][v] Re E* - (Vi w kappa)[m][v] Im E*
+//            with shaded (v, p) entries zeroed.
+// Each lane generates the E entries its MFMA B operand needs (FP64 phase and
+// fract, FP32 sin/cos), splits them into FP16 hi + lo and issues three
+// products per tile (hi*hi + hi*lo + lo*hi) into FP32 accumulators.  The map
+// operand (I, or the weighted V) is pre-split the same way, scaled per map by
+// a power of two into FP16 range, and staged through LDS for all waves of the
+// block.  Instruction: v_mfma_f32_32x32x16_f16 (M = 32 maps, N = 32, K = 16).
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+#include "mfma.hpp"
+
+namespace vrs {
+
+namespace {
+
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+constexpr int kMT = 32;            // rows of one M tile (maps)
+constexpr int kMaps = 64;          // maps per pass (two M tiles)
+constexpr int kNT = 32;            // columns per wave (visibilities or pixels)
+constexpr int kKS = 16;            // K per MFMA
+constexpr int kStage = 64;         // K per LDS stage
+constexpr int kWaves = 4;
+constexpr int kThreads = kWaves * 64;
+constexpr int kAPad = 8;           // halves of padding per LDS row (bank spread)
+constexpr int kARow = kStage + kAPad;
+
+static_assert(kChunk % kStage == 0, "stages must tile chunks");
+
+__device__ __forceinline__ floatx16 mfma(half8 a, half8 b, floatx16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+}
+
+// x = hi + lo with hi, lo in FP16 (hi rounded toward zero, lo the remainder).
+__device__ __forceinline__ void split2(float x0, float x1, uint32_t &hi, uint32_t &lo) {
+    const auto h = __builtin_amdgcn_cvt_pkrtz(x0, x1);  // 2 x f16
+    const float r0 = x0 - (float)h[0];
+    const float r1 = x1 - (float)h[1];
+    const auto l = __builtin_amdgcn_cvt_pkrtz(r0, r1);
+    hi = __builtin_bit_cast(uint32_t, h);
+    lo = __builtin_bit_cast(uint32_t, l);
+}
+
+union H8 {
+    half8 v;
+    uint32_t u[4];
+};
+
+__device__ __forceinline__ bool occ32m(float sx, float sy, float sz, const CapRec &a, bool &unc) {
+    const float x = fmaf(sz, a.dz, fmaf(sy, a.dy, sx * a.dx));
+    unc = unc || (fabsf(x - a.c) <= a.g);
+    return x >= a.c;
+}
+
+__device__ __forceinline__ bool occ64m(const double4 r, double sx, double sy, double sz,
+                                       const LimbConsts &L) {
+    const double d = fma(r.z, sz, fma(r.y, sy, r.x * sx));
+    if (!(d < 0.0)) return false;
+    const double q = fma(-d, d, r.w);
+    return q <= L.R2;
+}
+
+__device__ __forceinline__ int classify_cap_m(const ChunkRec &ch, const CapRec &a) {
+    if (ch.flags & kChunkAlwaysStraddle) return kStraddle;
+    const float x = fmaf(ch.gz, a.dz, fmaf(ch.gy, a.dy, ch.gx * a.dx));
+    const float mg = a.g + kChunkMargin;
+    if ((x >= fmaf(a.c, ch.crho, a.sa * ch.srho) + mg) && (a.c < ch.crho)) return kIn;
+    if (x < fmaf(a.c, ch.crho, -a.sa * ch.srho) - mg) return kOut;
+    return kStraddle;
+}
+
+__device__ __forceinline__ int classify_m(const ChunkRec &ch, const CapRec &a, const CapRec &b) {
+    const int ca = classify_cap_m(ch, a), cb = classify_cap_m(ch, b);
+    return (ca == kIn || cb == kIn) ? kIn : ((ca == kOut && cb == kOut) ? kOut : kStraddle);
+}
+
+// Shade decision for one (visibility, pixel) pair inside a straddling chunk.
+__device__ __forceinline__ bool pair_shaded(const GeomDev &g, const double4 s64,
+                                            const CapRec &A, const CapRec &B, int ia, int ib) {
+    const float sx = (float)s64.x, sy = (float)s64.y, sz = (float)s64.z;
+    bool unc = false;
+    bool sh = occ32m(sx, sy, sz, A, unc) | occ32m(sx, sy, sz, B, unc);
+    if (unc)
+        sh = occ64m(g.pos[ia], s64.x, s64.y, s64.z, g.limb) |
+             occ64m(g.pos[ib], s64.x, s64.y, s64.z, g.limb);
+    return sh;
+}
+
+// ------------------------------------------------------------------ prep
+__global__ void zero_u32_kernel(unsigned int *__restrict__ x, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = 0u;
+}
+
+// Per-row scale exponents: e[m] such that max |x[m][:]| * 2^e[m] <= 2^14.
+__global__ void __launch_bounds__(256) row_absmax_kernel(const float *__restrict__ x, int64_t n,
+                                                         int64_t stride_row, int complex_in,
+                                                         const float *__restrict__ w,
+                                                         unsigned int *__restrict__ amax) {
+    const int64_t row = blockIdx.y;
+    const float *xr = x + row * stride_row;
+    float m = 0.0f;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const float ww = w ? fabsf(w[i]) : 1.0f;
+        if (complex_in) {
+            m = fmaxf(m, fabsf(xr[2 * i]) * ww);
+            m = fmaxf(m, fabsf(xr[2 * i + 1]) * ww);
+        } else {
+            m = fmaxf(m, fabsf(xr[i]) * ww);
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(&amax[row], __float_as_uint(m));
+}
+
+__device__ __forceinline__ float row_scale(unsigned int amax_bits, float extra) {
+    const float m = __uint_as_float(amax_bits) * extra;
+    if (!(m > 0.0f) || !isfinite(m)) return 1.0f;
+    int e;
+    frexpf(m, &e);              // m = f 2^e, f in [0.5, 1)
+    return ldexpf(1.0f, 14 - e);  // m * scale < 2^14
+}
+
+// I [n_maps][n_pix] RING -> hi/lo FP16 [group][2][kMaps][n_pad] chunk order, scaled.
+__global__ void __launch_bounds__(256) split_maps_kernel(GeomDev g, int32_t n_maps, int64_t n_pix,
+                                                         const float *__restrict__ I,
+                                                         const unsigned int *__restrict__ amax,
+                                                         _Float16 *__restrict__ out,
+                                                         float *__restrict__ inv_scale) {
+    const int64_t n_groups = (n_maps + kMaps - 1) / kMaps;
+    const int64_t total = n_groups * kMaps * g.n_pad;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t pi = idx % g.n_pad;
+        const int64_t mrow = idx / g.n_pad;  // group * kMaps + row
+        const int64_t grp = mrow / kMaps, row = mrow % kMaps;
+        const int64_t m = grp * kMaps + row;
+        float x = 0.0f, sc = 1.0f;
+        if (m < n_maps) {
+            sc = row_scale(amax[m], 1.0f);
+            const int r = g.ring[pi];
+            if (r >= 0) x = I[m * n_pix + r] * sc;
+            if (pi == 0) inv_scale[m] = 1.0f / sc;
+        }
+        const _Float16 h = (_Float16)x;  // nearest
+        const _Float16 l = (_Float16)(x - (float)h);
+        out[((grp * 2 + 0) * kMaps + row) * g.n_pad + pi] = h;
+        out[((grp * 2 + 1) * kMaps + row) * g.n_pad + pi] = l;
+    }
+}
+
+// V [n_maps][n_vis] complex -> hi/lo FP16 [group][re,im][hi,lo][kMaps][n_vis_pad],
+// weighted by w * kappa and scaled per map.
+__global__ void __launch_bounds__(256) split_vis_kernel(int64_t n_vis, int64_t n_vis_pad,
+                                                        int32_t n_maps,
+                                                        const float2 *__restrict__ V,
+                                                        const float *__restrict__ w, float kappa,
+                                                        const unsigned int *__restrict__ amax,
+                                                        _Float16 *__restrict__ out,
+                                                        float *__restrict__ inv_scale) {
+    const int64_t n_groups = (n_maps + kMaps - 1) / kMaps;
+    const int64_t total = n_groups * kMaps * n_vis_pad;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t v = idx % n_vis_pad;
+        const int64_t mrow = idx / n_vis_pad;
+        const int64_t grp = mrow / kMaps, row = mrow % kMaps;
+        const int64_t m = grp * kMaps + row;
+        float xr = 0.0f, xi = 0.0f;
+        if (m < n_maps) {
+            const float sc = row_scale(amax[m], fabsf(kappa));
+            if (v < n_vis) {
+                const float2 q = V[m * n_vis + v];
+                const float ww = (w ? w[v] : 1.0f) * kappa * sc;
+                xr = q.x * ww;
+                xi = q.y * ww;
+            }
+            if (v == 0) inv_scale[m] = 1.0f / sc;
+        }
+        const _Float16 hr = (_Float16)xr, hi_ = (_Float16)xi;
+        const _Float16 lr = (_Float16)(xr - (float)hr), li = (_Float16)(xi - (float)hi_);
+        const int64_t base = grp * 4 * kMaps;
+        out[((base + 0 * kMaps) + row) * n_vis_pad + v] = hr;  // re hi
+        out[((base + 1 * kMaps) + row) * n_vis_pad + v] = lr;  // re lo
+        out[((base + 2 * kMaps) + row) * n_vis_pad + v] = hi_; // im hi
+        out[((base + 3 * kMaps) + row) * n_vis_pad + v] = li;  // im lo
+    }
+}
+
+// Output row of accumulator element r for this lane in a 32x32 tile.
+__device__ __forceinline__ int acc_row(int r, int lane) {
+    return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+}
+
+// ------------------------------------------------------------------ vis-gen
+// Block: 4 waves x 32 visibilities; all waves share the staged map tile.
+__global__ void __launch_bounds__(kThreads, 2) mfma_visgen_kernel(
+    GeomDev g, PhaseKnobs k, int32_t n_maps, const _Float16 *__restrict__ A16,
+    const float *__restrict__ inv_scale, float2 *__restrict__ V) {
+    __shared__ _Float16 sA[2][kMaps][kARow];   // [hi/lo][map][pixel]
+    __shared__ double4 sP[kStage];             // pixel (l, m, n) FP64
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int col = lane & 31;                 // this lane's visibility within the wave tile
+    const int kg = lane >> 5;                  // which 8 of the 16 K values this lane supplies
+    const int64_t n_vtiles = (g.n_vis + kWaves * kNT - 1) / (kWaves * kNT);
+    const int64_t n_groups = (n_maps + kMaps - 1) / kMaps;
+    const int64_t n_tiles = n_vtiles * n_groups;
+    const float t_occ = k.t_occ;
+
+    for (int64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+        const int64_t grp = tile / n_vtiles;
+        const int64_t v = (tile % n_vtiles) * (kWaves * kNT) + wave * kNT + col;
+        const bool vok = v < g.n_vis;
+        // this lane's visibility
+        double u = 0.0, vv = 0.0, w = 0.0;
+        int ia = 0, ib = 0;
+        CapRec A = {0.f, 0.f, 0.f, 2.0f, 0.f, 0.f, 0.f, 0.f}, B = A;
+        if (vok) {
+            const int b = (int)(v / g.n_t), t = (int)(v % g.n_t);
+            const int2 pr = g.pairs[b];
+            ia = pr.x * g.n_t + t;
+            ib = pr.y * g.n_t + t;
+            A = g.cap[ia];
+            B = g.cap[ib];
+            const double4 q = g.uvw64[v];
+            u = q.x * k.s; vv = q.y * k.s; w = q.z * k.s;
+        }
+        const double wo = k.wfac ? 0.0 : -w;
+        floatx16 acc[2][2];  // [m tile][re, im]
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int c = 0; c < 2; ++c) acc[a][c] = floatx16{};
+        float gre = 0.0f, gim = 0.0f;  // sum of E over shaded pixels
+        int cls = kOut;
+
+        const _Float16 *Ag = A16 + grp * 2 * kMaps * g.n_pad;
+        for (int64_t p0 = 0; p0 < g.n_pad; p0 += kStage) {
+            __syncthreads();
+            // stage the map tile (2 x 64 maps x 64 pixels) and the pixel vectors
+            for (int i = threadIdx.x; i < 2 * kMaps * (kStage / 8); i += kThreads) {
+                const int hl = i / (kMaps * (kStage / 8));
+                const int rem = i % (kMaps * (kStage / 8));
+                const int row = rem / (kStage / 8), seg = rem % (kStage / 8);
+                const uint4 val = *reinterpret_cast<const uint4 *>(
+                    Ag + ((int64_t)hl * kMaps + row) * g.n_pad + p0 + seg * 8);
+                *reinterpret_cast<uint4 *>(&sA[hl][row][seg * 8]) = val;
+            }
+            if (threadIdx.x < kStage) sP[threadIdx.x] = g.pix64[p0 + threadIdx.x];
+            if ((p0 % kChunk) == 0) cls = classify_m(g.chunk[p0 / kChunk], A, B);
+            __syncthreads();
+#pragma unroll
+            for (int ks = 0; ks < kStage / kKS; ++ks) {
+                H8 bre_h, bre_l, bim_h, bim_l;
+#pragma unroll
+                for (int t2 = 0; t2 < 4; ++t2) {
+                    float cs[2], sn[2];
+#pragma unroll
+                    for (int e = 0; e < 2; ++e) {
+                        const int kk = ks * kKS + kg * 8 + t2 * 2 + e;
+                        const double4 s = sP[kk];
+                        const double ph = fma(u, s.x, fma(vv, s.y, fma(w, s.z, wo)));
+                        const float r = (float)__builtin_amdgcn_fract(ph);
+                        float c = __builtin_amdgcn_cosf(r);
+                        float sv = __builtin_amdgcn_sinf(r);
+                        bool sh = (cls == kIn);
+                        if (cls == kStraddle) sh = pair_shaded(g, s, A, B, ia, ib);
+                        if (sh) {
+                            gre += c;
+                            gim += sv;
+                            c = 0.0f;
+                            sv = 0.0f;
+                        }
+                        cs[e] = c;
+                        sn[e] = -sv;  // E = cos - i sin
+                    }
+                    split2(cs[0], cs[1], bre_h.u[t2], bre_l.u[t2]);
+                    split2(sn[0], sn[1], bim_h.u[t2], bim_l.u[t2]);
+                }
+#pragma unroll
+                for (int mt = 0; mt < 2; ++mt) {
+                    half8 ah, al;
+                    ah = *reinterpret_cast<const half8 *>(&sA[0][mt * kMT + col][ks * kKS + kg * 8]);
+                    al = *reinterpret_cast<const half8 *>(&sA[1][mt * kMT + col][ks * kKS + kg * 8]);
+                    acc[mt][0] = mfma(ah, bre_h.v, acc[mt][0]);
+                    acc[mt][0] = mfma(ah, bre_l.v, acc[mt][0]);
+                    acc[mt][0] = mfma(al, bre_h.v, acc[mt][0]);
+                    acc[mt][1] = mfma(ah, bim_h.v, acc[mt][1]);
+                    acc[mt][1] = mfma(ah, bim_l.v, acc[mt][1]);
+                    acc[mt][1] = mfma(al, bim_h.v, acc[mt][1]);
+                }
+            }
+        }
+        // shaded-pixel term: both K halves of this column
+        gre += __shfl_xor(gre, 32, 64);
+        gim += __shfl_xor(gim, 32, 64);
+        if (vok) {
+#pragma unroll
+            for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    const int64_t m = grp * kMaps + mt * kMT + acc_row(r, lane);
+                    if (m < n_maps) {
+                        const float is = inv_scale[m];
+                        V[m * g.n_vis + v] = make_float2(acc[mt][0][r] * is + t_occ * gre,
+                                                         acc[mt][1][r] * is - t_occ * gim);
+                    }
+                }
+        }
+    }
+}
+
+// ------------------------------------------------------------------ recon
+// Block: 4 waves x 32 pixels; K runs over visibilities, staged 64 at a time.
+__global__ void __launch_bounds__(kThreads, 2) mfma_recon_kernel(
+    GeomDev g, PhaseKnobs k, int32_t n_maps, int64_t n_vis_pad, const _Float16 *__restrict__ A16,
+    const float *__restrict__ inv_scale, int32_t n_seg, float *__restrict__ part) {
+    __shared__ _Float16 sA[4][kMaps][kARow];   // [re hi, re lo, im hi, im lo][map][vis]
+    struct VisS {
+        double u, v, w, wo;
+        CapRec a, b;
+        int ia, ib, cls, pad0;
+    };
+    __shared__ VisS sV[kStage];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int col = lane & 31;
+    const int kg = lane >> 5;
+    const int64_t n_ptiles = g.n_pad / (kWaves * kNT);  // n_pad is a multiple of 128
+    const int64_t n_groups = (n_maps + kMaps - 1) / kMaps;
+    const int64_t per_seg = (n_vis_pad + n_seg - 1) / n_seg;
+    const int64_t seg_len = (per_seg + kStage - 1) / kStage * kStage;
+    const int64_t n_items = n_ptiles * n_groups * n_seg;
+
+    for (int64_t item = blockIdx.x; item < n_items; item += gridDim.x) {
+        const int64_t seg = item % n_seg;
+        const int64_t rest = item / n_seg;
+        const int64_t pt = rest % n_ptiles;
+        const int64_t grp = rest / n_ptiles;
+        const int64_t pi = pt * (kWaves * kNT) + wave * kNT + col;  // chunk-order pixel
+        const double4 s = g.pix64[pi];
+        // the block's 128 pixels lie in one chunk (tiles are aligned to kChunk / 2)
+        const ChunkRec ch = g.chunk[(pt * (kWaves * kNT)) / kChunk];
+        floatx16 acc[2];
+        acc[0] = floatx16{};
+        acc[1] = floatx16{};
+        const _Float16 *Ag = A16 + grp * 4 * kMaps * n_vis_pad;
+        const int64_t vb = seg * seg_len;
+        const int64_t ve = (vb + seg_len < n_vis_pad) ? vb + seg_len : n_vis_pad;
+        for (int64_t v0 = vb; v0 < ve; v0 += kStage) {
+            __syncthreads();
+            for (int i = threadIdx.x; i < 4 * kMaps * (kStage / 8); i += kThreads) {
+                const int hl = i / (kMaps * (kStage / 8));
+                const int rem = i % (kMaps * (kStage / 8));
+                const int row = rem / (kStage / 8), sg = rem % (kStage / 8);
+                const uint4 val = *reinterpret_cast<const uint4 *>(
+                    Ag + ((int64_t)hl * kMaps + row) * n_vis_pad + v0 + sg * 8);
+                *reinterpret_cast<uint4 *>(&sA[hl][row][sg * 8]) = val;
+            }
+            if (threadIdx.x < kStage) {
+                const int64_t vv = v0 + threadIdx.x;
+                VisS &e = sV[threadIdx.x];
+                if (vv < g.n_vis) {
+                    const int b = (int)(vv / g.n_t), t = (int)(vv % g.n_t);
+                    const int2 pr = g.pairs[b];
+                    e.ia = pr.x * g.n_t + t;
+                    e.ib = pr.y * g.n_t + t;
+                    e.a = g.cap[e.ia];
+                    e.b = g.cap[e.ib];
+                    const double4 q = g.uvw64[vv];
+                    e.u = q.x * k.s; e.v = q.y * k.s; e.w = q.z * k.s;
+                    e.wo = k.wfac ? 0.0 : -e.w;
+                    e.cls = classify_m(ch, e.a, e.b);
+                } else {
+                    e.ia = e.ib = 0;
+                    const CapRec none = {0.f, 0.f, 0.f, 2.0f, 0.f, 0.f, 0.f, 0.f};
+                    e.a = none;
+                    e.b = none;
+                    e.u = e.v = e.w = e.wo = 0.0;
+                    e.cls = kIn;  // padding: contributes nothing
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int ks = 0; ks < kStage / kKS; ++ks) {
+                H8 bre_h, bre_l, bim_h, bim_l;
+#pragma unroll
+                for (int t2 = 0; t2 < 4; ++t2) {
+                    float cs[2], sn[2];
+#pragma unroll
+                    for (int e = 0; e < 2; ++e) {
+                        const VisS &q = sV[ks * kKS + kg * 8 + t2 * 2 + e];
+                        const int cls = q.cls;
+                        float c = 0.0f, sv = 0.0f;
+                        bool sh = (cls == kIn);
+                        if (cls == kStraddle) sh = pair_shaded(g, s, q.a, q.b, q.ia, q.ib);
+                        if (!sh) {
+                            const double ph = fma(q.u, s.x, fma(q.v, s.y, fma(q.w, s.z, q.wo)));
+                            const float r = (float)__builtin_amdgcn_fract(ph);
+                            c = __builtin_amdgcn_cosf(r);
+                            sv = __builtin_amdgcn_sinf(r);
+                        }
+                        cs[e] = c;
+                        sn[e] = -sv;  // Re(V e^{+i theta}) = Vr cos - Vi sin
+                    }
+                    split2(cs[0], cs[1], bre_h.u[t2], bre_l.u[t2]);
+                    split2(sn[0], sn[1], bim_h.u[t2], bim_l.u[t2]);
+                }
+#pragma unroll
+                for (int mt = 0; mt < 2; ++mt) {
+                    const int row = mt * kMT + col;
+                    const int kc = ks * kKS + kg * 8;
+                    const half8 arh = *reinterpret_cast<const half8 *>(&sA[0][row][kc]);
+                    const half8 arl = *reinterpret_cast<const half8 *>(&sA[1][row][kc]);
+                    const half8 aih = *reinterpret_cast<const half8 *>(&sA[2][row][kc]);
+                    const half8 ail = *reinterpret_cast<const half8 *>(&sA[3][row][kc]);
+                    acc[mt] = mfma(arh, bre_h.v, acc[mt]);
+                    acc[mt] = mfma(arh, bre_l.v, acc[mt]);
+                    acc[mt] = mfma(arl, bre_h.v, acc[mt]);
+                    acc[mt] = mfma(aih, bim_h.v, acc[mt]);
+                    acc[mt] = mfma(aih, bim_l.v, acc[mt]);
+                    acc[mt] = mfma(ail, bim_h.v, acc[mt]);
+                }
+            }
+        }
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int64_t m = grp * kMaps + mt * kMT + acc_row(r, lane);
+                if (m < n_maps)
+                    part[(m * n_seg + seg) * g.n_pad + pi] = acc[mt][r] * inv_scale[m];
+            }
+    }
+}
+
+}  // namespace
+
+int mfma_vis_pad(int64_t n_vis, int64_t *n_vis_pad) {
+    *n_vis_pad = (n_vis + kStage - 1) / kStage * kStage;
+    return 0;
+}
+
+size_t mfma_a16_bytes_visgen(int32_t n_maps, int64_t n_pad) {
+    const int64_t groups = (n_maps + kMaps - 1) / kMaps;
+    return (size_t)groups * 2 * kMaps * n_pad * sizeof(_Float16);
+}
+
+size_t mfma_a16_bytes_recon(int32_t n_maps, int64_t n_vis_pad) {
+    const int64_t groups = (n_maps + kMaps - 1) / kMaps;
+    return (size_t)groups * 4 * kMaps * n_vis_pad * sizeof(_Float16);
+}
+
+hipError_t launch_mfma_visgen(const GeomDev &g, const PhaseKnobs &k, int32_t n_maps, int64_t n_pix,
+                              const float *I, unsigned int *amax, float *inv_scale, void *a16,
+                              float2 *V, int grid, hipStream_t st) {
+    hipLaunchKernelGGL(zero_u32_kernel, dim3((n_maps + 255) / 256), dim3(256), 0, st, amax, n_maps);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(row_absmax_kernel, dim3(64, n_maps), dim3(256), 0, st, I, n_pix, n_pix, 0,
+                       (const float *)nullptr, amax);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(split_maps_kernel, dim3(2048), dim3(256), 0, st, g, n_maps, n_pix, I, amax,
+                       (_Float16 *)a16, inv_scale);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(mfma_visgen_kernel, dim3(grid), dim3(kThreads), 0, st, g, k, n_maps,
+                       (const _Float16 *)a16, inv_scale, V);
+    return hipGetLastError();
+}
+
+hipError_t launch_mfma_recon(const GeomDev &g, const PhaseKnobs &k, int32_t n_maps,
+                             const float2 *V, const float *w, float kappa, int64_t n_vis_pad,
+                             unsigned int *amax, float *inv_scale, void *a16, int32_t n_seg,
+                             float *part, int grid, hipStream_t st) {
+    hipLaunchKernelGGL(zero_u32_kernel, dim3((n_maps + 255) / 256), dim3(256), 0, st, amax, n_maps);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(row_absmax_kernel, dim3(64, n_maps), dim3(256), 0, st,
+                       (const float *)V, g.n_vis, 2 * g.n_vis, 1, w, amax);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(split_vis_kernel, dim3(2048), dim3(256), 0, st, g.n_vis, n_vis_pad, n_maps,
+                       V, w, kappa, amax, (_Float16 *)a16, inv_scale);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(mfma_recon_kernel, dim3(grid), dim3(kThreads), 0, st, g, k, n_maps,
+                       n_vis_pad, (const _Float16 *)a16, inv_scale, n_seg, part);
+    return hipGetLastError();
+}
+
+const void *mfma_visgen_kernel_ptr() { return (const void *)mfma_visgen_kernel; }
+const void *mfma_recon_kernel_ptr() { return (const void *)mfma_recon_kernel; }
+int mfma_block_threads() { return kThreads; }
+int mfma_vis_per_block() { return kWaves * kNT; }
+int mfma_pix_per_block() { return kWaves * kNT; }
+
+}  // namespace vrs

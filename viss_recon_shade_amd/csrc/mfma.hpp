@@ -1,0 +1,33 @@
+// Launch interface of the split-FP16 matrix-core path (mfma.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "kernels.hpp"
+
+namespace vrs {
+
+int mfma_vis_pad(int64_t n_vis, int64_t *n_vis_pad);
+size_t mfma_a16_bytes_visgen(int32_t n_maps, int64_t n_pad);
+size_t mfma_a16_bytes_recon(int32_t n_maps, int64_t n_vis_pad);
+
+// I: [n_maps][n_pix] float (RING); amax/inv_scale: [n_maps]; a16: mfma_a16_bytes_visgen;
+// V: [n_maps][n_vis] complex64.
+hipError_t launch_mfma_visgen(const GeomDev &g, const PhaseKnobs &k, int32_t n_maps, int64_t n_pix,
+                              const float *I, unsigned int *amax, float *inv_scale, void *a16,
+                              float2 *V, int grid, hipStream_t st);
+// part: [n_maps][n_seg][n_pad] partial images, summed by launch_recon_finalize.
+hipError_t launch_mfma_recon(const GeomDev &g, const PhaseKnobs &k, int32_t n_maps,
+                             const float2 *V, const float *w, float kappa, int64_t n_vis_pad,
+                             unsigned int *amax, float *inv_scale, void *a16, int32_t n_seg,
+                             float *part, int grid, hipStream_t st);
+
+const void *mfma_visgen_kernel_ptr();
+const void *mfma_recon_kernel_ptr();
+int mfma_block_threads();
+int mfma_vis_per_block();
+int mfma_pix_per_block();
+
+}  // namespace vrs
