@@ -36,7 +36,8 @@ def parse():
     ap.add_argument("--workload", default="c2")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-accuracy", action="store_true")
-    ap.add_argument("--cpu-stride", type=int, default=128)
+    ap.add_argument("--cpu-stride", type=int, default=32)
+    ap.add_argument("--no-mfma", action="store_true", help="skip the multi-map matrix-core legs")
     return ap.parse_args()
 
 
@@ -82,6 +83,62 @@ def cpu_baseline(g, wl, pos, uvw, lmn, I, V, stride):
             "sample": f"every {stride}th visibility x all {wl.n_pix} pixels (vis-gen) + every "
                       f"{stride}th pixel x all {wl.n_vis} visibilities (recon): {terms:.3e} terms "
                       f"in {t2 - t0:.1f} s; FP32 OpenMP restatement of visscal/imagerecon"}
+
+
+def mfma_legs(torch, ctx, vrs, W, wl, g, pos, uvw, lmn, args):
+    """Multi-map contraction on the matrix cores: 64 maps on the same 10 MHz
+    geometry (Nside 128), and the 64-map Nside-64 precision configuration.
+    Terms are map x vis x pix per direction."""
+    import oracle as o
+    res = {}
+    km = vrs.knobs(precision="mfma")
+    for name, geom, nside in (("c2_geometry_64maps", g, wl.nside), ("c3", None, 64)):
+        gg = geom
+        wl3 = W.CONFIGS["c3"]
+        if gg is None:
+            gg = ctx.workload_geometry(wl3)
+        n_pix = 12 * nside * nside
+        I = W.make_sky(W.Workload("m", wl.n_sat, wl.n_t, wl.freq_hz, nside, 64, wl3.sky))
+        It = torch.from_numpy(I).cuda()
+        st = {}
+
+        def step():
+            gg.MIncline(export=False)
+            V = gg.visscal(It, km)
+            st["V"] = V
+            st["F"] = gg.imagerecon(V, None, km)
+
+        ms = timed_steps(torch, step, max(2, args.steps // 2), 1)
+        kt = kernel_avg_ms(ctx, step, ["mfma_visgen", "mfma_recon"], 2)
+        terms = 2 * 64 * gg.n_vis * n_pix
+        entry = {"value": round(terms / (ms * 1e-3) / 1e9, 1), "unit": "Gterms/s (map x vis x pix)",
+                 "ms_per_step": round(ms, 3), "kernels_ms": {k: round(v, 3) for k, v in kt.items()},
+                 "n_maps": 64, "nside": nside}
+        # MFMA roofline: 4 algorithmic flops per map-term (two real MACs), dense fp16 peak
+        mf = max(kt.values())
+        entry["roofline"] = {"bound": "mfma", "achieved": round(4 * 64 * gg.n_vis * n_pix / (mf * 1e-3) / 1e12, 2),
+                             "peak": 2500.0, "unit": "TFLOP/s"}
+        entry["roofline"]["frac"] = round(entry["roofline"]["achieved"] / 2500.0, 4)
+        if not args.no_accuracy:
+            V = st["V"].cpu().numpy()
+            F = st["F"].cpu().numpy()
+            p2, u2 = gg.MIncline()
+            l2 = gg.lmn()
+            sel = np.linspace(0, gg.n_vis - 1, 16).astype(np.int64)
+            maps = [0, 31, 63]
+            Vo = o.visgen(p2, gg.pairs, u2, l2, I[maps], gg.r_moon_km, vis_sel=sel)
+            Vc = V[maps][:, sel, 0].astype(np.float64) + 1j * V[maps][:, sel, 1]
+            psel = np.linspace(0, n_pix - 1, 4).astype(np.int64)
+            Vfull = V[maps][..., 0].astype(np.float64) + 1j * V[maps][..., 1]
+            Fo = o.recon(p2, gg.pairs, u2, l2, Vfull, gg.r_moon_km, pix_sel=psel)
+            entry["accuracy"] = {
+                "V_rel_l2_max": float((np.linalg.norm(Vc - Vo, axis=1) / np.linalg.norm(Vo, axis=1)).max()),
+                "F_rel_l2_max": float((np.linalg.norm(F[maps][:, psel] - Fo, axis=1) / np.linalg.norm(Fo, axis=1)).max()),
+                "sample": "maps 0, 31, 63; 16 visibilities x all pixels; 4 pixels x all visibilities"}
+        res[name] = entry
+        del It, st
+        torch.cuda.empty_cache()
+    return res
 
 
 def accuracy(g, wl, pos, uvw, lmn, I, V, F):
@@ -165,6 +222,8 @@ def main():
                            "ms_per_step": round(ms_mixed, 3),
                            "kernels_ms": {k: round(v, 3) for k, v in kt_mixed.items()}},
     }
+    if not args.no_mfma:
+        out["mfma_split16"] = mfma_legs(torch, ctx, vrs, W, wl, g, pos, uvw, lmn, args)
     if not args.no_accuracy:
         out["accuracy"] = {"plain": accuracy(g, wl, pos, uvw, lmn, I, Vp, Fp),
                            "mixed": accuracy(g, wl, pos, uvw, lmn, I, Vm, Fm)}

@@ -160,6 +160,48 @@ vrs_status vrs_recon(vrs_ctx *ctx, const vrs_geom *g, const vrs_knobs *k, int32_
 vrs_status vrs_adjoint(vrs_ctx *ctx, const vrs_geom *g, const vrs_knobs *k, const float *resid,
                        float *grad);
 
+/* ----------------------------------------------------------- ragged batches */
+/* All frequency channels of one array in one go: channel c uses the first
+ * n_t[c] steps of the geometry's orbits at wavelength c / freq_hz[c] and keeps
+ * the (baseline, t) samples whose two satellites see each other (the segment
+ * between them misses the Moon) when `compact` is set, so per-channel sizes
+ * are uneven.  vrs_batch_run() replays one captured hipGraph: a chunk-order
+ * gather of the per-channel skies, ONE persistent vis-gen launch over the
+ * concatenated sample lists of all channels and ONE persistent recon launch
+ * over (channel, visibility range, pixel chunk) work items.  Tile shapes are
+ * chosen by timing candidates on the batch itself (autotune) and cached per
+ * context.  No reference counterpart: the reference runs one period and one
+ * frequency per program (vissGen10M_shade.cu:291-298, shade10M.cu:152). */
+typedef struct vrs_batch vrs_batch;
+typedef struct vrs_batch_desc {
+    int32_t n_ch;
+    const double *freq_hz;   /* [n_ch] */
+    const int32_t *n_t;      /* [n_ch] steps used per channel, each <= the geometry's n_t */
+    int32_t compact;         /* 1: keep only mutually visible (baseline, t) samples */
+    int32_t precision;       /* VRS_PLAIN_F32 or VRS_MIXED_F64PHASE */
+    int32_t autotune;        /* 1: time candidate tiles and keep the fastest */
+} vrs_batch_desc;
+typedef struct vrs_batch_info {
+    int64_t n_vis_total;     /* kept samples over all channels */
+    int64_t n_vis_padded;    /* length of the concatenated V array (channels padded to 32) */
+    int64_t n_slots;         /* recon visibility ranges */
+    int32_t vis_tile;        /* chosen vis-gen tile (visibilities per block) */
+    int32_t recon_seg;       /* chosen recon range length */
+    double tune_visgen_ms;   /* time of the chosen vis-gen launch during tuning (0: not tuned) */
+    double tune_recon_ms;
+} vrs_batch_info;
+vrs_status vrs_batch_create(vrs_ctx *ctx, const vrs_geom *g, const vrs_batch_desc *d,
+                            vrs_batch **out);
+void vrs_batch_destroy(vrs_batch *b);
+vrs_status vrs_batch_info_get(const vrs_batch *b, vrs_batch_info *out);
+/* offsets: [n_ch + 1] start of each channel's samples in V; kept: [n_ch] real samples;
+ * sample_vt: [n_vis_padded] (baseline * n_t + t) of each entry, -1 for padding (nullable). */
+vrs_status vrs_batch_layout(const vrs_batch *b, int64_t *offsets, int64_t *kept, int32_t *sample_bt);
+/* I: [n_ch][n_pix] float (RING); V: [n_vis_padded] complex64 (padding entries are zero);
+ * F: [n_ch][n_pix] float.  kappa = 0 normalises each channel by 1 / kept[c]. */
+vrs_status vrs_batch_run(vrs_ctx *ctx, vrs_batch *b, const vrs_knobs *k, const float *I, float *V,
+                         float *F);
+
 /* ------------------------------------------------------- host-side helpers */
 /* Synthetic inputs (no reference counterpart; the reference reads MATLAB files).
  * These run on the host and need no GPU. */

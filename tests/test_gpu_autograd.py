@@ -1,0 +1,123 @@
+"""Differentiable form: torch.autograd.Function over vrs_visgen / vrs_adjoint
+and least-squares sky recovery.
+
+Checks:
+  * FP64 reference gradient (oracle, 2 A^H r on sampled pixels): rel L2 <= 1e-5
+  * central finite differences, h = 1e-3, of the FP64 oracle loss at a small
+    size (exact for this quadratic loss up to FP64 rounding): rel L2 <= 1e-5
+  * central finite differences, h = 1e-3, on 64 random pixels at the c5 size
+    with the loss evaluated in FP64 from GPU visibilities: the FP32 sums in
+    V(I +- h e_p) limit this to a few per cent, so the bound is 0.1
+  * 100 projected-gradient / projected-CG iterations lower the loss by >= 10x
+    (each step is an exact line minimiser before the projection onto I >= 0).
+"""
+import numpy as np
+import pytest
+
+import viss_recon_shade_amd as vrs
+from viss_recon_shade_amd import autograd as AG
+from viss_recon_shade_amd import lsq
+from viss_recon_shade_amd import workload as W
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+
+def cplx(V):
+    V = V.detach().cpu().numpy() if hasattr(V, "detach") else V
+    return V[..., 0].astype(np.float64) + 1j * V[..., 1].astype(np.float64)
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def setup(ctx, name, seed_noise=3):
+    wl = W.CONFIGS[name]
+    g = ctx.workload_geometry(wl)
+    I = torch.from_numpy(W.make_sky(wl)[0]).cuda()
+    k = vrs.knobs(precision="mixed")
+    V_true = AG.visgen(I, g, k).detach()
+    V_obs = lsq.add_noise(V_true, 20.0, seed_noise)
+    return wl, g, I, k, V_obs
+
+
+def test_autograd_gradient_matches_fp64_reference(ctx, oracle):
+    wl, g, I, k, V_obs = setup(ctx, "c5")
+    I0 = (0.5 * I).clone().requires_grad_(True)
+    L = AG.loss_sq(I0, g, V_obs, k)
+    L.backward()
+    grad = I0.grad.detach().cpu().numpy().astype(np.float64)
+    pos, uvw = g.MIncline()
+    lmn = g.lmn()
+    with torch.no_grad():
+        r = cplx(AG.visgen(I0.detach(), g, k) - V_obs)
+    psel = np.sort(np.random.default_rng(0).choice(wl.n_pix, 64, replace=False)).astype(np.int64)
+    gref = oracle.recon(pos, g.pairs, uvw, lmn, r, g.r_moon_km, kappa=2.0, pix_sel=psel)[0]
+    e = rel(grad[psel], gref)
+    print(f"[c5] autograd vs FP64 gradient (64 px): rel {e:.2e}", flush=True)
+    assert e <= 1e-5
+
+
+def test_central_fd_fp64_small(ctx, oracle):
+    wl, g, I, k, V_obs = setup(ctx, "small")
+    pos, uvw = g.MIncline()
+    lmn = g.lmn()
+    Iv = I.detach().cpu().numpy().astype(np.float32)
+    Vo = cplx(V_obs)
+    Ig = I.clone().requires_grad_(True)
+    AG.loss_sq(Ig, g, V_obs, k).backward()
+    grad = Ig.grad.cpu().numpy().astype(np.float64)
+    h = 1e-3
+    psel = np.random.default_rng(1).choice(wl.n_pix, 16, replace=False)
+    fd = []
+    for p in psel:
+        Ip, Im = Iv.astype(np.float64).copy(), Iv.astype(np.float64).copy()
+        Ip[p] += h
+        Im[p] -= h
+        both = np.stack([Ip, Im]).astype(np.float32)
+        # float32 input: use the representable perturbation actually applied
+        hh = (float(both[0, p]) - float(both[1, p])) / 2
+        V2 = oracle.visgen(pos, g.pairs, uvw, lmn, both, g.r_moon_km)
+        fd.append((np.sum(np.abs(V2[0] - Vo) ** 2) - np.sum(np.abs(V2[1] - Vo) ** 2)) / (2 * hh))
+    e = rel(grad[psel], np.array(fd))
+    print(f"[small] autograd vs FP64 central FD: rel {e:.2e}", flush=True)
+    assert e <= 1e-5
+
+
+def test_central_fd_gpu_c5(ctx):
+    wl, g, I, k, V_obs = setup(ctx, "c5")
+    Ig = I.clone().requires_grad_(True)
+    AG.loss_sq(Ig, g, V_obs, k).backward()
+    grad = Ig.grad.cpu().numpy().astype(np.float64)
+    h = 1e-3
+    psel = np.random.default_rng(2).choice(wl.n_pix, 64, replace=False)
+    Vo = cplx(V_obs)
+    fd = []
+    with torch.no_grad():
+        for p in psel:
+            Ip = I.clone()
+            Im = I.clone()
+            Ip[p] += h
+            Im[p] -= h
+            hh = float(Ip[p] - Im[p]) / 2
+            Vp, Vm = cplx(AG.visgen(Ip, g, k)), cplx(AG.visgen(Im, g, k))
+            fd.append(np.real(np.vdot(Vp - Vm, Vp + Vm - 2 * Vo)) / (2 * hh))
+    e = rel(grad[psel], np.array(fd))
+    print(f"[c5] autograd vs GPU central FD (h=1e-3, 64 px): rel {e:.2e}", flush=True)
+    assert e <= 0.1
+
+
+@pytest.mark.parametrize("method", ["pg", "cg"])
+def test_lsq_recovery_c5(ctx, method):
+    wl, g, I, k, V_obs = setup(ctx, "c5")
+    Irec, hist = lsq.recover(g, V_obs, n_iter=100, method=method, k=k)
+    h = np.array(hist)
+    err = float(torch.linalg.norm(Irec - I) / torch.linalg.norm(I))
+    print(f"[c5 lsq {method}] loss {h[0]:.4e} -> {h[-1]:.4e} in {len(h) - 1} iterations; "
+          f"sky rel err {err:.3f}", flush=True)
+    assert h[-1] < 0.1 * h[0]
+    assert (Irec >= 0).all()
+    # the line search minimises exactly along the free-set direction; the
+    # projection onto I >= 0 afterwards may cost a little, rarely
+    assert np.count_nonzero(np.diff(h) > 1e-6 * h[:-1]) <= 0.1 * (len(h) - 1)

@@ -49,6 +49,18 @@ class Knobs(C.Structure):
                 ("kappa", C.c_double), ("precision", C.c_int32)]
 
 
+class BatchDesc(C.Structure):
+    _fields_ = [("n_ch", C.c_int32), ("freq_hz", C.POINTER(C.c_double)),
+                ("n_t", C.POINTER(C.c_int32)), ("compact", C.c_int32), ("precision", C.c_int32),
+                ("autotune", C.c_int32)]
+
+
+class BatchInfo(C.Structure):
+    _fields_ = [("n_vis_total", C.c_int64), ("n_vis_padded", C.c_int64), ("n_slots", C.c_int64),
+                ("vis_tile", C.c_int32), ("recon_seg", C.c_int32), ("tune_visgen_ms", C.c_double),
+                ("tune_recon_ms", C.c_double)]
+
+
 class ShadeStats(C.Structure):
     _fields_ = [("shaded_pairs", C.c_int64), ("near_limb", C.c_int64), ("fp64_tests", C.c_int64)]
 
@@ -78,6 +90,11 @@ SIGNATURES = {
     "vrs_visgen": (C.c_int, [_ctx, _vp, C.POINTER(Knobs), C.c_int32, _vp, _vp]),
     "vrs_recon": (C.c_int, [_ctx, _vp, C.POINTER(Knobs), C.c_int32, _vp, _vp, _vp]),
     "vrs_adjoint": (C.c_int, [_ctx, _vp, C.POINTER(Knobs), _vp, _vp]),
+    "vrs_batch_create": (C.c_int, [_ctx, _vp, C.POINTER(BatchDesc), C.POINTER(C.c_void_p)]),
+    "vrs_batch_destroy": (None, [_vp]),
+    "vrs_batch_info_get": (C.c_int, [_vp, C.POINTER(BatchInfo)]),
+    "vrs_batch_layout": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "vrs_batch_run": (C.c_int, [_ctx, _vp, C.POINTER(Knobs), _vp, _vp, _vp]),
     "vrs_rng_normals": (C.c_int, [C.c_uint64, C.c_int64, _vp]),
     "vrs_rng_uniforms": (C.c_int, [C.c_uint64, C.c_int64, _vp]),
     "vrs_sky_lognormal": (C.c_int, [C.c_uint64, C.c_double, C.c_int64, C.c_int32, C.c_double,

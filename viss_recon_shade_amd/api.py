@@ -98,6 +98,9 @@ class Context:
     def workload_geometry(self, wl):
         return self.geometry(wl.n_sat, wl.n_t, wl.freq_hz, wl.nside)
 
+    def batch(self, geom, freqs_hz, n_t, compact=True, precision="mixed", autotune=True):
+        return Batch(self, geom, freqs_hz, n_t, compact, precision, autotune)
+
     def close(self):
         if self.handle:
             for g in list(self._geoms):   # geometries first: they live on this context
@@ -240,6 +243,57 @@ class Geometry:
         if self.handle:
             if self.ctx.handle:
                 self.ctx._lib.vrs_geom_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """All channels of one array in one captured hipGraph (vrs_batch): channel c
+    uses the first n_t[c] steps of `geom`'s orbits at freqs_hz[c], compacted to
+    the mutually visible (baseline, t) samples when `compact`."""
+
+    def __init__(self, ctx, geom, freqs_hz, n_t, compact, precision, autotune):
+        self.ctx, self.geom = ctx, geom
+        lib = ctx._lib
+        self.freqs = np.ascontiguousarray(freqs_hz, np.float64)
+        self.n_t = np.ascontiguousarray(n_t, np.int32)
+        self.n_ch = len(self.freqs)
+        self.precision = knobs(precision=precision).precision
+        d = _lib.BatchDesc(self.n_ch, self.freqs.ctypes.data_as(C.POINTER(C.c_double)),
+                           self.n_t.ctypes.data_as(C.POINTER(C.c_int32)), 1 if compact else 0,
+                           self.precision, 1 if autotune else 0)
+        h = C.c_void_p()
+        ctx.check(lib.vrs_batch_create(ctx.handle, geom.handle, C.byref(d), C.byref(h)))
+        self.handle = h
+        info = _lib.BatchInfo()
+        ctx.check(lib.vrs_batch_info_get(h, C.byref(info)))
+        self.info = {f: getattr(info, f) for f, _ in _lib.BatchInfo._fields_}
+        self.offsets = np.empty(self.n_ch + 1, np.int64)
+        self.kept = np.empty(self.n_ch, np.int64)
+        self.sample_bt = np.empty(self.info["n_vis_padded"], np.int32)
+        ctx.check(lib.vrs_batch_layout(h, _ptr(self.offsets), _ptr(self.kept), _ptr(self.sample_bt)))
+
+    def run(self, I, k=None, want_V=True):
+        """I [n_ch][n_pix] -> (V [n_vis_padded][2] or None, F [n_ch][n_pix])."""
+        k = k or knobs(precision=self.precision)
+        if I.shape != (self.n_ch, self.geom.n_pix):
+            raise ValueError(f"I must be [{self.n_ch}][{self.geom.n_pix}]")
+        V = _empty_like_kind(I, (self.info["n_vis_padded"], 2), np.float32) if want_V else None
+        F = _empty_like_kind(I, (self.n_ch, self.geom.n_pix), np.float32)
+        self.ctx._sync_stream(I)
+        self.ctx.check(self.ctx._lib.vrs_batch_run(self.ctx.handle, self.handle, C.byref(k),
+                                                   _ptr(I), _ptr(V) if V is not None else None,
+                                                   _ptr(F)))
+        return V, F
+
+    def close(self):
+        if self.handle:
+            self.ctx._lib.vrs_batch_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

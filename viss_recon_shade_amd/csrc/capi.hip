@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <tuple>
 #include <string>
 #include <vector>
 
@@ -38,6 +39,8 @@ struct vrs_ctx {
     void *ws[4] = {nullptr, nullptr, nullptr, nullptr};
     size_t ws_bytes[4] = {0, 0, 0, 0};
     unsigned long long *d_tot = nullptr;
+    // autotuned batch tiles, keyed by (padded visibilities, pixels, channels, precision)
+    std::map<std::tuple<int64_t, int64_t, int32_t, int32_t>, std::pair<int, int>> tuned;
 };
 
 struct vrs_geom {
@@ -61,7 +64,7 @@ struct vrs_geom {
     int32_t *d_ring = nullptr;
     ChunkRec *d_chunk = nullptr;
     GeomDev dev() const {
-        GeomDev g;
+        GeomDev g = {};
         g.n_sat = n_sat; g.n_t = n_t; g.n_base = n_base; g.n_chunk = n_chunk;
         g.n_vis = n_vis; g.n_pad = n_pad;
         g.pos = d_pos; g.cap = d_cap; g.uvw64 = d_uvw64; g.uvw32 = d_uvw32; g.pairs = d_pairs;
@@ -709,11 +712,12 @@ vrs_status vrs_visgen(vrs_ctx *ctx, const vrs_geom *g, const vrs_knobs *k, int32
     VRS_HIP(ctx, launch_to_chunk_order(gd, n_maps, g->n_pix, (const float *)sI.dev, (float *)ich,
                                        ctx->stream));
     const bool f64 = k->precision == VRS_MIXED_F64PHASE;
-    const int64_t n_tiles = ((g->n_vis + visgen_tile_vis() - 1) / visgen_tile_vis()) * n_maps;
-    const int grid = persistent_grid(ctx, visgen_kernel_ptr(f64), n_tiles);
+    const int vt = visgen_tile_vis();
+    const int64_t n_tiles = ((g->n_vis + vt - 1) / vt) * n_maps;
+    const int grid = persistent_grid(ctx, visgen_kernel_ptr(f64, vt), n_tiles);
     {
         Timer tm(ctx, "visgen");
-        VRS_HIP(ctx, launch_visgen(gd, phase_knobs(k), f64, n_maps, (const float *)ich,
+        VRS_HIP(ctx, launch_visgen(gd, phase_knobs(k), f64, vt, n_maps, (const float *)ich,
                                    (float2 *)sV.dev, grid, ctx->stream));
     }
     if (vrs_status s = sV.finish()) return s;
@@ -801,6 +805,382 @@ vrs_status vrs_adjoint(vrs_ctx *ctx, const vrs_geom *g, const vrs_knobs *k, cons
     vrs_knobs kk = *k;
     if (kk.precision == VRS_MFMA_SPLIT16) kk.precision = VRS_MIXED_F64PHASE;
     return recon_common(ctx, g, &kk, 1, resid, nullptr, grad, 2.0, "adjoint");
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- ragged batches
+struct vrs_batch {
+    vrs_ctx *ctx = nullptr;
+    const vrs_geom *g = nullptr;
+    int device = 0;
+    int32_t n_ch = 0, precision = 0;
+    int64_t n_vis = 0, n_pad_vis = 0, n_slots = 0;
+    int vt = 16, seg = 16384;
+    double tune_v = 0, tune_r = 0;
+    std::vector<int64_t> offsets, kept;   // [n_ch + 1], [n_ch]
+    std::vector<int32_t> sample_bt;       // [n_pad_vis]
+    std::vector<double> inv_lambda;
+    // device
+    int2 *d_vis_sat = nullptr;
+    int32_t *d_tile_map = nullptr;
+    double *d_inv_lambda = nullptr;
+    int4 *d_slots = nullptr;
+    int32_t *d_map_slot0 = nullptr;
+    float *d_kappa = nullptr;
+    float *d_I = nullptr, *d_Ich = nullptr, *d_F = nullptr, *d_part = nullptr;
+    float2 *d_V = nullptr;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_in = nullptr, ev_out = nullptr, ev_t0 = nullptr, ev_t1 = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+    // knobs the graph was captured with
+    double cap_s = 0, cap_kappa = -1;
+    int32_t cap_wfac = -1;
+    float cap_tocc = 0;
+    GeomDev dev() const {
+        GeomDev d = g->dev();
+        d.n_vis = n_pad_vis;
+        d.vis_sat = d_vis_sat;
+        d.tile_map = d_tile_map;
+        d.inv_lambda = d_inv_lambda;
+        d.slots = d_slots;
+        d.map_slot0 = d_map_slot0;
+        d.kappa_map = d_kappa;
+        d.n_slots = n_slots;
+        return d;
+    }
+};
+
+namespace {
+
+// Satellites a and b see each other: the segment between them misses the closed sphere.
+bool line_of_sight(const double *ra, const double *rb, double R) {
+    const double dx = rb[0] - ra[0], dy = rb[1] - ra[1], dz = rb[2] - ra[2];
+    const double dd = dx * dx + dy * dy + dz * dz;
+    double t = dd > 0 ? -(ra[0] * dx + ra[1] * dy + ra[2] * dz) / dd : 0.0;
+    t = std::min(1.0, std::max(0.0, t));
+    const double px = ra[0] + t * dx, py = ra[1] + t * dy, pz = ra[2] + t * dz;
+    return px * px + py * py + pz * pz > R * R;
+}
+
+vrs_status batch_slots(vrs_batch *b, int seg, std::vector<int4> &slots, std::vector<int32_t> &slot0) {
+    slots.clear();
+    slot0.assign(b->n_ch + 1, 0);
+    for (int c = 0; c < b->n_ch; ++c) {
+        slot0[c] = (int32_t)slots.size();
+        const int64_t v0 = b->offsets[c], v1 = b->offsets[c + 1];
+        for (int64_t v = v0; v < v1; v += seg) {
+            int4 sl;
+            sl.x = c;
+            sl.y = (int32_t)v;
+            sl.z = (int32_t)std::min<int64_t>(v1, v + seg);
+            sl.w = 0;
+            slots.push_back(sl);
+        }
+    }
+    slot0[b->n_ch] = (int32_t)slots.size();
+    return VRS_OK;
+}
+
+// Upload slots for range length `seg` and size the partial-image buffer.
+vrs_status batch_set_seg(vrs_batch *b, int seg) {
+    vrs_ctx *ctx = b->ctx;
+    std::vector<int4> slots;
+    std::vector<int32_t> slot0;
+    batch_slots(b, seg, slots, slot0);
+    VRS_HIP(ctx, hipStreamSynchronize(b->stream));
+    if (b->d_slots) VRS_HIP(ctx, hipFree(b->d_slots));
+    if (b->d_part) VRS_HIP(ctx, hipFree(b->d_part));
+    b->d_slots = nullptr;
+    b->d_part = nullptr;
+    VRS_HIP(ctx, hipMalloc(&b->d_slots, sizeof(int4) * std::max<size_t>(slots.size(), 1)));
+    VRS_HIP(ctx, hipMalloc(&b->d_part, sizeof(float) * std::max<size_t>(slots.size(), 1) * b->g->n_pad));
+    VRS_HIP(ctx, hipMemcpy(b->d_slots, slots.data(), sizeof(int4) * slots.size(), hipMemcpyHostToDevice));
+    VRS_HIP(ctx, hipMemcpy(b->d_map_slot0, slot0.data(), sizeof(int32_t) * slot0.size(), hipMemcpyHostToDevice));
+    b->n_slots = (int64_t)slots.size();
+    b->seg = seg;
+    return VRS_OK;
+}
+
+// The kernels of one batch step, launched on the batch stream (also the capture body).
+vrs_status batch_launch(vrs_batch *b, const PhaseKnobs &pk, bool skip_recon = false,
+                        bool skip_visgen = false) {
+    vrs_ctx *ctx = b->ctx;
+    const GeomDev gd = b->dev();
+    const bool f64 = b->precision == VRS_MIXED_F64PHASE;
+    if (!skip_visgen) {
+        VRS_HIP(ctx, launch_to_chunk_order(gd, b->n_ch, b->g->n_pix, b->d_I, b->d_Ich, b->stream));
+        const int grid = persistent_grid(ctx, visgen_kernel_ptr(f64, b->vt), b->n_pad_vis / b->vt);
+        VRS_HIP(ctx, launch_visgen(gd, pk, f64, b->vt, b->n_ch, b->d_Ich, b->d_V, grid, b->stream));
+    }
+    if (!skip_recon) {
+        const int grid = persistent_grid(ctx, recon_kernel_ptr(f64), b->n_slots * b->g->n_chunk);
+        VRS_HIP(ctx, launch_recon(gd, pk, f64, b->n_ch, b->d_V, nullptr, 0.0f, 1, b->d_part, grid,
+                                  b->stream));
+        VRS_HIP(ctx, launch_recon_finalize(gd, b->n_ch, 1, b->g->n_pix, b->d_part, b->d_F, b->stream));
+    }
+    return VRS_OK;
+}
+
+vrs_status batch_time(vrs_batch *b, const PhaseKnobs &pk, bool visgen, double *ms) {
+    vrs_ctx *ctx = b->ctx;
+    VRS_HIP(ctx, hipEventRecord(b->ev_t0, b->stream));
+    if (vrs_status s = batch_launch(b, pk, visgen, !visgen)) return s;
+    VRS_HIP(ctx, hipEventRecord(b->ev_t1, b->stream));
+    VRS_HIP(ctx, hipEventSynchronize(b->ev_t1));
+    float f;
+    VRS_HIP(ctx, hipEventElapsedTime(&f, b->ev_t0, b->ev_t1));
+    *ms = f;
+    return VRS_OK;
+}
+
+vrs_status batch_capture(vrs_batch *b, const PhaseKnobs &pk) {
+    vrs_ctx *ctx = b->ctx;
+    if (b->exec) {
+        VRS_HIP(ctx, hipGraphExecDestroy(b->exec));
+        b->exec = nullptr;
+    }
+    if (b->graph) {
+        VRS_HIP(ctx, hipGraphDestroy(b->graph));
+        b->graph = nullptr;
+    }
+    VRS_HIP(ctx, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    vrs_status st = batch_launch(b, pk);
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(b->stream, &graph);
+    if (st != VRS_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return st;
+    }
+    if (e != hipSuccess) return fail(ctx, VRS_E_HIP, "hipStreamEndCapture failed: %s", hipGetErrorString(e));
+    b->graph = graph;
+    VRS_HIP(ctx, hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0));
+    b->cap_s = pk.s;
+    b->cap_wfac = pk.wfac;
+    b->cap_tocc = pk.t_occ;
+    return VRS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void vrs_batch_destroy(vrs_batch *b) {
+    if (!b) return;
+    DeviceGuard dg(b->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    void *ptrs[] = {b->d_vis_sat, b->d_tile_map, b->d_inv_lambda, b->d_slots, b->d_map_slot0,
+                    b->d_kappa, b->d_I, b->d_Ich, b->d_F, b->d_part, b->d_V};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    hipEvent_t evs[] = {b->ev_in, b->ev_out, b->ev_t0, b->ev_t1};
+    for (hipEvent_t e : evs)
+        if (e) (void)hipEventDestroy(e);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+vrs_status vrs_batch_create(vrs_ctx *ctx, const vrs_geom *g, const vrs_batch_desc *d,
+                            vrs_batch **out) {
+    if (vrs_status s = check_ctx(ctx)) return s;
+    if (vrs_status s = check_geom(ctx, g)) return s;
+    if (!d || !out) return fail(ctx, VRS_E_ARG, "desc and out must be non-NULL");
+    *out = nullptr;
+    if (d->n_ch < 1 || d->n_ch > 4096) return fail(ctx, VRS_E_ARG, "n_ch must be in [1, 4096]");
+    if (!d->freq_hz || !d->n_t) return fail(ctx, VRS_E_ARG, "freq_hz and n_t are required");
+    if (d->precision != VRS_PLAIN_F32 && d->precision != VRS_MIXED_F64PHASE)
+        return fail(ctx, VRS_E_ARG, "batches run VRS_PLAIN_F32 or VRS_MIXED_F64PHASE");
+    for (int c = 0; c < d->n_ch; ++c) {
+        if (!(d->freq_hz[c] > 0) || !std::isfinite(d->freq_hz[c]))
+            return fail(ctx, VRS_E_ARG, "freq_hz[%d] must be > 0", c);
+        if (d->n_t[c] < 1 || d->n_t[c] > g->n_t)
+            return fail(ctx, VRS_E_ARG, "n_t[%d] = %d outside [1, %d]", c, d->n_t[c], g->n_t);
+    }
+    DeviceGuard dg(ctx->device);
+    // positions for the line-of-sight test (FP64, exported from stage 1)
+    std::vector<double> pos((size_t)g->n_sat * g->n_t * 3);
+    VRS_HIP(ctx, hipMemcpy2D(pos.data(), 3 * sizeof(double), g->d_pos, sizeof(double4),
+                             3 * sizeof(double), (size_t)g->n_sat * g->n_t, hipMemcpyDeviceToHost));
+    vrs_batch *b = new vrs_batch;
+    b->ctx = ctx;
+    b->g = g;
+    b->device = ctx->device;
+    b->n_ch = d->n_ch;
+    b->precision = d->precision;
+    const int pad = list_pad();
+    std::vector<int2> vis_sat;
+    std::vector<int32_t> tile_map;
+    b->offsets.push_back(0);
+    for (int c = 0; c < d->n_ch; ++c) {
+        int64_t kept = 0;
+        for (int bl = 0; bl < g->n_base; ++bl) {
+            const int pa = g->pairs[2 * bl], pb = g->pairs[2 * bl + 1];
+            for (int t = 0; t < d->n_t[c]; ++t) {
+                const int ia = pa * g->n_t + t, ib = pb * g->n_t + t;
+                if (d->compact && !line_of_sight(&pos[3 * (size_t)ia], &pos[3 * (size_t)ib], g->r_moon))
+                    continue;
+                vis_sat.push_back(make_int2(ia, ib));
+                b->sample_bt.push_back(bl * g->n_t + t);
+                ++kept;
+            }
+        }
+        while (vis_sat.size() % pad) {
+            vis_sat.push_back(make_int2(-1, -1));
+            b->sample_bt.push_back(-1);
+        }
+        b->kept.push_back(kept);
+        b->offsets.push_back((int64_t)vis_sat.size());
+        b->n_vis += kept;
+        b->inv_lambda.push_back(1e3 / (kC / d->freq_hz[c]));
+    }
+    for (int c = 0; c < d->n_ch; ++c)
+        for (int64_t v = b->offsets[c]; v < b->offsets[c + 1]; v += pad) tile_map.push_back(c);
+    b->n_pad_vis = (int64_t)vis_sat.size();
+    if (b->n_pad_vis == 0 || b->n_pad_vis > ((int64_t)1 << 31)) {
+        vrs_batch_destroy(b);
+        return fail(ctx, VRS_E_ARG, "batch has %lld samples (must be in [1, 2^31])",
+                    (long long)b->n_pad_vis);
+    }
+    auto cleanup = [&](vrs_status s) { vrs_batch_destroy(b); return s; };
+#define VRS_BHIP(call)                                                                        \
+    do {                                                                                      \
+        hipError_t e_ = (call);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return cleanup(fail(ctx, e_ == hipErrorOutOfMemory ? VRS_E_OOM : VRS_E_HIP,       \
+                                "%s failed: %s", #call, hipGetErrorString(e_)));              \
+    } while (0)
+    VRS_BHIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    VRS_BHIP(hipEventCreate(&b->ev_in));
+    VRS_BHIP(hipEventCreate(&b->ev_out));
+    VRS_BHIP(hipEventCreate(&b->ev_t0));
+    VRS_BHIP(hipEventCreate(&b->ev_t1));
+    VRS_BHIP(hipMalloc(&b->d_vis_sat, sizeof(int2) * b->n_pad_vis));
+    VRS_BHIP(hipMalloc(&b->d_tile_map, sizeof(int32_t) * tile_map.size()));
+    VRS_BHIP(hipMalloc(&b->d_inv_lambda, sizeof(double) * b->n_ch));
+    VRS_BHIP(hipMalloc(&b->d_map_slot0, sizeof(int32_t) * (b->n_ch + 1)));
+    VRS_BHIP(hipMalloc(&b->d_kappa, sizeof(float) * b->n_ch));
+    VRS_BHIP(hipMalloc(&b->d_I, sizeof(float) * b->n_ch * g->n_pix));
+    VRS_BHIP(hipMalloc(&b->d_Ich, sizeof(float) * b->n_ch * g->n_pad));
+    VRS_BHIP(hipMalloc(&b->d_F, sizeof(float) * b->n_ch * g->n_pix));
+    VRS_BHIP(hipMalloc(&b->d_V, sizeof(float2) * b->n_pad_vis));
+    VRS_BHIP(hipMemset(b->d_V, 0, sizeof(float2) * b->n_pad_vis));  // padding entries stay zero
+    VRS_BHIP(hipMemset(b->d_I, 0, sizeof(float) * b->n_ch * g->n_pix));
+    VRS_BHIP(hipMemcpy(b->d_vis_sat, vis_sat.data(), sizeof(int2) * vis_sat.size(), hipMemcpyHostToDevice));
+    VRS_BHIP(hipMemcpy(b->d_tile_map, tile_map.data(), sizeof(int32_t) * tile_map.size(), hipMemcpyHostToDevice));
+    VRS_BHIP(hipMemcpy(b->d_inv_lambda, b->inv_lambda.data(), sizeof(double) * b->n_ch, hipMemcpyHostToDevice));
+    if (vrs_status s = batch_set_seg(b, 16384)) return cleanup(s);
+
+    // autotune: vis-gen tile (8, 16, 32 visibilities) and recon range length
+    const auto key = std::make_tuple(b->n_pad_vis, g->n_pix, b->n_ch, b->precision);
+    auto hit = ctx->tuned.find(key);
+    if (hit != ctx->tuned.end()) {
+        b->vt = hit->second.first;
+        if (vrs_status s = batch_set_seg(b, hit->second.second)) return cleanup(s);
+    } else if (d->autotune) {
+        PhaseKnobs pk;
+        pk.s = 1.0;
+        pk.wfac = 0;
+        pk.t_occ = 240.0f;
+        double best = 1e300, ms = 0;
+        int best_vt = 16;
+        if (vrs_status s = batch_time(b, pk, true, &ms)) return cleanup(s);  // warm-up
+        for (int vt : {8, 16, 32}) {
+            b->vt = vt;
+            if (vrs_status s = batch_time(b, pk, true, &ms)) return cleanup(s);
+            if (ms < best) { best = ms; best_vt = vt; }
+        }
+        b->vt = best_vt;
+        b->tune_v = best;
+        best = 1e300;
+        int best_seg = 16384;
+        for (int seg : {4096, 16384, 65536}) {
+            if (vrs_status s = batch_set_seg(b, seg)) return cleanup(s);
+            if (vrs_status s = batch_time(b, pk, false, &ms)) return cleanup(s);
+            if (ms < best) { best = ms; best_seg = seg; }
+        }
+        b->tune_r = best;
+        if (vrs_status s = batch_set_seg(b, best_seg)) return cleanup(s);
+        ctx->tuned[key] = std::make_pair(b->vt, b->seg);
+    }
+#undef VRS_BHIP
+    *out = b;
+    return VRS_OK;
+}
+
+vrs_status vrs_batch_info_get(const vrs_batch *b, vrs_batch_info *out) {
+    if (!b || !out) return fail(nullptr, VRS_E_ARG, "batch and out must be non-NULL");
+    out->n_vis_total = b->n_vis;
+    out->n_vis_padded = b->n_pad_vis;
+    out->n_slots = b->n_slots;
+    out->vis_tile = b->vt;
+    out->recon_seg = b->seg;
+    out->tune_visgen_ms = b->tune_v;
+    out->tune_recon_ms = b->tune_r;
+    return VRS_OK;
+}
+
+vrs_status vrs_batch_layout(const vrs_batch *b, int64_t *offsets, int64_t *kept, int32_t *sample_bt) {
+    if (!b) return fail(nullptr, VRS_E_ARG, "batch is NULL");
+    if (offsets) std::memcpy(offsets, b->offsets.data(), sizeof(int64_t) * b->offsets.size());
+    if (kept) std::memcpy(kept, b->kept.data(), sizeof(int64_t) * b->kept.size());
+    if (sample_bt) std::memcpy(sample_bt, b->sample_bt.data(), sizeof(int32_t) * b->sample_bt.size());
+    return VRS_OK;
+}
+
+vrs_status vrs_batch_run(vrs_ctx *ctx, vrs_batch *b, const vrs_knobs *k, const float *I, float *V,
+                         float *F) {
+    if (vrs_status s = check_ctx(ctx)) return s;
+    if (!b || b->ctx != ctx) return fail(ctx, VRS_E_ARG, "batch does not belong to this context");
+    if (vrs_status s = check_knobs(ctx, k)) return s;
+    if (!I) return fail(ctx, VRS_E_ARG, "I must be non-NULL");
+    if (k->precision != b->precision)
+        return fail(ctx, VRS_E_ARG, "knobs precision %d differs from the batch's %d", k->precision,
+                    b->precision);
+    DeviceGuard dg(ctx->device);
+    const vrs_geom *g = b->g;
+    // per-channel normalisation: kappa, or 1 / kept samples
+    std::vector<float> kap(b->n_ch);
+    for (int c = 0; c < b->n_ch; ++c)
+        kap[c] = (float)(k->kappa > 0 ? k->kappa : (b->kept[c] > 0 ? 1.0 / (double)b->kept[c] : 0.0));
+    const PhaseKnobs pk = phase_knobs(k);
+    if (!b->exec || pk.s != b->cap_s || pk.wfac != b->cap_wfac || pk.t_occ != b->cap_tocc) {
+        if (vrs_status s = batch_capture(b, pk)) return s;
+    }
+    // order after the caller's stream, run on the batch stream, order the caller after it
+    VRS_HIP(ctx, hipEventRecord(b->ev_in, ctx->stream));
+    VRS_HIP(ctx, hipStreamWaitEvent(b->stream, b->ev_in, 0));
+    VRS_HIP(ctx, hipMemcpyAsync(b->d_kappa, kap.data(), sizeof(float) * b->n_ch,
+                                hipMemcpyHostToDevice, b->stream));
+    VRS_HIP(ctx, hipMemcpyAsync(b->d_I, I, sizeof(float) * b->n_ch * g->n_pix, hipMemcpyDefault,
+                                b->stream));
+    if (ctx->timing) {
+        auto it = ctx->events.find("batch");
+        if (it == ctx->events.end()) {
+            hipEvent_t e0, e1;
+            VRS_HIP(ctx, hipEventCreate(&e0));
+            VRS_HIP(ctx, hipEventCreate(&e1));
+            it = ctx->events.emplace("batch", std::make_pair(e0, e1)).first;
+        }
+        VRS_HIP(ctx, hipEventRecord(it->second.first, b->stream));
+        VRS_HIP(ctx, hipGraphLaunch(b->exec, b->stream));
+        VRS_HIP(ctx, hipEventRecord(it->second.second, b->stream));
+    } else {
+        VRS_HIP(ctx, hipGraphLaunch(b->exec, b->stream));
+    }
+    if (V)
+        VRS_HIP(ctx, hipMemcpyAsync(V, b->d_V, sizeof(float2) * b->n_pad_vis, hipMemcpyDefault,
+                                    b->stream));
+    if (F)
+        VRS_HIP(ctx, hipMemcpyAsync(F, b->d_F, sizeof(float) * b->n_ch * g->n_pix, hipMemcpyDefault,
+                                    b->stream));
+    VRS_HIP(ctx, hipEventRecord(b->ev_out, b->stream));
+    VRS_HIP(ctx, hipStreamWaitEvent(ctx->stream, b->ev_out, 0));
+    // host destinations (or the pageable kappa source) need the copies finished
+    VRS_HIP(ctx, hipStreamSynchronize(b->stream));
+    return VRS_OK;
 }
 
 }  // extern "C"

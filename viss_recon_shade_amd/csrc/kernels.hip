@@ -239,7 +239,11 @@ __global__ void __launch_bounds__(256) to_chunk_order_kernel(GeomDev g, int32_t 
 }
 
 // ------------------------------------------------------------------ stage 3
-constexpr int kVisTile = 16;  // visibilities per vis-gen tile (accumulators per lane: 2x)
+// List mode (ragged batches): g.vis_sat != nullptr.  Visibility v is then the
+// v-th entry of a concatenation of per-map (per-channel) sample lists, each
+// padded to a multiple of kListPad entries; padding entries have vis_sat.x < 0.
+// uvw = (r_b - r_a) * inv_lambda[map], formed exactly as stage 1 forms it.
+constexpr int kListPad = 32;
 
 template <bool F64> struct PhaseT { using T = float; };
 template <> struct PhaseT<true> { using T = double; };
@@ -250,32 +254,56 @@ template <bool F64> struct VisEnt {
     int32_t ia, ib, pad0, pad1;
 };
 
-template <bool F64>
-__device__ __forceinline__ void fill_vis_ent(const GeomDev &g, const PhaseKnobs &k, int64_t v,
-                                             VisEnt<F64> &e) {
-    using T = typename PhaseT<F64>::T;
-    if (v < g.n_vis) {
+// (ia, ib, uvw) of visibility v; returns false for padding / tail entries.
+__device__ __forceinline__ bool vis_geometry(const GeomDev &g, int64_t v, int64_t map, int &ia,
+                                             int &ib, double &u, double &vv, double &w) {
+    if (v >= g.n_vis) return false;
+    if (g.vis_sat) {
+        const int2 s = g.vis_sat[v];
+        if (s.x < 0) return false;
+        ia = s.x;
+        ib = s.y;
+        const double4 ra = g.pos[ia], rb = g.pos[ib];
+        const double il = g.inv_lambda[map];
+        u = (rb.x - ra.x) * il;
+        vv = (rb.y - ra.y) * il;
+        w = (rb.z - ra.z) * il;
+    } else {
         const int b = (int)(v / g.n_t), t = (int)(v % g.n_t);
         const int2 pr = g.pairs[b];
-        e.ia = pr.x * g.n_t + t;
-        e.ib = pr.y * g.n_t + t;
-        e.a = g.cap[e.ia];
-        e.b = g.cap[e.ib];
+        ia = pr.x * g.n_t + t;
+        ib = pr.y * g.n_t + t;
+        const double4 q = g.uvw64[v];
+        u = q.x; vv = q.y; w = q.z;
+    }
+    return true;
+}
+
+template <bool F64>
+__device__ __forceinline__ void fill_vis_ent(const GeomDev &g, const PhaseKnobs &k, int64_t v,
+                                             int64_t map, VisEnt<F64> &e) {
+    using T = typename PhaseT<F64>::T;
+    int ia = 0, ib = 0;
+    double u, vv, w;
+    if (vis_geometry(g, v, map, ia, ib, u, vv, w)) {
+        e.ia = ia;
+        e.ib = ib;
+        e.a = g.cap[ia];
+        e.b = g.cap[ib];
         if (F64) {
-            const double4 q = g.uvw64[v];
-            e.u = (T)(q.x * k.s);
-            e.v = (T)(q.y * k.s);
-            e.w = (T)(q.z * k.s);
+            e.u = (T)(u * k.s);
+            e.v = (T)(vv * k.s);
+            e.w = (T)(w * k.s);
         } else {
-            const float4 q = g.uvw32[v];
+            // the FP32 path rounds uvw to FP32 first, as the stored uvw32 array is
             const float s = (float)k.s;
-            e.u = (T)(q.x * s);
-            e.v = (T)(q.y * s);
-            e.w = (T)(q.z * s);
+            e.u = (T)((float)u * s);
+            e.v = (T)((float)vv * s);
+            e.w = (T)((float)w * s);
         }
         e.wo = k.wfac ? (T)0 : -e.w;
     } else {
-        // tail: no shading (c = 2 is never reached), zero phase; never written out
+        // tail / padding: no shading (c = 2 is never reached), zero phase; never written out
         e.ia = e.ib = 0;
         CapRec none = {0.f, 0.f, 0.f, 2.0f, 0.f, 0.f, 0.f, 0.f};
         e.a = none;
@@ -284,43 +312,46 @@ __device__ __forceinline__ void fill_vis_ent(const GeomDev &g, const PhaseKnobs 
     }
 }
 
-template <bool F64>
+template <bool F64, int VT>
 __global__ void __launch_bounds__(kBlock) visgen_kernel(GeomDev g, PhaseKnobs k, int32_t n_maps,
                                                         const float *__restrict__ Ich,
                                                         float2 *__restrict__ V) {
     using T = typename PhaseT<F64>::T;
-    __shared__ VisEnt<F64> ents[kVisTile];
-    __shared__ float red[kWavesPerBlock][kVisTile][2];
-    __shared__ float corr[kWavesPerBlock][kVisTile][2];  // straddling pairs, per wave
-    constexpr uint32_t kTileMask = (kVisTile == 32) ? 0xffffffffu : ((1u << kVisTile) - 1u);
+    static_assert(VT == 8 || VT == 16 || VT == 32, "tile of 8, 16 or 32 visibilities");
+    static_assert(kListPad % VT == 0, "list padding must be whole tiles");
+    __shared__ VisEnt<F64> ents[VT];
+    __shared__ float red[kWavesPerBlock][VT][2];
+    __shared__ float corr[kWavesPerBlock][VT][2];  // straddling pairs, per wave
+    constexpr uint32_t kTileMask = (VT == 32) ? 0xffffffffu : ((1u << VT) - 1u);
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x >> 6;
-    const int64_t n_vtiles = (g.n_vis + kVisTile - 1) / kVisTile;
-    const int64_t n_tiles = n_vtiles * n_maps;
+    const bool list = g.vis_sat != nullptr;
+    const int64_t n_vtiles = (g.n_vis + VT - 1) / VT;
+    const int64_t n_tiles = list ? n_vtiles : n_vtiles * n_maps;
     const float t_occ = k.t_occ;
 
     for (int64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
-        const int64_t map = tile / n_vtiles;
-        const int64_t v0 = (tile % n_vtiles) * kVisTile;
+        const int64_t map = list ? g.tile_map[(tile * VT) / kListPad] : tile / n_vtiles;
+        const int64_t v0 = (list ? tile : tile % n_vtiles) * VT;
         const float *__restrict__ Im = Ich + map * g.n_pad;
-        if (threadIdx.x < kVisTile) fill_vis_ent<F64>(g, k, v0 + threadIdx.x, ents[threadIdx.x]);
-        if (threadIdx.x < kWavesPerBlock * kVisTile * 2) (&corr[0][0][0])[threadIdx.x] = 0.0f;
+        if (threadIdx.x < VT) fill_vis_ent<F64>(g, k, v0 + threadIdx.x, map, ents[threadIdx.x]);
+        if (threadIdx.x < kWavesPerBlock * VT * 2) (&corr[0][0][0])[threadIdx.x] = 0.0f;
         __syncthreads();
 
-        float are[kVisTile], aim[kVisTile];
+        float are[VT], aim[VT];
 #pragma unroll
-        for (int q = 0; q < kVisTile; ++q) are[q] = aim[q] = 0.0f;
+        for (int q = 0; q < VT; ++q) are[q] = aim[q] = 0.0f;
 
         for (int c = wave; c < g.n_chunk; c += kWavesPerBlock) {
             // The tile entries are invariant over this loop; keep the compiler
-            // from hoisting all of them into registers (it would need ~450).
+            // from hoisting all of them into registers.
             asm volatile("" ::: "memory");
             const ChunkRec ch = g.chunk[c];
             // lane-parallel classification of the tile's visibilities against
             // this chunk; the results become wave-uniform bit masks
             uint32_t in_m, out_m;
             {
-                const int q = lane & (kVisTile - 1);
+                const int q = lane & (VT - 1);
                 const int cls = classify(ch, ents[q].a, ents[q].b);
                 in_m = (uint32_t)__ballot(cls == kIn) & kTileMask;
                 out_m = (uint32_t)__ballot(cls == kOut) & kTileMask;
@@ -343,7 +374,7 @@ __global__ void __launch_bounds__(kBlock) visgen_kernel(GeomDev g, PhaseKnobs k,
             // are skipped and done pixel by pixel below.
             const uint32_t str_m = ~(in_m | out_m) & kTileMask;
 #pragma unroll
-            for (int q = 0; q < kVisTile; ++q) {
+            for (int q = 0; q < VT; ++q) {
                 if ((str_m >> q) & 1u) continue;
                 const bool sh = (in_m >> q) & 1u;
                 const T u = ents[q].u, vv = ents[q].v, w = ents[q].w, wo = ents[q].wo;
@@ -393,7 +424,7 @@ __global__ void __launch_bounds__(kBlock) visgen_kernel(GeomDev g, PhaseKnobs k,
             }
         }
 #pragma unroll
-        for (int q = 0; q < kVisTile; ++q) {
+        for (int q = 0; q < VT; ++q) {
             const float sr = wave_sum(are[q]);
             const float si = wave_sum(aim[q]);
             if (lane == 0) {
@@ -402,10 +433,11 @@ __global__ void __launch_bounds__(kBlock) visgen_kernel(GeomDev g, PhaseKnobs k,
             }
         }
         __syncthreads();
-        if (threadIdx.x < kVisTile) {
+        if (threadIdx.x < VT) {
             const int q = threadIdx.x;
             const int64_t v = v0 + q;
-            if (v < g.n_vis) {
+            const bool real = list ? (v < g.n_vis && g.vis_sat[v].x >= 0) : (v < g.n_vis);
+            if (real) {
                 float sr = 0.0f, si = 0.0f;
 #pragma unroll
                 for (int w = 0; w < kWavesPerBlock; ++w) {
@@ -413,7 +445,7 @@ __global__ void __launch_bounds__(kBlock) visgen_kernel(GeomDev g, PhaseKnobs k,
                     si += red[w][q][1] + corr[w][q][1];
                 }
                 // exp(-i theta): V = sum I (cos - i sin)
-                V[map * g.n_vis + v] = make_float2(sr, -si);
+                V[(list ? 0 : map * g.n_vis) + v] = make_float2(sr, -si);
             }
         }
         __syncthreads();
@@ -430,6 +462,9 @@ template <bool F64> struct RecEnt {
     int32_t ia, ib;
 };
 
+// Work item = (slot, chunk); a slot is a visibility range of one map.  In
+// geometry mode slot = map * n_seg + seg over equal segments; in list mode
+// the slots come from g.slots.
 template <bool F64>
 __global__ void __launch_bounds__(kBlock) recon_kernel(GeomDev g, PhaseKnobs k, int32_t n_maps,
                                                        const float2 *__restrict__ V,
@@ -441,15 +476,31 @@ __global__ void __launch_bounds__(kBlock) recon_kernel(GeomDev g, PhaseKnobs k, 
     __shared__ float red[kWavesPerBlock][kChunk];
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x >> 6;
+    const bool list = g.vis_sat != nullptr;
     const int64_t seg_len = (g.n_vis + n_seg - 1) / n_seg;
-    const int64_t n_items = (int64_t)n_maps * n_seg * g.n_chunk;
+    const int64_t n_slots = list ? g.n_slots : (int64_t)n_maps * n_seg;
+    const int64_t n_items = n_slots * g.n_chunk;
 
     for (int64_t item = blockIdx.x; item < n_items; item += gridDim.x) {
-        // segment varies fastest so that blocks on one XCD share visibility data
-        const int64_t seg = item % n_seg;
-        const int64_t rest = item / n_seg;
-        const int c = (int)(rest % g.n_chunk);
-        const int64_t map = rest / g.n_chunk;
+        // the slot varies fastest so that blocks on one XCD share visibility data
+        const int64_t slot = item % n_slots;
+        const int c = (int)(item / n_slots);
+        int64_t map, vb, ve;
+        float kap;
+        if (list) {
+            const int4 sl = g.slots[slot];
+            map = sl.x;
+            vb = sl.y;
+            ve = sl.z;
+            kap = g.kappa_map[map];
+        } else {
+            map = slot / n_seg;
+            const int64_t seg = slot % n_seg;
+            vb = seg * seg_len;
+            ve = (vb + seg_len < g.n_vis) ? vb + seg_len : g.n_vis;
+            kap = kappa;
+        }
+        const int64_t voff = list ? 0 : map * g.n_vis;
         const ChunkRec ch = g.chunk[c];
         T l[kPixPerLane], m[kPixPerLane], n[kPixPerLane];
         float sx[kPixPerLane], sy[kPixPerLane], sz[kPixPerLane], acc[kPixPerLane];
@@ -467,32 +518,28 @@ __global__ void __launch_bounds__(kBlock) recon_kernel(GeomDev g, PhaseKnobs k, 
             }
             acc[j] = 0.0f;
         }
-        const int64_t vb = seg * seg_len;
-        const int64_t ve = (vb + seg_len < g.n_vis) ? vb + seg_len : g.n_vis;
         for (int64_t vt = vb; vt < ve; vt += kReconStage) {
             if (threadIdx.x < kReconStage) {
                 const int64_t v = vt + threadIdx.x;
                 RecEnt<F64> &e = ents[threadIdx.x];
-                if (v < ve) {
-                    const int b = (int)(v / g.n_t), t = (int)(v % g.n_t);
-                    const int2 pr = g.pairs[b];
-                    e.ia = pr.x * g.n_t + t;
-                    e.ib = pr.y * g.n_t + t;
-                    e.a = g.cap[e.ia];
-                    e.b = g.cap[e.ib];
+                int ia = 0, ib = 0;
+                double u, vv, w;
+                if (v < ve && vis_geometry(g, v, map, ia, ib, u, vv, w)) {
+                    e.ia = ia;
+                    e.ib = ib;
+                    e.a = g.cap[ia];
+                    e.b = g.cap[ib];
                     if (F64) {
-                        const double4 q = g.uvw64[v];
-                        e.u = (T)(q.x * k.s); e.v = (T)(q.y * k.s); e.w = (T)(q.z * k.s);
+                        e.u = (T)(u * k.s); e.v = (T)(vv * k.s); e.w = (T)(w * k.s);
                     } else {
-                        const float4 q = g.uvw32[v];
                         const float s = (float)k.s;
-                        e.u = (T)(q.x * s); e.v = (T)(q.y * s); e.w = (T)(q.z * s);
+                        e.u = (T)((float)u * s); e.v = (T)((float)vv * s); e.w = (T)((float)w * s);
                     }
                     e.wo = k.wfac ? (T)0 : -e.w;
-                    const float2 vv = V[map * g.n_vis + v];
-                    const float sc = kappa * (wgt ? wgt[v] : 1.0f);
-                    e.vr = vv.x * sc;
-                    e.vi = vv.y * sc;
+                    const float2 q = V[voff + v];
+                    const float sc = kap * (wgt ? wgt[v] : 1.0f);
+                    e.vr = q.x * sc;
+                    e.vi = q.y * sc;
                 } else {
                     e.ia = e.ib = 0;
                     CapRec none = {0.f, 0.f, 0.f, 2.0f, 0.f, 0.f, 0.f, 0.f};
@@ -552,13 +599,14 @@ __global__ void __launch_bounds__(kBlock) recon_kernel(GeomDev g, PhaseKnobs k, 
         {
             const int p = threadIdx.x;  // kBlock == kChunk
             const float s = red[0][p] + red[1][p] + red[2][p] + red[3][p];
-            part[(map * n_seg + seg) * g.n_pad + (int64_t)c * kChunk + p] = s;
+            part[slot * g.n_pad + (int64_t)c * kChunk + p] = s;
         }
         __syncthreads();
     }
 }
 static_assert(kBlock == kChunk, "recon reduction assumes one thread per chunk pixel");
 
+// F[map][ring[p]] = sum over the map's slots of part[slot][p].
 __global__ void __launch_bounds__(256) recon_finalize_kernel(GeomDev g, int32_t n_maps,
                                                              int32_t n_seg, int64_t n_pix,
                                                              const float *__restrict__ part,
@@ -568,8 +616,10 @@ __global__ void __launch_bounds__(256) recon_finalize_kernel(GeomDev g, int32_t 
     const int64_t map = idx / g.n_pad, pi = idx % g.n_pad;
     const int r = g.ring[pi];
     if (r < 0) return;
+    const int64_t s0 = g.map_slot0 ? g.map_slot0[map] : map * n_seg;
+    const int64_t s1 = g.map_slot0 ? g.map_slot0[map + 1] : (map + 1) * n_seg;
     float s = 0.0f;
-    for (int q = 0; q < n_seg; ++q) s += part[(map * n_seg + q) * g.n_pad + pi];
+    for (int64_t q = s0; q < s1; ++q) s += part[q * g.n_pad + pi];
     F[map * n_pix + r] = s;
 }
 
@@ -583,8 +633,9 @@ hipError_t launch_orbit(const OrbitArgs &a, const GeomDev &g, double4 *pos, CapR
     hipLaunchKernelGGL(orbit_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, st, a, pos, cap);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(uvw_kernel, dim3(blocks_for(g.n_vis, 256)), dim3(256), 0, st, g,
-                       a.inv_lambda_km, uvw64, uvw32);
+    if (g.n_vis > 0)
+        hipLaunchKernelGGL(uvw_kernel, dim3(blocks_for(g.n_vis, 256)), dim3(256), 0, st, g,
+                           a.inv_lambda_km, uvw64, uvw32);
     return hipGetLastError();
 }
 
@@ -602,15 +653,27 @@ hipError_t launch_to_chunk_order(const GeomDev &g, int32_t n_maps, int64_t n_pix
     return hipGetLastError();
 }
 
-hipError_t launch_visgen(const GeomDev &g, const PhaseKnobs &k, bool f64_phase, int32_t n_maps,
-                         const float *Ich, float2 *V, int grid, hipStream_t st) {
-    if (f64_phase)
-        hipLaunchKernelGGL(visgen_kernel<true>, dim3(grid), dim3(kBlock), 0, st, g, k, n_maps,
-                           Ich, V);
-    else
-        hipLaunchKernelGGL(visgen_kernel<false>, dim3(grid), dim3(kBlock), 0, st, g, k, n_maps,
-                           Ich, V);
+template <bool F64, int VT>
+static hipError_t launch_visgen_t(const GeomDev &g, const PhaseKnobs &k, int32_t n_maps,
+                                  const float *Ich, float2 *V, int grid, hipStream_t st) {
+    hipLaunchKernelGGL((visgen_kernel<F64, VT>), dim3(grid), dim3(kBlock), 0, st, g, k, n_maps,
+                       Ich, V);
     return hipGetLastError();
+}
+
+hipError_t launch_visgen(const GeomDev &g, const PhaseKnobs &k, bool f64, int vt, int32_t n_maps,
+                         const float *Ich, float2 *V, int grid, hipStream_t st) {
+    switch (vt) {
+        case 8:
+            return f64 ? launch_visgen_t<true, 8>(g, k, n_maps, Ich, V, grid, st)
+                       : launch_visgen_t<false, 8>(g, k, n_maps, Ich, V, grid, st);
+        case 32:
+            return f64 ? launch_visgen_t<true, 32>(g, k, n_maps, Ich, V, grid, st)
+                       : launch_visgen_t<false, 32>(g, k, n_maps, Ich, V, grid, st);
+        default:
+            return f64 ? launch_visgen_t<true, 16>(g, k, n_maps, Ich, V, grid, st)
+                       : launch_visgen_t<false, 16>(g, k, n_maps, Ich, V, grid, st);
+    }
 }
 
 hipError_t launch_recon(const GeomDev &g, const PhaseKnobs &k, bool f64_phase, int32_t n_maps,
@@ -632,10 +695,15 @@ hipError_t launch_recon_finalize(const GeomDev &g, int32_t n_maps, int32_t n_seg
     return hipGetLastError();
 }
 
-int visgen_tile_vis() { return kVisTile; }
+int visgen_tile_vis() { return 16; }
+int list_pad() { return kListPad; }
 int recon_stage_vis() { return kReconStage; }
-const void *visgen_kernel_ptr(bool f64) {
-    return f64 ? (const void *)visgen_kernel<true> : (const void *)visgen_kernel<false>;
+const void *visgen_kernel_ptr(bool f64, int vt) {
+    switch (vt) {
+        case 8: return f64 ? (const void *)visgen_kernel<true, 8> : (const void *)visgen_kernel<false, 8>;
+        case 32: return f64 ? (const void *)visgen_kernel<true, 32> : (const void *)visgen_kernel<false, 32>;
+        default: return f64 ? (const void *)visgen_kernel<true, 16> : (const void *)visgen_kernel<false, 16>;
+    }
 }
 const void *recon_kernel_ptr(bool f64) {
     return f64 ? (const void *)recon_kernel<true> : (const void *)recon_kernel<false>;

@@ -23,6 +23,14 @@ struct GeomDev {
     const int32_t *ring;   // [n_pad]
     const ChunkRec *chunk; // [n_chunk]
     LimbConsts limb;
+    // Ragged batch ("list mode"); all null for a single geometry.
+    const int2 *vis_sat;       // [n_vis] (sat_a * n_t + t, sat_b * n_t + t); x < 0: padding
+    const int32_t *tile_map;   // [n_vis / list_pad()] map (channel) of each padded group
+    const double *inv_lambda;  // [n_maps] 1e3 / lambda_m: km -> wavelengths
+    const int4 *slots;         // [n_slots] (map, v_begin, v_end, 0) recon visibility ranges
+    const int32_t *map_slot0;  // [n_maps + 1] first slot of each map
+    const float *kappa_map;    // [n_maps] recon normalisation of each map
+    int64_t n_slots;
 };
 
 struct OrbitArgs {
@@ -50,8 +58,9 @@ hipError_t launch_to_chunk_order(const GeomDev &g, int32_t n_maps, int64_t n_pix
                                  const float *I, float *Ich, hipStream_t st);
 
 // Stage 3.  Ich: [n_maps][n_pad]; V: [n_maps][n_vis] complex64.
-hipError_t launch_visgen(const GeomDev &g, const PhaseKnobs &k, bool f64_phase, int32_t n_maps,
-                         const float *Ich, float2 *V, int grid, hipStream_t st);
+// vt = visibilities per tile (8, 16 or 32).
+hipError_t launch_visgen(const GeomDev &g, const PhaseKnobs &k, bool f64_phase, int vt,
+                         int32_t n_maps, const float *Ich, float2 *V, int grid, hipStream_t st);
 
 // Stage 4.  V: [n_maps][n_vis] complex64; w: [n_vis] or null; part: [n_maps][n_seg][n_pad].
 hipError_t launch_recon(const GeomDev &g, const PhaseKnobs &k, bool f64_phase, int32_t n_maps,
@@ -61,9 +70,10 @@ hipError_t launch_recon(const GeomDev &g, const PhaseKnobs &k, bool f64_phase, i
 hipError_t launch_recon_finalize(const GeomDev &g, int32_t n_maps, int32_t n_seg, int64_t n_pix,
                                  const float *part, float *F, hipStream_t st);
 
-int visgen_tile_vis();           // visibilities per vis-gen tile
+int visgen_tile_vis();           // default visibilities per vis-gen tile
+int list_pad();                  // list-mode padding granule (entries)
 int recon_stage_vis();           // visibilities staged per recon LDS tile
-const void *visgen_kernel_ptr(bool f64_phase);
+const void *visgen_kernel_ptr(bool f64_phase, int vt);
 const void *recon_kernel_ptr(bool f64_phase);
 const void *shade_kernel_ptr();
 
