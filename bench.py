@@ -38,6 +38,7 @@ def parse():
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--cpu-stride", type=int, default=32)
     ap.add_argument("--no-mfma", action="store_true", help="skip the multi-map matrix-core legs")
+    ap.add_argument("--no-batch", action="store_true", help="skip the multi-channel batch leg")
     return ap.parse_args()
 
 
@@ -141,6 +142,37 @@ def mfma_legs(torch, ctx, vrs, W, wl, g, pos, uvw, lmn, args):
     return res
 
 
+def batch_leg(torch, ctx, vrs, W, args):
+    """Batched form: 32 channels log-spaced over 1-30 MHz, per-channel steps
+    T_c ~ U{2000..8640} (seed 2), line-of-sight compaction, Nside 64,
+    I_c = I0 (nu_c / 1 MHz)^alpha_c with alpha_c ~ N(-2.5, 0.1); one captured
+    hipGraph per step, autotuned tiles.  Terms = sum_c kept_c x N_pix per direction."""
+    freqs, n_t, alpha = W.c4_channels()
+    nside = 64
+    g = ctx.geometry(16, int(n_t.max()), 1e7, nside)
+    b = ctx.batch(g, freqs, n_t, compact=True, precision="mixed", autotune=True)
+    I0 = W.make_sky(W.Workload("c4", 16, int(n_t.max()), 1e7, nside, 1, [(1, 1.0, 0, 1.0, 1e3)]))[0]
+    I = np.stack([I0 * (f / 1e6) ** a for f, a in zip(freqs, alpha)]).astype(np.float32)
+    It = torch.from_numpy(I).cuda()
+    k = vrs.knobs(precision="mixed")
+
+    def step():
+        b.run(It, k, want_V=False)
+
+    ms = timed_steps(torch, step, max(2, args.steps // 2), 1)
+    kt = kernel_avg_ms(ctx, step, ["batch"], 2)
+    terms = 2 * int(b.kept.sum()) * 12 * nside * nside
+    out = {"value": round(terms / (ms * 1e-3) / 1e9, 1), "unit": "Gterms/s (vis x pix)",
+           "ms_per_step": round(ms, 3), "graph_ms": round(kt["batch"], 3),
+           "channels": len(freqs), "kept_samples": int(b.kept.sum()),
+           "samples_before_compaction": int(120 * n_t.sum()),
+           "tiles": {"vis_tile": b.info["vis_tile"], "recon_seg": b.info["recon_seg"]},
+           "precision": "VRS_MIXED_F64PHASE"}
+    b.close()
+    g.close()
+    return out
+
+
 def accuracy(g, wl, pos, uvw, lmn, I, V, F):
     import oracle as o
     sel = np.linspace(0, wl.n_vis - 1, 32).astype(np.int64)
@@ -224,6 +256,8 @@ def main():
     }
     if not args.no_mfma:
         out["mfma_split16"] = mfma_legs(torch, ctx, vrs, W, wl, g, pos, uvw, lmn, args)
+    if not args.no_batch:
+        out["batch_c4"] = batch_leg(torch, ctx, vrs, W, args)
     if not args.no_accuracy:
         out["accuracy"] = {"plain": accuracy(g, wl, pos, uvw, lmn, I, Vp, Fp),
                            "mixed": accuracy(g, wl, pos, uvw, lmn, I, Vm, Fm)}

@@ -18,6 +18,10 @@ pytestmark = pytest.mark.gpu
 
 FREQS = [1e6, 4e6, 2.5e6]
 NT = [96, 40, 70]
+# orbit planes 0.25 rad apart: satellites see each other part of the time, so the
+# line-of-sight compaction keeps an uneven share of samples per channel
+RAAN = np.array([0.0, 0.25, 0.5, 0.75])
+M0 = np.array([0.0, 0.3, 0.6, 0.9])
 
 
 def cplx(V):
@@ -31,7 +35,7 @@ def rel(a, b):
 @pytest.fixture(scope="module")
 def setup(ctx):
     wl = W.CONFIGS["small"]
-    g = ctx.workload_geometry(wl)
+    g = ctx.geometry(wl.n_sat, wl.n_t, wl.freq_hz, wl.nside, raan=RAAN, m0=M0)
     rng = np.random.default_rng(9)
     I = np.exp(rng.standard_normal((len(FREQS), wl.n_pix))).astype(np.float32)
     return wl, g, I
@@ -45,10 +49,11 @@ def test_batch_matches_per_channel_runs(ctx, oracle, setup, prec):
     V, F = b.run(I, k)
     print(f"[batch {prec}] info {b.info} kept {list(b.kept)}", flush=True)
     assert b.info["n_vis_total"] == int(b.kept.sum())
-    assert b.kept.max() <= g.n_base * max(NT) and (b.kept > 0).all()
+    assert (b.kept > 0).all()
+    assert b.kept.sum() < g.n_base * sum(NT)   # compaction dropped samples
     lmn = g.lmn()
     for c, (f, nt) in enumerate(zip(FREQS, NT)):
-        gc = ctx.geometry(wl.n_sat, nt, f, wl.nside)
+        gc = ctx.geometry(wl.n_sat, nt, f, wl.nside, raan=RAAN, m0=M0)
         Vc = cplx(gc.visscal(np.ascontiguousarray(I[c:c + 1]), k))[0]
         sl = slice(b.offsets[c], b.offsets[c + 1])
         bt = b.sample_bt[sl]
