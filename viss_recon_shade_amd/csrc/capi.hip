@@ -764,10 +764,11 @@ static vrs_status recon_common(vrs_ctx *ctx, const vrs_geom *g, const vrs_knobs 
     // a multiple of 8 so that each XCD keeps to its own segments
     // (independent of n_maps, so that each map's sums are grouped the same way)
     const int64_t units = (int64_t)n_maps * g->n_chunk;
+    // a multiple of 8 segments, so that the blocks on each XCD stream their own segments
     int64_t n_seg = (4096 + g->n_chunk - 1) / g->n_chunk;
-    n_seg = std::min<int64_t>(n_seg, std::max<int64_t>(1, g->n_vis / 256));
-    if (n_seg > 8) n_seg = (n_seg + 7) / 8 * 8;
-    n_seg = std::max<int64_t>(1, std::min<int64_t>(n_seg, 1024));
+    n_seg = (n_seg + 7) / 8 * 8;
+    while (n_seg > 8 && g->n_vis / n_seg < 1024) n_seg -= 8;
+    n_seg = std::min<int64_t>(n_seg, 1024);
     void *part;
     if (vrs_status s = ws_get(ctx, 1, sizeof(float) * n_seg * n_maps * g->n_pad, &part)) return s;
     const GeomDev gd = g->dev();

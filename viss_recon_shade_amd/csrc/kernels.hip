@@ -453,18 +453,25 @@ __global__ void __launch_bounds__(kBlock) visgen_kernel(GeomDev g, PhaseKnobs k,
 }
 
 // ------------------------------------------------------------------ stage 4
-constexpr int kReconStage = 64;  // visibilities staged in LDS per step
+constexpr int kReconStage = kBlock;  // visibilities staged in LDS per step (one per thread)
 
-template <bool F64> struct RecEnt {
+// Per-visibility data the pixel loop reads for every staged visibility ...
+template <bool F64> struct RecHot {
     typename PhaseT<F64>::T u, v, w, wo;
-    CapRec a, b;
     float vr, vi;
-    int32_t ia, ib;
+    int32_t cls, pad;
+};
+// ... and what it reads only when the chunk straddles a limb.
+struct RecCold {
+    CapRec a, b;
+    int32_t ia, ib, pad0, pad1;
 };
 
 // Work item = (slot, chunk); a slot is a visibility range of one map.  In
 // geometry mode slot = map * n_seg + seg over equal segments; in list mode
-// the slots come from g.slots.
+// the slots come from g.slots.  Each step stages kReconStage visibilities
+// (one per thread), classifies them against the block's chunk once, and
+// compacts away those that shade the whole chunk.
 template <bool F64>
 __global__ void __launch_bounds__(kBlock) recon_kernel(GeomDev g, PhaseKnobs k, int32_t n_maps,
                                                        const float2 *__restrict__ V,
@@ -472,7 +479,9 @@ __global__ void __launch_bounds__(kBlock) recon_kernel(GeomDev g, PhaseKnobs k, 
                                                        float kappa, int32_t n_seg,
                                                        float *__restrict__ part) {
     using T = typename PhaseT<F64>::T;
-    __shared__ RecEnt<F64> ents[kReconStage];
+    __shared__ RecHot<F64> hot[kReconStage];
+    __shared__ RecCold cold[kReconStage];
+    __shared__ int wave_live[kWavesPerBlock];
     __shared__ float red[kWavesPerBlock][kChunk];
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x >> 6;
@@ -482,7 +491,7 @@ __global__ void __launch_bounds__(kBlock) recon_kernel(GeomDev g, PhaseKnobs k, 
     const int64_t n_items = n_slots * g.n_chunk;
 
     for (int64_t item = blockIdx.x; item < n_items; item += gridDim.x) {
-        // the slot varies fastest so that blocks on one XCD share visibility data
+        // the slot varies fastest: with n_slots a multiple of 8 each XCD keeps to its own slots
         const int64_t slot = item % n_slots;
         const int c = (int)(item / n_slots);
         int64_t map, vb, ve;
@@ -519,55 +528,65 @@ __global__ void __launch_bounds__(kBlock) recon_kernel(GeomDev g, PhaseKnobs k, 
             acc[j] = 0.0f;
         }
         for (int64_t vt = vb; vt < ve; vt += kReconStage) {
-            if (threadIdx.x < kReconStage) {
-                const int64_t v = vt + threadIdx.x;
-                RecEnt<F64> &e = ents[threadIdx.x];
-                int ia = 0, ib = 0;
-                double u, vv, w;
-                if (v < ve && vis_geometry(g, v, map, ia, ib, u, vv, w)) {
-                    e.ia = ia;
-                    e.ib = ib;
-                    e.a = g.cap[ia];
-                    e.b = g.cap[ib];
-                    if (F64) {
-                        e.u = (T)(u * k.s); e.v = (T)(vv * k.s); e.w = (T)(w * k.s);
-                    } else {
-                        const float s = (float)k.s;
-                        e.u = (T)((float)u * s); e.v = (T)((float)vv * s); e.w = (T)((float)w * s);
-                    }
-                    e.wo = k.wfac ? (T)0 : -e.w;
-                    const float2 q = V[voff + v];
-                    const float sc = kap * (wgt ? wgt[v] : 1.0f);
-                    e.vr = q.x * sc;
-                    e.vi = q.y * sc;
+            // ---- stage: one visibility per thread, classified, compacted
+            const int64_t v = vt + threadIdx.x;
+            int ia = 0, ib = 0, cls = kIn;
+            double u = 0, vv = 0, w = 0;
+            CapRec A, B;
+            if (v < ve && vis_geometry(g, v, map, ia, ib, u, vv, w)) {
+                A = g.cap[ia];
+                B = g.cap[ib];
+                cls = classify(ch, A, B);
+            }
+            const bool live = cls != kIn;
+            const uint64_t bal = __ballot(live);
+            if (lane == 0) wave_live[wave] = __popcll(bal);
+            __syncthreads();
+            int base = 0, n_live = 0;
+#pragma unroll
+            for (int q = 0; q < kWavesPerBlock; ++q) {
+                base += (q < wave) ? wave_live[q] : 0;
+                n_live += wave_live[q];
+            }
+            if (live) {
+                const int pos = base + __popcll(bal & ((1ull << lane) - 1ull));
+                RecHot<F64> &e = hot[pos];
+                if (F64) {
+                    e.u = (T)(u * k.s); e.v = (T)(vv * k.s); e.w = (T)(w * k.s);
                 } else {
-                    e.ia = e.ib = 0;
-                    CapRec none = {0.f, 0.f, 0.f, 2.0f, 0.f, 0.f, 0.f, 0.f};
-                    e.a = none; e.b = none;
-                    e.u = e.v = e.w = e.wo = (T)0;
-                    e.vr = e.vi = 0.0f;
+                    const float s = (float)k.s;
+                    e.u = (T)((float)u * s); e.v = (T)((float)vv * s); e.w = (T)((float)w * s);
                 }
+                e.wo = k.wfac ? (T)0 : -e.w;
+                const float2 q = V[voff + v];
+                const float sc = kap * (wgt ? wgt[v] : 1.0f);
+                e.vr = q.x * sc;
+                e.vi = q.y * sc;
+                e.cls = cls;
+                RecCold &o = cold[pos];
+                o.a = A;
+                o.b = B;
+                o.ia = ia;
+                o.ib = ib;
             }
             __syncthreads();
-            const int nstage = (int)((ve - vt < kReconStage) ? ve - vt : kReconStage);
+            // ---- pixel loop over the live visibilities
 #pragma unroll 2
-            for (int q = wave; q < nstage; q += kWavesPerBlock) {
-                const RecEnt<F64> &e = ents[q];
-                const CapRec A = e.a, B = e.b;
-                const int cls = __builtin_amdgcn_readfirstlane(classify(ch, A, B));
-                if (cls == kIn) continue;  // shaded for every pixel of the chunk
+            for (int q = wave; q < n_live; q += kWavesPerBlock) {
+                const RecHot<F64> e = hot[q];
                 float vr[kPixPerLane], vi[kPixPerLane];
-                if (cls == kStraddle) {
+                if (__builtin_amdgcn_readfirstlane(e.cls) == kStraddle) {
+                    const CapRec A2 = cold[q].a, B2 = cold[q].b;
 #pragma unroll
                     for (int j = 0; j < kPixPerLane; ++j) {
                         bool unc = false;
-                        bool sh = occ32(sx[j], sy[j], sz[j], A, unc) |
-                                  occ32(sx[j], sy[j], sz[j], B, unc);
+                        bool sh = occ32(sx[j], sy[j], sz[j], A2, unc) |
+                                  occ32(sx[j], sy[j], sz[j], B2, unc);
                         if (unc) {
                             int nl;
                             sh = occ_pair64(g.pos, g.pix64,
-                                            (int64_t)c * kChunk + j * kWave + lane, e.ia, e.ib,
-                                            g.limb, &nl);
+                                            (int64_t)c * kChunk + j * kWave + lane, cold[q].ia,
+                                            cold[q].ib, g.limb, &nl);
                         }
                         vr[j] = sh ? 0.0f : e.vr;
                         vi[j] = sh ? 0.0f : e.vi;
@@ -579,10 +598,9 @@ __global__ void __launch_bounds__(kBlock) recon_kernel(GeomDev g, PhaseKnobs k, 
                         vi[j] = e.vi;
                     }
                 }
-                const T u = e.u, vv = e.v, w = e.w, wo = e.wo;
 #pragma unroll
                 for (int j = 0; j < kPixPerLane; ++j) {
-                    const T ph = fma(u, l[j], fma(vv, m[j], fma(w, n[j], wo)));
+                    const T ph = fma(e.u, l[j], fma(e.v, m[j], fma(e.w, n[j], e.wo)));
                     const float r = fract_rev(ph);
                     const float sn = __builtin_amdgcn_sinf(r);
                     const float cs = __builtin_amdgcn_cosf(r);
