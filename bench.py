@@ -216,6 +216,8 @@ def main():
         return step
 
     terms = 2 * wl.terms_per_direction
+    tile_plain = g.autotune(It, kp)
+    tile_mixed = g.autotune(It, km)
     ms_plain = timed_steps(torch, step_with(kp), args.steps, args.warmup)
     kt_plain = kernel_avg_ms(ctx, step_with(kp), ["orbit", "visgen", "recon"], max(2, args.steps // 2))
     Vp, Fp = state["V"].cpu().numpy(), state["F"].cpu().numpy()
@@ -250,6 +252,7 @@ def main():
                      "trans_frac": round(2 * wl.terms_per_direction / (dom_ms * 1e-3) / TRANS_PEAK, 4),
                      "kernel_ms": round(dom_ms, 3)},
         "kernels_ms": {k: round(v, 3) for k, v in kt_plain.items()},
+        "visgen_tile": {"plain": tile_plain[0], "mixed": tile_mixed[0]},
         "mixed_f64phase": {"value": round(terms / (ms_mixed * 1e-3) / 1e9, 2),
                            "ms_per_step": round(ms_mixed, 3),
                            "kernels_ms": {k: round(v, 3) for k, v in kt_mixed.items()}},

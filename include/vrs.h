@@ -153,6 +153,13 @@ vrs_status vrs_visgen(vrs_ctx *ctx, const vrs_geom *g, const vrs_knobs *k, int32
  * reference's 1/count weights, vissGen10M_shade.cu:364). F: [n_maps][n_pix] float. */
 vrs_status vrs_recon(vrs_ctx *ctx, const vrs_geom *g, const vrs_knobs *k, int32_t n_maps,
                      const float *V, const float *w, float *F);
+/* Times the vis-gen tile candidates (8, 16, 32 visibilities per block) on this
+ * geometry with input I and caches the fastest for (n_vis, n_pix, n_maps,
+ * precision) in the context; later vrs_visgen calls of that shape use it.
+ * tile_out / ms_out (nullable) report the choice and its time.  Results do not
+ * depend on the tile (each visibility's sums run in the same order). */
+vrs_status vrs_autotune_visgen(vrs_ctx *ctx, const vrs_geom *g, const vrs_knobs *k,
+                               int32_t n_maps, const float *I, int32_t *tile_out, double *ms_out);
 /* Gradient of L(I) = sum_v |V(I)[v] - V_obs[v]|^2 given resid = V(I) - V_obs
  * (complex64 [n_vis]): grad[p] = 2 sum_v (1 - M) Re(resid[v] exp(+2 pi i s psi)).
  * The shaded back-projection; no reference counterpart (the reference has no
@@ -201,6 +208,50 @@ vrs_status vrs_batch_layout(const vrs_batch *b, int64_t *offsets, int64_t *kept,
  * F: [n_ch][n_pix] float.  kappa = 0 normalises each channel by 1 / kept[c]. */
 vrs_status vrs_batch_run(vrs_ctx *ctx, vrs_batch *b, const vrs_knobs *k, const float *I, float *V,
                          float *F);
+
+/* ------------------------------------------------- reference compatibility */
+/* The reference's box predicate (ShadeMatrix, shade10M.cu:73-105, with
+ * Cart2SphKernel :62-70 and the mean-|uvw| set-up :218-224) in FP32 on the
+ * GPU: per uvw the 1-based index box [thetaS thetaE phiS phiE] on a res x res
+ * (theta, phi) grid around the antipode of the baseline direction.  uvw: [n][3]
+ * float, box4: [n][4] int32 (host or device).  r_moon_ref_km: 1737.74 in the
+ * reference.  Kept for shadeM file interchange; the physical mask is vrs_shade. */
+vrs_status vrs_shade_box(vrs_ctx *ctx, const float *uvw, int64_t n, int32_t res,
+                         float r_moon_ref_km, int32_t *box4);
+
+/* uvw gridding, de-duplication and cell counts: uvw = ceil(dx * 1e3 / lambda)
+ * (computeUVW, MIncline_generate_uvw_float.cu:153-160), all samples sorted
+ * lexicographically by (u, v, w) and made unique (:374-397); counts[i] = number
+ * of gridded samples in unique cell i (the 4th column of the updated_uvw files,
+ * whose 1/count is the recon weight, vissGen10M_shade.cu:357-366).
+ * flags: VRS_GRID_ORDERED = grid every ordered satellite pair m != n as the
+ * reference does (else the geometry's baselines); VRS_GRID_LAMBDA_INT =
+ * lambda = (int)(3e8 / f) as the reference (MIncline_generate_uvw_float.cu:190)
+ * (else c / f).  Positions are the geometry's FP64 ones.
+ * uvw_unique [cap][3] and counts [cap] may be NULL (then only *n_unique is
+ * returned); weight / vis_count [n_vis] (nullable) give each visibility
+ * (baseline a < b, t) 1/count and count of its own cell.  Host or device pointers. */
+#define VRS_GRID_ORDERED 1
+#define VRS_GRID_LAMBDA_INT 2
+vrs_status vrs_grid_uvw(vrs_ctx *ctx, const vrs_geom *g, int32_t flags, int64_t cap,
+                        float *uvw_unique, int32_t *counts, int64_t *n_unique, float *weight,
+                        int32_t *vis_count);
+
+/* Text files of the reference pipeline, same layout and default ostream
+ * formatting (host memory only; no GPU needed):
+ *   uvw[/updated_uvw]  "u v w[ count]"  MIncline_generate_uvw_float.cu:420-427, read at
+ *                                       vissGen10M_shade.cu:351-366 (col4 nullable)
+ *   shadeM             "tS tE pS pE"    shade10M.cu:108-127; the reader orders each pair
+ *                                       and makes it 0-based (vissGen10M_shade.cu:385-399)
+ *   F image            one value/line   vissGen10M_shade.cu:563-583
+ * Readers stop with VRS_E_ARG instead of writing past `cap` rows. */
+vrs_status vrs_write_uvw_txt(const char *path, const double *uvw, int64_t n, const float *col4);
+vrs_status vrs_read_uvw_txt(const char *path, int32_t n_cols, double *uvw, float *col4,
+                            int64_t cap, int64_t *n_read);
+vrs_status vrs_write_shade_txt(const char *path, const int32_t *box4, int64_t n);
+vrs_status vrs_read_shade_txt(const char *path, int32_t *box4, int64_t cap, int64_t *n_read);
+vrs_status vrs_write_image_txt(const char *path, const float *F, int64_t n);
+vrs_status vrs_read_image_txt(const char *path, float *F, int64_t cap, int64_t *n_read);
 
 /* ------------------------------------------------------- host-side helpers */
 /* Synthetic inputs (no reference counterpart; the reference reads MATLAB files).

@@ -414,8 +414,8 @@ void vo_shade_box(int64_t n, const float *u, const float *v, const float *w, int
         sum_r += sqrtf(u[i] * u[i] + v[i] * v[i] + w[i] * w[i]);
     float mean_r = sum_r / (float)n;
     float r0 = atanf(r_moon_ref / (mean_r + r_moon_ref));
-    float dtheta = 2.0f * (float)VO_PI / (float)res;
-    float dphi = (float)VO_PI / (float)res;
+    float dtheta = (float)(2.0 * VO_PI / (double)res); /* float dtheta = 2 * M_PI / RES */
+    float dphi = (float)(VO_PI / (double)res);
     for (int64_t i = 0; i < n; ++i) {
         float r = sqrtf(u[i] * u[i] + v[i] * v[i] + w[i] * w[i]);
         float th = atan2f(v[i], u[i]);
@@ -435,4 +435,53 @@ void vo_shade_box(int64_t n, const float *u, const float *v, const float *w, int
         if (pE > res) pE -= res;
         out4[4 * i] = tS; out4[4 * i + 1] = tE; out4[4 * i + 2] = pS; out4[4 * i + 3] = pE;
     }
+}
+
+/* ------------------------------------------------ uvw gridding and counts */
+/* computeUVW with ceil (MIncline_generate_uvw_float.cu:156-158) for the given
+ * pairs, sorted lexicographically by (u, v, w) (SortCompare, :60-70) and made
+ * unique (:374-397); counts = samples per unique cell.  Keys pack
+ * (u + 2^20, v + 2^20, w + 2^20) into 21-bit fields.  Returns the number of
+ * unique cells, or -1 when a component leaves (-2^20, 2^20). */
+static int vo_cmp_u64(const void *a, const void *b)
+{
+    uint64_t x = *(const uint64_t *)a, y = *(const uint64_t *)b;
+    return x < y ? -1 : (x > y);
+}
+
+int64_t vo_grid_keys(int64_t n_t, const double *pos, int n_pairs, const int *pairs, double sc,
+                     uint64_t *keys)
+{
+    for (int p = 0; p < n_pairs; ++p)
+        for (int64_t t = 0; t < n_t; ++t) {
+            const double *ra = pos + ((int64_t)pairs[2 * p] * n_t + t) * 3;
+            const double *rb = pos + ((int64_t)pairs[2 * p + 1] * n_t + t) * 3;
+            double u = ceil((rb[0] - ra[0]) * sc), v = ceil((rb[1] - ra[1]) * sc),
+                   w = ceil((rb[2] - ra[2]) * sc);
+            const double B = 1048576.0;
+            if (!(fabs(u) < B && fabs(v) < B && fabs(w) < B)) return -1;
+            keys[(int64_t)p * n_t + t] = ((uint64_t)((int64_t)u + 1048576) << 42) |
+                                         ((uint64_t)((int64_t)v + 1048576) << 21) |
+                                         (uint64_t)((int64_t)w + 1048576);
+        }
+    return (int64_t)n_pairs * n_t;
+}
+
+int64_t vo_grid_unique(int64_t n, uint64_t *keys, float *uvw_unique, int32_t *counts)
+{
+    qsort(keys, (size_t)n, sizeof(uint64_t), vo_cmp_u64);
+    int64_t nu = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        if (i == 0 || keys[i] != keys[i - 1]) {
+            const uint64_t k = keys[i];
+            const uint64_t m = (1ull << 21) - 1;
+            uvw_unique[3 * nu] = (float)((int64_t)((k >> 42) & m) - 1048576);
+            uvw_unique[3 * nu + 1] = (float)((int64_t)((k >> 21) & m) - 1048576);
+            uvw_unique[3 * nu + 2] = (float)((int64_t)(k & m) - 1048576);
+            counts[nu] = 0;
+            ++nu;
+        }
+        counts[nu - 1] += 1;
+    }
+    return nu;
 }

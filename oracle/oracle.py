@@ -62,6 +62,11 @@ def _load():
     L.vo_recon.argtypes = [C.c_int64, _f64, _i32, C.c_int, _f64, C.c_int64, _f64, C.c_int,
                            _f64, C.c_void_p, C.c_double, C.c_double, C.c_int, C.c_double,
                            C.c_int64, C.c_void_p, _f64]
+    L.vo_grid_keys.restype = C.c_int64
+    L.vo_grid_keys.argtypes = [C.c_int64, _f64, C.c_int, _i32, C.c_double,
+                               _P(dtype=np.uint64, flags="C_CONTIGUOUS")]
+    L.vo_grid_unique.restype = C.c_int64
+    L.vo_grid_unique.argtypes = [C.c_int64, _P(dtype=np.uint64, flags="C_CONTIGUOUS"), _f32, _i32]
     L.vo_shade_box.argtypes = [C.c_int64, _f32, _f32, _f32, C.c_int, C.c_float, _i32]
     B = C.CDLL(sb)
     B.vb_visgen_f32.argtypes = [C.c_int64, _f32, _i32, _f32, C.c_int64, _f32, _f32, C.c_float,
@@ -222,6 +227,33 @@ def recon(pos, pairs, uvw_, lmn, V, R, s=1.0, wfac=0, kappa=None, weights=None, 
                   np.ascontiguousarray(lmn, np.float64), V.shape[0], Vr, wp, R, s, wfac,
                   kappa, n_sel, selp, out)
     return out
+
+
+def grid_uvw(pos, pairs, lambda_m, vis_pairs=None):
+    """Gridded, sorted, unique uvw with per-cell counts (MIncline_generate_uvw_float.cu:
+    153-160, 374-397).  Returns (uvw_unique [n][3] float32, counts [n] int32,
+    vis_count [len(vis_pairs) * n_t] or None)."""
+    _load()
+    pos = np.ascontiguousarray(pos, np.float64)
+    pairs = np.ascontiguousarray(pairs, np.int32)
+    n_t = pos.shape[1]
+    keys = np.empty(len(pairs) * n_t, np.uint64)
+    sc = 1e3 / lambda_m
+    if _lib.vo_grid_keys(n_t, pos, len(pairs), pairs, sc, keys) < 0:
+        raise ValueError("gridded uvw out of range")
+    uu = np.empty((len(keys), 3), np.float32)
+    cc = np.empty(len(keys), np.int32)
+    sorted_keys = keys.copy()
+    nu = _lib.vo_grid_unique(len(keys), sorted_keys, uu, cc)
+    vc = None
+    if vis_pairs is not None:
+        vp = np.ascontiguousarray(vis_pairs, np.int32)
+        vk = np.empty(len(vp) * n_t, np.uint64)
+        _lib.vo_grid_keys(n_t, pos, len(vp), vp, sc, vk)
+        uk = np.unique(sorted_keys)
+        idx = np.searchsorted(uk, vk)
+        vc = cc[:nu][idx]
+    return uu[:nu], cc[:nu], vc
 
 
 def shade_box(u, v, w, res, r_moon_ref=1737.74):

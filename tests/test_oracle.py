@@ -305,3 +305,23 @@ def test_shade_box_reference_formula(oracle):
     ti = round(math.pi / dth)
     pi_ = round((math.pi / 2) / (math.pi / res))
     assert list(out) == [ti - tR, ti + tR, pi_ - pR, pi_ + pR]
+
+
+def test_grid_uvw_matches_numpy_unique(oracle):
+    # the oracle's sort/unique/count against numpy's lexsort-based unique
+    raan = np.array([0.0, 2.1, 4.2, 1.0])
+    pos = oracle.orbit_positions(40, 5.0, A, INCL, MU, raan, np.array([0, .1, .2, .3]))
+    pairs = [(m, n) for m in range(4) for n in range(4) if m != n]
+    lam = float(int(3e8 / 3e6))
+    u, c, vc = oracle.grid_uvw(pos, pairs, lam, vis_pairs=[(0, 1), (2, 3)])
+    raw = np.concatenate([np.ceil((pos[n] - pos[m]) * 1e3 / lam) for m, n in pairs])
+    uu, cnt = np.unique(raw, axis=0, return_counts=True)   # lexicographic row order
+    assert np.array_equal(u.astype(np.float64), uu) and np.array_equal(c, cnt)
+    assert c.sum() == len(pairs) * 40
+    # ceil breaks antisymmetry: (m, n) and (n, m) cells differ (reference quirk Q6)
+    g01 = np.ceil((pos[1] - pos[0]) * 1e3 / lam)
+    g10 = np.ceil((pos[0] - pos[1]) * 1e3 / lam)
+    assert not np.array_equal(g01, -g10)
+    # per-visibility counts are the counts of the visibility's own cell
+    row = {tuple(r): k for r, k in zip(uu, cnt)}
+    assert list(vc[:40]) == [row[tuple(r)] for r in g01]

@@ -16,6 +16,8 @@ STATUS_NAMES = {0: "VRS_OK", -1: "VRS_E_ARG", -2: "VRS_E_OOM", -3: "VRS_E_HIP", 
 VRS_PLAIN_F32 = 0
 VRS_MIXED_F64PHASE = 1
 VRS_MFMA_SPLIT16 = 2
+VRS_GRID_ORDERED = 1
+VRS_GRID_LAMBDA_INT = 2
 
 
 class VrsError(RuntimeError):
@@ -90,11 +92,23 @@ SIGNATURES = {
     "vrs_visgen": (C.c_int, [_ctx, _vp, C.POINTER(Knobs), C.c_int32, _vp, _vp]),
     "vrs_recon": (C.c_int, [_ctx, _vp, C.POINTER(Knobs), C.c_int32, _vp, _vp, _vp]),
     "vrs_adjoint": (C.c_int, [_ctx, _vp, C.POINTER(Knobs), _vp, _vp]),
+    "vrs_autotune_visgen": (C.c_int, [_ctx, _vp, C.POINTER(Knobs), C.c_int32, _vp,
+                                      C.POINTER(C.c_int32), C.POINTER(C.c_double)]),
     "vrs_batch_create": (C.c_int, [_ctx, _vp, C.POINTER(BatchDesc), C.POINTER(C.c_void_p)]),
     "vrs_batch_destroy": (None, [_vp]),
     "vrs_batch_info_get": (C.c_int, [_vp, C.POINTER(BatchInfo)]),
     "vrs_batch_layout": (C.c_int, [_vp, _vp, _vp, _vp]),
     "vrs_batch_run": (C.c_int, [_ctx, _vp, C.POINTER(Knobs), _vp, _vp, _vp]),
+    "vrs_shade_box": (C.c_int, [_ctx, _vp, C.c_int64, C.c_int32, C.c_float, _vp]),
+    "vrs_grid_uvw": (C.c_int, [_ctx, _vp, C.c_int32, C.c_int64, _vp, _vp, C.POINTER(C.c_int64),
+                               _vp, _vp]),
+    "vrs_write_uvw_txt": (C.c_int, [C.c_char_p, _vp, C.c_int64, _vp]),
+    "vrs_read_uvw_txt": (C.c_int, [C.c_char_p, C.c_int32, _vp, _vp, C.c_int64,
+                                   C.POINTER(C.c_int64)]),
+    "vrs_write_shade_txt": (C.c_int, [C.c_char_p, _vp, C.c_int64]),
+    "vrs_read_shade_txt": (C.c_int, [C.c_char_p, _vp, C.c_int64, C.POINTER(C.c_int64)]),
+    "vrs_write_image_txt": (C.c_int, [C.c_char_p, _vp, C.c_int64]),
+    "vrs_read_image_txt": (C.c_int, [C.c_char_p, _vp, C.c_int64, C.POINTER(C.c_int64)]),
     "vrs_rng_normals": (C.c_int, [C.c_uint64, C.c_int64, _vp]),
     "vrs_rng_uniforms": (C.c_int, [C.c_uint64, C.c_int64, _vp]),
     "vrs_sky_lognormal": (C.c_int, [C.c_uint64, C.c_double, C.c_int64, C.c_int32, C.c_double,

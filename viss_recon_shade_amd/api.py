@@ -98,6 +98,19 @@ class Context:
     def workload_geometry(self, wl):
         return self.geometry(wl.n_sat, wl.n_t, wl.freq_hz, wl.nside)
 
+    def shade_box(self, uvw, res, r_moon_ref_km=1737.74):
+        """The reference's ShadeMatrix index boxes (1-based [thetaS thetaE phiS phiE])
+        for FP32 uvw [n][3] on a res x res grid (shade10M.cu:73-105)."""
+        uvw = uvw if _is_torch(uvw) else np.ascontiguousarray(uvw, np.float32)
+        n = uvw.shape[0]
+        out = _empty_like_kind(uvw, (n, 4), np.int32)
+        self._sync_stream(uvw)
+        self.check(self._lib.vrs_shade_box(self.handle, _ptr(uvw), n, res, r_moon_ref_km,
+                                           _ptr(out)))
+        return out
+
+    ShadeMatrix_box = shade_box
+
     def batch(self, geom, freqs_hz, n_t, compact=True, precision="mixed", autotune=True):
         return Batch(self, geom, freqs_hz, n_t, compact, precision, autotune)
 
@@ -174,6 +187,24 @@ class Geometry:
 
     orbit_uvw = MIncline
 
+    def grid_uvw(self, ordered=True, lambda_int=True, weights=True):
+        """Gridded (ceil), de-duplicated, lexicographically sorted uvw with cell
+        counts -- the reference's uvw / updated_uvw content -- and per-visibility
+        1/count weights and counts.  Returns a dict."""
+        lib = self.ctx._lib
+        flags = (_lib.VRS_GRID_ORDERED if ordered else 0) | (_lib.VRS_GRID_LAMBDA_INT if lambda_int else 0)
+        n = C.c_int64()
+        self.ctx.check(lib.vrs_grid_uvw(self.ctx.handle, self.handle, flags, 0, None, None,
+                                        C.byref(n), None, None))
+        uvw = np.empty((max(n.value, 1), 3), np.float32)
+        cnt = np.empty(max(n.value, 1), np.int32)
+        w = np.empty(self.n_vis, np.float32) if weights else None
+        vc = np.empty(self.n_vis, np.int32) if weights else None
+        self.ctx.check(lib.vrs_grid_uvw(self.ctx.handle, self.handle, flags, len(cnt), _ptr(uvw),
+                                        _ptr(cnt), C.byref(n), _ptr(w) if weights else None,
+                                        _ptr(vc) if weights else None))
+        return {"uvw": uvw[:n.value], "counts": cnt[:n.value], "weight": w, "vis_count": vc}
+
     def lmn(self):
         out = np.empty((self.n_pix, 3), np.float64)
         self.ctx.check(self.ctx._lib.vrs_geom_lmn(self.ctx.handle, self.handle, _ptr(out)))
@@ -211,6 +242,18 @@ class Geometry:
         return V
 
     visgen = visscal
+
+    def autotune(self, I, k=None):
+        """Time the vis-gen tile candidates on input I; returns (tile, ms)."""
+        k = k or knobs()
+        n_maps = 1 if I.ndim == 1 else I.shape[0]
+        t = C.c_int32()
+        ms = C.c_double()
+        self.ctx._sync_stream(I)
+        self.ctx.check(self.ctx._lib.vrs_autotune_visgen(self.ctx.handle, self.handle, C.byref(k),
+                                                         n_maps, _ptr(I), C.byref(t),
+                                                         C.byref(ms)))
+        return t.value, ms.value
 
     # ------------------------------------------------------------- stage 4
     def imagerecon(self, V, w=None, k=None):

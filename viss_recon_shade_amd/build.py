@@ -19,7 +19,7 @@ LIB = os.path.join(PKG, "libvrs.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["capi.hip", "kernels.hip", "mfma.hip", "host_sky.cpp"]
+SOURCES = ["capi.hip", "kernels.hip", "mfma.hip", "refcompat.hip", "grid.hip", "host_sky.cpp", "io.cpp"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-I", os.path.join(ROOT, "include"), "-I", CSRC,
           "-Wall", "-Wno-unused-function"]

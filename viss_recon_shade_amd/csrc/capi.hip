@@ -17,6 +17,9 @@
 #include "host_sky.hpp"
 #include "kernels.hpp"
 #include "mfma.hpp"
+#include "io.hpp"
+#include "refcompat.hpp"
+#include "grid.hpp"
 #include "vrs.h"
 #include "vrs_internal.hpp"
 
@@ -41,6 +44,8 @@ struct vrs_ctx {
     unsigned long long *d_tot = nullptr;
     // autotuned batch tiles, keyed by (padded visibilities, pixels, channels, precision)
     std::map<std::tuple<int64_t, int64_t, int32_t, int32_t>, std::pair<int, int>> tuned;
+    // autotuned single-geometry vis-gen tiles, keyed by (n_vis, n_pix, n_maps, precision)
+    std::map<std::tuple<int64_t, int64_t, int32_t, int32_t>, int> tuned_vt;
 };
 
 struct vrs_geom {
@@ -48,7 +53,7 @@ struct vrs_geom {
     int device = 0;
     int32_t n_sat = 0, n_t = 0, n_base = 0, nside = 0, n_chunk = 0;
     int64_t n_vis = 0, n_pix = 0, n_pad = 0;
-    double lambda_m = 0, r_moon = 0;
+    double lambda_m = 0, r_moon = 0, freq_hz = 0;
     OrbitArgs orb{};
     std::vector<int32_t> pairs;
     std::vector<double> lmn_ring;  // [n_pix][3] host copy, RING order
@@ -422,6 +427,7 @@ static vrs_status geom_build(vrs_ctx *ctx, const vrs_geom_desc *d, vrs_geom *g) 
     g->nside = d->nside;
     g->r_moon = d->r_moon_km;
     g->lambda_m = kC / d->freq_hz;
+    g->freq_hz = d->freq_hz;
     if (d->n_base == 0) {
         for (int a = 0; a < o.n_sat; ++a)
             for (int b = a + 1; b < o.n_sat; ++b) {
@@ -712,7 +718,11 @@ vrs_status vrs_visgen(vrs_ctx *ctx, const vrs_geom *g, const vrs_knobs *k, int32
     VRS_HIP(ctx, launch_to_chunk_order(gd, n_maps, g->n_pix, (const float *)sI.dev, (float *)ich,
                                        ctx->stream));
     const bool f64 = k->precision == VRS_MIXED_F64PHASE;
-    const int vt = visgen_tile_vis();
+    int vt = visgen_tile_vis();
+    {
+        auto it = ctx->tuned_vt.find(std::make_tuple(g->n_vis, g->n_pix, n_maps, k->precision));
+        if (it != ctx->tuned_vt.end()) vt = it->second;
+    }
     const int64_t n_tiles = ((g->n_vis + vt - 1) / vt) * n_maps;
     const int grid = persistent_grid(ctx, visgen_kernel_ptr(f64, vt), n_tiles);
     {
@@ -721,6 +731,48 @@ vrs_status vrs_visgen(vrs_ctx *ctx, const vrs_geom *g, const vrs_knobs *k, int32
                                    (float2 *)sV.dev, grid, ctx->stream));
     }
     if (vrs_status s = sV.finish()) return s;
+    return VRS_OK;
+}
+
+vrs_status vrs_autotune_visgen(vrs_ctx *ctx, const vrs_geom *g, const vrs_knobs *k,
+                               int32_t n_maps, const float *I, int32_t *tile_out, double *ms_out) {
+    if (vrs_status s = check_ctx(ctx)) return s;
+    if (vrs_status s = check_geom(ctx, g)) return s;
+    if (vrs_status s = check_knobs(ctx, k)) return s;
+    if (k->precision == VRS_MFMA_SPLIT16)
+        return fail(ctx, VRS_E_ARG, "the matrix-core path has fixed tiles");
+    if (n_maps < 1 || n_maps > 65536 || !I) return fail(ctx, VRS_E_ARG, "bad n_maps or I");
+    DeviceGuard dg(ctx->device);
+    void *V;
+    if (vrs_status s = ws_get(ctx, 2, sizeof(float) * 2 * n_maps * g->n_vis, &V)) return s;
+    const auto key = std::make_tuple(g->n_vis, g->n_pix, n_maps, k->precision);
+    hipEvent_t e0, e1;
+    VRS_HIP(ctx, hipEventCreate(&e0));
+    VRS_HIP(ctx, hipEventCreate(&e1));
+    double best = 1e300;
+    int best_vt = visgen_tile_vis();
+    vrs_status st = VRS_OK;
+    for (int rep = 0; rep < 2 && st == VRS_OK; ++rep)     // first round warms up
+        for (int vt : {8, 16, 32}) {
+            ctx->tuned_vt[key] = vt;
+            if (hipEventRecord(e0, ctx->stream) != hipSuccess) { st = VRS_E_HIP; break; }
+            st = vrs_visgen(ctx, g, k, n_maps, I, (float *)V);
+            if (st != VRS_OK) break;
+            float ms = 0;
+            if (hipEventRecord(e1, ctx->stream) != hipSuccess ||
+                hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&ms, e0, e1) != hipSuccess) { st = VRS_E_HIP; break; }
+            if (rep == 1 && ms < best) { best = ms; best_vt = vt; }
+        }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (st != VRS_OK) {
+        ctx->tuned_vt.erase(key);
+        return st == VRS_E_HIP ? fail(ctx, st, "event timing failed during autotune") : st;
+    }
+    ctx->tuned_vt[key] = best_vt;
+    if (tile_out) *tile_out = best_vt;
+    if (ms_out) *ms_out = best;
     return VRS_OK;
 }
 
@@ -1185,3 +1237,166 @@ vrs_status vrs_batch_run(vrs_ctx *ctx, vrs_batch *b, const vrs_knobs *k, const f
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------- reference compatibility
+namespace {
+vrs_status io_status(int rc, const char *what, const char *path) {
+    switch (rc) {
+        case 0: return VRS_OK;
+        case -2: return fail(nullptr, VRS_E_ARG, "%s: more rows in %s than the capacity", what, path);
+        case -3: return fail(nullptr, VRS_E_IO, "%s: truncated row in %s", what, path);
+        default: return fail(nullptr, VRS_E_IO, "%s: cannot open or write %s", what, path);
+    }
+}
+}  // namespace
+
+extern "C" {
+
+vrs_status vrs_shade_box(vrs_ctx *ctx, const float *uvw, int64_t n, int32_t res,
+                         float r_moon_ref_km, int32_t *box4) {
+    if (vrs_status s = check_ctx(ctx)) return s;
+    if (!uvw || !box4 || n < 1) return fail(ctx, VRS_E_ARG, "uvw, box4 and n >= 1 are required");
+    if (res < 2 || !(r_moon_ref_km > 0)) return fail(ctx, VRS_E_ARG, "res >= 2 and r_moon_ref_km > 0");
+    DeviceGuard dg(ctx->device);
+    Staged su(ctx, uvw, sizeof(float) * 3 * n, false);
+    Staged sb(ctx, box4, sizeof(int32_t) * 4 * n, true);
+    if (vrs_status s = su.prepare()) return s;
+    if (vrs_status s = sb.prepare()) return s;
+    void *partial;
+    if (vrs_status s = ws_get(ctx, 3, sizeof(float) * shade_box_partials(), &partial)) return s;
+    {
+        Timer tm(ctx, "shade_box");
+        VRS_HIP(ctx, launch_shade_box((const float *)su.dev, n, res, r_moon_ref_km, (float *)partial,
+                                      (int32_t *)sb.dev, ctx->stream));
+    }
+    return sb.finish();
+}
+
+vrs_status vrs_write_uvw_txt(const char *path, const double *uvw, int64_t n, const float *col4) {
+    if (!path || (n > 0 && !uvw) || n < 0) return fail(nullptr, VRS_E_ARG, "bad path, uvw or n");
+    return io_status(write_uvw_txt(path, uvw, n, col4), "vrs_write_uvw_txt", path);
+}
+
+vrs_status vrs_read_uvw_txt(const char *path, int32_t n_cols, double *uvw, float *col4,
+                            int64_t cap, int64_t *n_read) {
+    if (!path || !uvw || !n_read || cap < 0 || (n_cols != 3 && n_cols != 4))
+        return fail(nullptr, VRS_E_ARG, "bad arguments (n_cols must be 3 or 4)");
+    *n_read = 0;
+    return io_status(read_uvw_txt(path, n_cols, uvw, col4, cap, n_read), "vrs_read_uvw_txt", path);
+}
+
+vrs_status vrs_write_shade_txt(const char *path, const int32_t *box4, int64_t n) {
+    if (!path || (n > 0 && !box4) || n < 0) return fail(nullptr, VRS_E_ARG, "bad path, box4 or n");
+    return io_status(write_shade_txt(path, box4, n), "vrs_write_shade_txt", path);
+}
+
+vrs_status vrs_read_shade_txt(const char *path, int32_t *box4, int64_t cap, int64_t *n_read) {
+    if (!path || !box4 || !n_read || cap < 0) return fail(nullptr, VRS_E_ARG, "bad arguments");
+    *n_read = 0;
+    return io_status(read_shade_txt(path, box4, cap, n_read), "vrs_read_shade_txt", path);
+}
+
+vrs_status vrs_write_image_txt(const char *path, const float *F, int64_t n) {
+    if (!path || (n > 0 && !F) || n < 0) return fail(nullptr, VRS_E_ARG, "bad path, F or n");
+    return io_status(write_image_txt(path, F, n), "vrs_write_image_txt", path);
+}
+
+vrs_status vrs_read_image_txt(const char *path, float *F, int64_t cap, int64_t *n_read) {
+    if (!path || !F || !n_read || cap < 0) return fail(nullptr, VRS_E_ARG, "bad arguments");
+    *n_read = 0;
+    return io_status(read_image_txt(path, F, cap, n_read), "vrs_read_image_txt", path);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- uvw gridding
+extern "C" vrs_status vrs_grid_uvw(vrs_ctx *ctx, const vrs_geom *g, int32_t flags, int64_t cap,
+                                   float *uvw_unique, int32_t *counts, int64_t *n_unique,
+                                   float *weight, int32_t *vis_count) {
+    if (vrs_status s = check_ctx(ctx)) return s;
+    if (vrs_status s = check_geom(ctx, g)) return s;
+    if (!n_unique) return fail(ctx, VRS_E_ARG, "n_unique must be non-NULL");
+    if ((flags & ~(VRS_GRID_ORDERED | VRS_GRID_LAMBDA_INT)) != 0)
+        return fail(ctx, VRS_E_ARG, "unknown grid flags 0x%x", flags);
+    if ((uvw_unique || counts) && cap < 1) return fail(ctx, VRS_E_ARG, "cap must be >= 1");
+    DeviceGuard dg(ctx->device);
+    std::vector<int32_t> pr;
+    if (flags & VRS_GRID_ORDERED) {
+        for (int m = 0; m < g->n_sat; ++m)
+            for (int n = 0; n < g->n_sat; ++n)
+                if (m != n) { pr.push_back(m); pr.push_back(n); }
+    } else {
+        pr = g->pairs;
+    }
+    const int n_pairs = (int)(pr.size() / 2);
+    const int64_t n = (int64_t)n_pairs * g->n_t;
+    if (n > ((int64_t)1 << 31) - 1) return fail(ctx, VRS_E_ARG, "too many gridded samples");
+    const double lam = (flags & VRS_GRID_LAMBDA_INT) ? (double)(int)(3e8 / g->freq_hz) : g->lambda_m;
+    if (!(lam > 0)) return fail(ctx, VRS_E_ARG, "wavelength rounds to zero");
+    const double sc = 1e3 / lam;
+    // scratch (not the hot path: plain allocations, freed on return)
+    struct Buf {
+        void *p = nullptr;
+        ~Buf() { if (p) (void)hipFree(p); }
+    } b_pairs, b_keys, b_tmp, b_isc, b_small, b_ukeys, b_cnt, b_vkeys, b_bad;
+    const int64_t nb = (n + grid_tile() - 1) / grid_tile();
+    VRS_HIP(ctx, hipMalloc(&b_pairs.p, sizeof(int32_t) * pr.size()));
+    VRS_HIP(ctx, hipMalloc(&b_keys.p, sizeof(uint64_t) * n));
+    VRS_HIP(ctx, hipMalloc(&b_tmp.p, sizeof(uint64_t) * n));
+    VRS_HIP(ctx, hipMalloc(&b_isc.p, sizeof(int) * 2 * n));
+    VRS_HIP(ctx, hipMalloc(&b_small.p, sizeof(int) * (nb + 4)));
+    VRS_HIP(ctx, hipMalloc(&b_ukeys.p, sizeof(uint64_t) * n));
+    VRS_HIP(ctx, hipMalloc(&b_cnt.p, sizeof(int32_t) * n));
+    VRS_HIP(ctx, hipMalloc(&b_bad.p, sizeof(int) * 2));
+    VRS_HIP(ctx, hipMemcpyAsync(b_pairs.p, pr.data(), sizeof(int32_t) * pr.size(),
+                                hipMemcpyHostToDevice, ctx->stream));
+    VRS_HIP(ctx, hipMemsetAsync(b_bad.p, 0, sizeof(int) * 2, ctx->stream));
+    int *n_u_dev = (int *)b_bad.p + 1;
+    {
+        Timer tm(ctx, "grid");
+        VRS_HIP(ctx, grid_build_keys(g->d_pos, g->n_t, (const int2 *)b_pairs.p, n_pairs, sc,
+                                     (uint64_t *)b_keys.p, (int *)b_bad.p, ctx->stream));
+        VRS_HIP(ctx, grid_sort_unique((uint64_t *)b_keys.p, (uint64_t *)b_tmp.p, n, (int *)b_isc.p,
+                                      (int *)b_small.p, (uint64_t *)b_ukeys.p, (int32_t *)b_cnt.p,
+                                      n_u_dev, ctx->stream));
+    }
+    int hb[2];
+    VRS_HIP(ctx, hipMemcpyAsync(hb, b_bad.p, sizeof hb, hipMemcpyDeviceToHost, ctx->stream));
+    VRS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (hb[0]) return fail(ctx, VRS_E_ARG, "gridded |uvw| exceeds 2^20 wavelengths");
+    const int nu = hb[1];
+    *n_unique = nu;
+    VRS_HIP(ctx, grid_counts((const int *)b_isc.p + n, nu, n, (int32_t *)b_cnt.p, ctx->stream));
+    if (uvw_unique || counts) {
+        if (nu > cap) return fail(ctx, VRS_E_ARG, "%d unique cells exceed cap %lld", nu, (long long)cap);
+        if (uvw_unique) {
+            Staged su(ctx, uvw_unique, sizeof(float) * 3 * nu, true);
+            if (vrs_status s = su.prepare()) return s;
+            VRS_HIP(ctx, grid_decode((const uint64_t *)b_ukeys.p, nu, (float *)su.dev, ctx->stream));
+            if (vrs_status s = su.finish()) return s;
+        }
+        if (counts) {
+            Staged sc2(ctx, counts, sizeof(int32_t) * nu, true);
+            if (vrs_status s = sc2.prepare()) return s;
+            VRS_HIP(ctx, hipMemcpyAsync(sc2.dev, b_cnt.p, sizeof(int32_t) * nu,
+                                        hipMemcpyDeviceToDevice, ctx->stream));
+            if (vrs_status s = sc2.finish()) return s;
+        }
+    }
+    if (weight || vis_count) {
+        VRS_HIP(ctx, hipMalloc(&b_vkeys.p, sizeof(uint64_t) * std::max<int64_t>(g->n_vis, 1)));
+        VRS_HIP(ctx, grid_build_keys(g->d_pos, g->n_t, g->d_pairs, g->n_base, sc,
+                                     (uint64_t *)b_vkeys.p, (int *)b_bad.p, ctx->stream));
+        Staged sw(ctx, weight, sizeof(float) * g->n_vis, true);
+        Staged svc(ctx, vis_count, sizeof(int32_t) * g->n_vis, true);
+        if (vrs_status s = sw.prepare()) return s;
+        if (vrs_status s = svc.prepare()) return s;
+        VRS_HIP(ctx, grid_vis_weights((const uint64_t *)b_vkeys.p, g->n_vis,
+                                      (const uint64_t *)b_ukeys.p, (const int32_t *)b_cnt.p, nu,
+                                      (float *)sw.dev, (int32_t *)svc.dev, ctx->stream));
+        if (vrs_status s = sw.finish()) return s;
+        if (vrs_status s = svc.finish()) return s;
+    }
+    VRS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return VRS_OK;
+}

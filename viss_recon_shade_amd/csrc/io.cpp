@@ -1,0 +1,98 @@
+// Text files of the reference pipeline, written and read with the same
+// layout and the default std::ostream formatting (6 significant digits) the
+// reference uses, so outputs can be exchanged with it file for file:
+//   uvw / updated_uvw  "u v w[ count]" per line   MIncline_generate_uvw_float.cu:420-427,
+//                                                 read at shade10M.cu:155-171 and
+//                                                 vissGen10M_shade.cu:351-366
+//   shadeM             "tS tE pS pE" per line     shade10M.cu:108-127, read at
+//                                                 vissGen10M_shade.cu:376-411
+//   F (image)          one real value per line    vissGen10M_shade.cu:563-583
+// The readers never write past the caller's capacity (the reference's fixed
+// uvw_presize buffers have no bound check, vissGen10M_shade.cu:56,359).
+#include <cstdint>
+#include <fstream>
+#include <string>
+
+#include "io.hpp"
+
+namespace vrs {
+
+int write_uvw_txt(const char *path, const double *uvw, int64_t n, const float *col4) {
+    std::ofstream f(path);
+    if (!f.is_open()) return -1;
+    for (int64_t i = 0; i < n; ++i) {
+        f << (float)uvw[3 * i] << " " << (float)uvw[3 * i + 1] << " " << (float)uvw[3 * i + 2];
+        if (col4) f << " " << col4[i];
+        f << std::endl;
+    }
+    return f.good() ? 0 : -1;
+}
+
+int read_uvw_txt(const char *path, int n_cols, double *uvw, float *col4, int64_t cap,
+                 int64_t *n_read) {
+    std::ifstream f(path);
+    if (!f.is_open()) return -1;
+    int64_t n = 0;
+    float u, v, w, c = 0.0f;
+    while (f >> u >> v >> w) {
+        if (n_cols == 4 && !(f >> c)) return -3;  // truncated line
+        if (n >= cap) return -2;
+        uvw[3 * n] = u;
+        uvw[3 * n + 1] = v;
+        uvw[3 * n + 2] = w;
+        if (col4) col4[n] = c;
+        ++n;
+    }
+    *n_read = n;
+    return 0;
+}
+
+int write_shade_txt(const char *path, const int32_t *box4, int64_t n) {
+    std::ofstream f(path);
+    if (!f.is_open()) return -1;
+    for (int64_t i = 0; i < n; ++i)
+        f << box4[4 * i] << " " << box4[4 * i + 1] << " " << box4[4 * i + 2] << " "
+          << box4[4 * i + 3] << std::endl;
+    return f.good() ? 0 : -1;
+}
+
+// As the reader of vissGen10M_shade.cu:385-399: each pair is ordered S <= E and
+// made 0-based (the files are 1-based, written for MATLAB).
+int read_shade_txt(const char *path, int32_t *box4, int64_t cap, int64_t *n_read) {
+    std::ifstream f(path);
+    if (!f.is_open()) return -1;
+    int64_t n = 0;
+    int a, b, c, d;
+    while (f >> a >> b >> c >> d) {
+        if (n >= cap) return -2;
+        box4[4 * n] = (a > b ? b : a) - 1;
+        box4[4 * n + 1] = (a > b ? a : b) - 1;
+        box4[4 * n + 2] = (c > d ? d : c) - 1;
+        box4[4 * n + 3] = (c > d ? c : d) - 1;
+        ++n;
+    }
+    *n_read = n;
+    return 0;
+}
+
+int write_image_txt(const char *path, const float *F, int64_t n) {
+    std::ofstream f(path);
+    if (!f.is_open()) return -1;
+    for (int64_t i = 0; i < n; ++i) f << F[i] << std::endl;
+    return f.good() ? 0 : -1;
+}
+
+int read_image_txt(const char *path, float *F, int64_t cap, int64_t *n_read) {
+    std::ifstream f(path);
+    if (!f.is_open()) return -1;
+    int64_t n = 0;
+    float x;
+    while (f >> x) {
+        if (n >= cap) return -2;
+        F[n++] = x;
+    }
+    *n_read = n;
+    return 0;
+}
+
+}  // namespace vrs

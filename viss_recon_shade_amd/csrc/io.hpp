@@ -1,0 +1,13 @@
+// Reference text formats (io.cpp).  Return 0, -1 (open/IO), -2 (capacity), -3 (truncated).
+#pragma once
+#include <cstdint>
+
+namespace vrs {
+int write_uvw_txt(const char *path, const double *uvw, int64_t n, const float *col4);
+int read_uvw_txt(const char *path, int n_cols, double *uvw, float *col4, int64_t cap,
+                 int64_t *n_read);
+int write_shade_txt(const char *path, const int32_t *box4, int64_t n);
+int read_shade_txt(const char *path, int32_t *box4, int64_t cap, int64_t *n_read);
+int write_image_txt(const char *path, const float *F, int64_t n);
+int read_image_txt(const char *path, float *F, int64_t cap, int64_t *n_read);
+}  // namespace vrs

@@ -1,0 +1,88 @@
+"""The reference pipeline's file layout: names and text formats of the
+inter-stage files, so that this build's outputs can replace the reference's
+file for file.
+
+Names (directory per frequency, one file per period p, 1-based):
+  frequency_{F}M/uvw{p}frequency{F}M.txt            MIncline_generate_uvw_float.cu:421
+  frequency_{F}M/updated_uvw{p}frequency{F}M.txt    read at vissGen10M_shade.cu:351
+  frequency_{F}M/shadeM{p}frequency{F}M.txt         shade10M.cu:269, vissGen10M_shade.cu:377
+  F_recon_{F}M/F{p}period{F}M.txt                   vissGen10M_shade.cu:573
+The readers and writers are the C-ABI's (vrs_read_* / vrs_write_*).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import _lib
+
+
+def _mhz(freq_hz):
+    f = freq_hz / 1e6
+    return str(int(round(f))) if abs(f - round(f)) < 1e-9 else ("%g" % f)
+
+
+def paths(root, freq_hz, period):
+    f = _mhz(freq_hz)
+    d = os.path.join(root, f"frequency_{f}M")
+    return {
+        "uvw": os.path.join(d, f"uvw{period}frequency{f}M.txt"),
+        "updated_uvw": os.path.join(d, f"updated_uvw{period}frequency{f}M.txt"),
+        "shadeM": os.path.join(d, f"shadeM{period}frequency{f}M.txt"),
+        "F": os.path.join(root, f"F_recon_{f}M", f"F{period}period{f}M.txt"),
+    }
+
+
+def _mk(path):
+    os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+    return path.encode()
+
+
+def write_uvw(path, uvw, counts=None):
+    uvw = np.ascontiguousarray(uvw, np.float64).reshape(-1, 3)
+    c = None if counts is None else np.ascontiguousarray(counts, np.float32)
+    _lib.check(_lib.load().vrs_write_uvw_txt(_mk(path), uvw.ctypes.data, len(uvw),
+                                             None if c is None else c.ctypes.data))
+
+
+def read_uvw(path, n_cols=3, cap=1 << 26):
+    uvw = np.empty((cap, 3), np.float64) if cap <= (1 << 20) else None
+    if uvw is None:  # size from the line count first
+        with open(path) as f:
+            cap = sum(1 for _ in f)
+        uvw = np.empty((max(cap, 1), 3), np.float64)
+    col4 = np.empty(len(uvw), np.float32)
+    n = C.c_int64()
+    _lib.check(_lib.load().vrs_read_uvw_txt(path.encode(), n_cols, uvw.ctypes.data,
+                                            col4.ctypes.data, len(uvw), C.byref(n)))
+    return (uvw[:n.value], col4[:n.value]) if n_cols == 4 else uvw[:n.value]
+
+
+def write_shade(path, box4):
+    box4 = np.ascontiguousarray(box4, np.int32).reshape(-1, 4)
+    _lib.check(_lib.load().vrs_write_shade_txt(_mk(path), box4.ctypes.data, len(box4)))
+
+
+def read_shade(path, cap=None):
+    if cap is None:
+        with open(path) as f:
+            cap = max(sum(1 for _ in f), 1)
+    out = np.empty((cap, 4), np.int32)
+    n = C.c_int64()
+    _lib.check(_lib.load().vrs_read_shade_txt(path.encode(), out.ctypes.data, cap, C.byref(n)))
+    return out[:n.value]
+
+
+def write_image(path, F):
+    F = np.ascontiguousarray(F, np.float32).reshape(-1)
+    _lib.check(_lib.load().vrs_write_image_txt(_mk(path), F.ctypes.data, len(F)))
+
+
+def read_image(path, cap=None):
+    if cap is None:
+        with open(path) as f:
+            cap = max(sum(1 for _ in f), 1)
+    out = np.empty(cap, np.float32)
+    n = C.c_int64()
+    _lib.check(_lib.load().vrs_read_image_txt(path.encode(), out.ctypes.data, cap, C.byref(n)))
+    return out[:n.value]
