@@ -226,8 +226,15 @@ def main():
     Vm, Fm = state["V"].cpu().numpy(), state["F"].cpu().numpy()
 
     value = terms / (ms_plain * 1e-3) / 1e9
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "traffic_c2.json")
+    if wl.name == "c2" and os.path.exists(tfile):
+        kern = json.load(open(tfile))["kernels"]
+        key = {"visgen": f"visgen_kernel<false, {tile_plain[0]}>", "recon": "recon_kernel<false>"}
     dom = max(("visgen", "recon"), key=lambda n: kt_plain[n])
     dom_ms = kt_plain[dom]
+    if wl.name == "c2" and os.path.exists(tfile) and key[dom] in kern:
+        traffic = kern[key[dom]]["traffic_bytes"]
     achieved = FLOPS_PER_TERM * wl.terms_per_direction / (dom_ms * 1e-3) / 1e12
     out = {
         "metric": "Gterms/s (vis x pix) vis-gen + recon",
@@ -247,7 +254,10 @@ def main():
                    "step": "orbit+uvw+caps, vis-gen, recon"},
         "roofline": {"bound": "valu", "kernel": dom, "achieved": round(achieved, 2),
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                     "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": traffic,
+                     "traffic_source": "profiles/traffic_c2.json (rocprofv3 FETCH_SIZE/WRITE_SIZE, "
+                                       "copy-calibrated)" if traffic else None,
+                     "algorithmic_bytes": 33_000_000 if dom == "visgen" else 30_000_000,
                      "flops_per_term": FLOPS_PER_TERM,
                      "trans_frac": round(2 * wl.terms_per_direction / (dom_ms * 1e-3) / TRANS_PEAK, 4),
                      "kernel_ms": round(dom_ms, 3)},

@@ -11,16 +11,22 @@ from viss_recon_shade_amd import workload as W
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("name", ["tiny", "c1", "c2"])
+@pytest.mark.parametrize("name", ["tiny", "c1", "c2", "coarse"])
 def test_grid_uvw_bit_exact(ctx, oracle, name):
-    wl = W.CONFIGS[name]
+    if name == "coarse":
+        # 20 kHz: lambda = 15 km, so cells hold many samples and the counts matter
+        wl = W.Workload("coarse", 8, 4000, 2e4, 8, [(1, 1.0, 0, 1.0, 1e3)])
+    else:
+        wl = W.CONFIGS[name]
     g = ctx.workload_geometry(wl)
     pos, _ = g.MIncline()
     out = g.grid_uvw(ordered=True, lambda_int=True)
     ordered = [(m, n) for m in range(wl.n_sat) for n in range(wl.n_sat) if m != n]
     lam = float(int(3e8 / wl.freq_hz))
     u, c, vc = oracle.grid_uvw(pos, ordered, lam, vis_pairs=g.pairs)
-    print(f"[{name}] {len(c)} unique cells from {c.sum()} samples", flush=True)
+    print(f"[{name}] {len(c)} unique cells from {c.sum()} samples, max count {c.max()}", flush=True)
+    if name == "coarse":
+        assert c.max() > 10 and len(c) < c.sum() // 4
     assert np.array_equal(out["uvw"], u)
     assert np.array_equal(out["counts"], c)
     assert np.array_equal(out["vis_count"], vc)
