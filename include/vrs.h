@@ -237,6 +237,27 @@ vrs_status vrs_grid_uvw(vrs_ctx *ctx, const vrs_geom *g, int32_t flags, int64_t 
                         float *uvw_unique, int32_t *counts, int64_t *n_unique, float *weight,
                         int32_t *vis_count);
 
+/* Fine-cell sky with sub-pixel shading (the reference's visscal over FF / NX /
+ * countLoc / NXq, vissGen10M_shade.cu:121-138; index built on the GPU where the
+ * reference scans all cells per pixel, vissGen1M_shade_optim2.cu:59-119).
+ * Attaches n_cells cells to the geometry: unit direction lmn [n][3] (FP64),
+ * intensity ff [n], parent RING pixel nx [n] (0- or 1-based per nx_base; the
+ * reference's NX is 1-based).  Every sky pixel needs at least one cell (the
+ * reference divides by its cell count).  Replaces an attached sky; freed with
+ * the geometry.  Host or device pointers. */
+vrs_status vrs_geom_fine_sky(vrs_ctx *ctx, vrs_geom *g, int64_t n_cells, const double *lmn,
+                             const float *ff, const int32_t *nx, int32_t nx_base);
+/* The inverse index: count_loc [n_pix] cells per pixel, nxq [n_cells] cell
+ * indices grouped by pixel (ascending pixel, ascending cell), pixel_mean
+ * [n_pix] = mean FF of each pixel (all nullable). */
+vrs_status vrs_fine_index(vrs_ctx *ctx, const vrs_geom *g, int32_t *count_loc, int32_t *nxq,
+                          float *pixel_mean);
+/* V [n_vis] complex: pixel q contributes (1/count_q) sum over its cells c of
+ * (c shaded for this visibility ? t_occ : FF_c) at the pixel's direction;
+ * shading is the exact geometric test of each cell's own direction.
+ * Precision VRS_PLAIN_F32 or VRS_MIXED_F64PHASE. */
+vrs_status vrs_visgen_fine(vrs_ctx *ctx, const vrs_geom *g, const vrs_knobs *k, float *V);
+
 /* Text files of the reference pipeline, same layout and default ostream
  * formatting (host memory only; no GPU needed):
  *   uvw[/updated_uvw]  "u v w[ count]"  MIncline_generate_uvw_float.cu:420-427, read at

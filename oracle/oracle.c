@@ -27,6 +27,10 @@
  *                                      vissGen10M_shade.cu:149-175): F = kappa sum_v w_v (1 - M)
  *                                      Re(V exp(+2 pi i s psi)), psi uses n (wfac=1) or n-1 (wfac=0)
  *   vo_shade_box ..................... ShadeMatrix + Cart2SphKernel (shade10M.cu:62-105,218-224)
+ *   vo_fine_index .................... computeLocCount / computeLocViss
+ *                                      (vissGen1M_shade_optim2.cu:59-119) as one counting pass
+ *   vo_visgen_fine ................... visscal's fine-cell sum with 240 replacement
+ *                                      (vissGen10M_shade.cu:121-138), cell-level geometric mask
  *
  * Every floating-point decision that must be bit-exact against the GPU (the
  * occultation predicate and its near-limb band) is written with explicit fma()
@@ -363,6 +367,71 @@ void vo_visgen(int64_t n_t, const double *pos, const int *pairs, const double *u
 }
 
 /* -------------------------------------------------------------- recon */
+/* Inverse index of a fine-cell sky: count[q] = number of cells whose parent
+ * is q (computeLocCount, vissGen1M_shade_optim2.cu:59-82), start[q] = its
+ * exclusive prefix sum and idx = cell indices grouped by parent in ascending
+ * cell order (computeLocViss, :86-119), here by one counting pass. */
+void vo_fine_index(int64_t n_cells, const int32_t *nx, int32_t nx_base, int64_t n_pix,
+                   int32_t *count, int64_t *start, int32_t *idx)
+{
+    for (int64_t q = 0; q < n_pix; ++q) count[q] = 0;
+    for (int64_t c = 0; c < n_cells; ++c) ++count[nx[c] - nx_base];
+    int64_t acc = 0;
+    for (int64_t q = 0; q < n_pix; ++q) { start[q] = acc; acc += count[q]; }
+    int64_t *fill = (int64_t *)malloc(sizeof(int64_t) * (size_t)(n_pix > 0 ? n_pix : 1));
+    for (int64_t q = 0; q < n_pix; ++q) fill[q] = start[q];
+    for (int64_t c = 0; c < n_cells; ++c) idx[fill[nx[c] - nx_base]++] = (int32_t)c;
+    free(fill);
+}
+
+/* Vis-gen over a fine-cell sky (visscal, vissGen10M_shade.cu:108-145): pixel q
+ * contributes C_q(v) = (1/count_q) sum over its cells c of (c shaded ? t_occ :
+ * FF_c) at the pixel's direction lmn[q]; a cell is shaded when its own
+ * direction is occulted for either satellite (the geometric mask replaces the
+ * reference's index box, :129-134).  Same phase conventions as vo_visgen. */
+void vo_visgen_fine(int64_t n_t, const double *pos, const int *pairs, const double *uvw,
+                    int64_t n_pix, const double *lmn, int64_t n_cells, const double *cell_lmn,
+                    const float *ff, const int32_t *count, const int64_t *start,
+                    const int32_t *idx, double R, double s, int wfac, double t_occ,
+                    int64_t n_sel, const int64_t *vis_idx, double *V)
+{
+    (void)n_cells;
+    vo_limb L = vo_make_limb(R);
+#pragma omp parallel for schedule(dynamic, 2)
+    for (int64_t k = 0; k < n_sel; ++k) {
+        int64_t v = vis_idx ? vis_idx[k] : k;
+        int64_t b = v / n_t, t = v % n_t;
+        const double *ra = pos + ((int64_t)pairs[2 * b] * n_t + t) * 3;
+        const double *rb = pos + ((int64_t)pairs[2 * b + 1] * n_t + t) * 3;
+        double ra2 = vo_r2(ra), rb2 = vo_r2(rb);
+        double u = uvw[3 * v] * s, vv = uvw[3 * v + 1] * s, w = uvw[3 * v + 2] * s;
+        double wre = 1.0, wim = 0.0;
+        if (wfac) {
+            double r = w - rint(w);
+            wre = cos(2.0 * VO_PI * r);
+            wim = -sin(2.0 * VO_PI * r);
+        }
+        double are = 0.0, aim = 0.0;
+        for (int64_t p = 0; p < n_pix; ++p) {
+            double sum = 0.0;
+            for (int32_t j = 0; j < count[p]; ++j) {
+                int32_t c = idx[start[p] + j];
+                int na = 0, nb = 0;
+                int m = vo_occ(ra, ra2, cell_lmn + 3 * (int64_t)c, &L, &na) |
+                        vo_occ(rb, rb2, cell_lmn + 3 * (int64_t)c, &L, &nb);
+                sum += m ? t_occ : (double)ff[c];
+            }
+            double ip = count[p] > 0 ? sum / count[p] : 0.0;
+            double phi = u * lmn[3 * p] + vv * lmn[3 * p + 1] + w * (lmn[3 * p + 2] - 1.0);
+            double r = phi - rint(phi);
+            are += ip * cos(2.0 * VO_PI * r);
+            aim -= ip * sin(2.0 * VO_PI * r);
+        }
+        V[2 * k] = are * wre - aim * wim;
+        V[2 * k + 1] = are * wim + aim * wre;
+    }
+}
+
 /* F[map][sel] for the selected pixels (pix_idx == NULL: all):
  *   F_p = kappa sum_v w_v (1 - M_vp) Re(V_v exp(+2 pi i s psi_vp))
  *   psi = u l + v m + w n   (wfac = 1, vissGen10M_shade_optim1.cu:260)

@@ -67,6 +67,10 @@ def _load():
                                _P(dtype=np.uint64, flags="C_CONTIGUOUS")]
     L.vo_grid_unique.restype = C.c_int64
     L.vo_grid_unique.argtypes = [C.c_int64, _P(dtype=np.uint64, flags="C_CONTIGUOUS"), _f32, _i32]
+    L.vo_fine_index.argtypes = [C.c_int64, _i32, C.c_int32, C.c_int64, _i32, _i64, _i32]
+    L.vo_visgen_fine.argtypes = [C.c_int64, _f64, _i32, _f64, C.c_int64, _f64, C.c_int64, _f64,
+                                 _f32, _i32, _i64, _i32, C.c_double, C.c_double, C.c_int,
+                                 C.c_double, C.c_int64, C.c_void_p, _f64]
     L.vo_shade_box.argtypes = [C.c_int64, _f32, _f32, _f32, C.c_int, C.c_float, _i32]
     B = C.CDLL(sb)
     B.vb_visgen_f32.argtypes = [C.c_int64, _f32, _i32, _f32, C.c_int64, _f32, _f32, C.c_float,
@@ -205,6 +209,38 @@ def visgen(pos, pairs, uvw_, lmn, I, R, s=1.0, wfac=0, t_occ=240.0, vis_sel=None
                    np.ascontiguousarray(lmn, np.float64), I.shape[0], I, R, s, wfac, t_occ,
                    n_sel, selp, out)
     return out[..., 0] + 1j * out[..., 1]
+
+
+def fine_index(nx, n_pix, nx_base=1):
+    """Inverse index (countLoc, start, NXq) of a fine-cell sky
+    (vissGen1M_shade_optim2.cu:59-119)."""
+    _load()
+    nx = np.ascontiguousarray(nx, np.int32)
+    count = np.empty(n_pix, np.int32)
+    start = np.empty(n_pix, np.int64)
+    idx = np.empty(len(nx), np.int32)
+    _lib.vo_fine_index(len(nx), nx, nx_base, n_pix, count, start, idx)
+    return count, start, idx
+
+
+def visgen_fine(pos, pairs, uvw_, lmn, cell_lmn, ff, nx, R, s=1.0, wfac=0, t_occ=240.0,
+                nx_base=1, vis_sel=None):
+    """visscal over a fine-cell sky with sub-pixel shading, FP64
+    (vissGen10M_shade.cu:108-145); returns complex128 [n_sel]."""
+    _load()
+    pos = np.ascontiguousarray(pos, np.float64)
+    n_t = pos.shape[1]
+    count, start, idx = fine_index(nx, len(lmn), nx_base)
+    sel, selp = _idx(vis_sel)
+    n_sel = len(sel) if sel is not None else len(pairs) * n_t
+    out = np.empty((n_sel, 2), np.float64)
+    _lib.vo_visgen_fine(n_t, pos, np.ascontiguousarray(pairs, np.int32),
+                        np.ascontiguousarray(uvw_, np.float64).reshape(-1), len(lmn),
+                        np.ascontiguousarray(lmn, np.float64), len(cell_lmn),
+                        np.ascontiguousarray(cell_lmn, np.float64),
+                        np.ascontiguousarray(ff, np.float32), count, start, idx, R, s, wfac,
+                        t_occ, n_sel, selp, out)
+    return out[:, 0] + 1j * out[:, 1]
 
 
 def recon(pos, pairs, uvw_, lmn, V, R, s=1.0, wfac=0, kappa=None, weights=None, pix_sel=None):

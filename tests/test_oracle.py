@@ -325,3 +325,67 @@ def test_grid_uvw_matches_numpy_unique(oracle):
     # per-visibility counts are the counts of the visibility's own cell
     row = {tuple(r): k for r, k in zip(uu, cnt)}
     assert list(vc[:40]) == [row[tuple(r)] for r in g01]
+
+
+# ------------------------------------------------------- fine-cell sky
+def test_fine_index_matches_stable_argsort(oracle):
+    # countLoc / NXq (vissGen1M_shade_optim2.cu:59-119) against numpy's stable sort
+    rng = np.random.default_rng(5)
+    n_pix = 37
+    nx = rng.integers(1, n_pix + 1, 1000).astype(np.int32)     # 1-based like NX
+    cnt, start, idx = oracle.fine_index(nx, n_pix, nx_base=1)
+    assert np.array_equal(cnt, np.bincount(nx - 1, minlength=n_pix))
+    assert np.array_equal(start, np.concatenate([[0], np.cumsum(cnt)[:-1]]))
+    assert np.array_equal(idx, np.argsort(nx, kind="stable"))
+
+
+def test_visgen_fine_one_cell_per_pixel_is_visgen(oracle):
+    # cells at the pixel centres, one each, listed in shuffled order: the
+    # fine-cell sum reduces to the pixel-level vis-gen exactly
+    pos, pairs, uvw = _tiny_geom(oracle)
+    lmn = oracle.healpix_ring(4)
+    I = np.random.default_rng(3).random(len(lmn)).astype(np.float32)
+    perm = np.random.default_rng(4).permutation(len(lmn))
+    V = oracle.visgen(pos, pairs, uvw, lmn, I, R, s=0.5, wfac=1)
+    Vf = oracle.visgen_fine(pos, pairs, uvw, lmn, lmn[perm], I[perm], (perm + 1).astype(np.int32),
+                            R, s=0.5, wfac=1)
+    assert np.abs(Vf - V[0]).max() < 1e-12 * np.abs(V).max()
+
+
+def test_visgen_fine_brute_force(oracle):
+    from viss_recon_shade_amd import workload as W
+    pos, pairs, uvw = _tiny_geom(oracle, n_t=4)
+    lmn = oracle.healpix_ring(2)
+    cl, ff, nx = W.fine_sky(2, factor=2, seed=11)
+    Vf = oracle.visgen_fine(pos, pairs, uvw, lmn, cl, ff, nx, R, s=0.5, t_occ=240.0)
+    flat = uvw.reshape(-1, 3)
+    n_shaded_total = 0
+    for v in range(len(flat)):
+        b, t = divmod(v, pos.shape[1])
+        acc = 0j
+        for p in range(len(lmn)):
+            cells = np.nonzero(nx == p + 1)[0]
+            vals = []
+            for c in cells:
+                sh = any(oracle.occ_one(pos[k, t], cl[c], R)[0] for k in pairs[b])
+                n_shaded_total += sh
+                vals.append(240.0 if sh else float(ff[c]))
+            ph = 0.5 * (flat[v] @ (lmn[p] - [0, 0, 1]))
+            acc += np.mean(vals) * np.exp(-2j * math.pi * ph)
+        assert abs(acc - Vf[v]) < 1e-9 * abs(acc) + 1e-9
+    assert n_shaded_total > 0
+
+
+def test_fine_sky_children(oracle):
+    # every pixel gets factor^2 cells, each nearer its own parent centre than
+    # the pixel size allows for any other assignment
+    from viss_recon_shade_amd import workload as W
+    nside, f = 4, 4
+    cl, ff, nx = W.fine_sky(nside, factor=f)
+    lmn = oracle.healpix_ring(nside)
+    assert np.array_equal(np.bincount(nx - 1, minlength=len(lmn)), np.full(len(lmn), f * f))
+    assert np.allclose(np.linalg.norm(cl, axis=1), 1.0, atol=1e-14)
+    cosang = np.einsum("ij,ij->i", cl, lmn[nx - 1])
+    # max centre-to-child angle of a HEALPix pixel is below ~1.4 x the mean pixel spacing
+    assert np.arccos(np.clip(cosang, -1, 1)).max() < 1.4 * math.sqrt(4 * math.pi / len(lmn))
+    assert ff.dtype == np.float32 and (ff > 0).all()

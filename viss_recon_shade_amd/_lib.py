@@ -100,6 +100,9 @@ SIGNATURES = {
     "vrs_batch_layout": (C.c_int, [_vp, _vp, _vp, _vp]),
     "vrs_batch_run": (C.c_int, [_ctx, _vp, C.POINTER(Knobs), _vp, _vp, _vp]),
     "vrs_shade_box": (C.c_int, [_ctx, _vp, C.c_int64, C.c_int32, C.c_float, _vp]),
+    "vrs_geom_fine_sky": (C.c_int, [_ctx, _vp, C.c_int64, _vp, _vp, _vp, C.c_int32]),
+    "vrs_fine_index": (C.c_int, [_ctx, _vp, _vp, _vp, _vp]),
+    "vrs_visgen_fine": (C.c_int, [_ctx, _vp, _vp, _vp]),
     "vrs_grid_uvw": (C.c_int, [_ctx, _vp, C.c_int32, C.c_int64, _vp, _vp, C.POINTER(C.c_int64),
                                _vp, _vp]),
     "vrs_write_uvw_txt": (C.c_int, [C.c_char_p, _vp, C.c_int64, _vp]),

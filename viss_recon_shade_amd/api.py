@@ -255,6 +255,48 @@ class Geometry:
                                                          C.byref(ms)))
         return t.value, ms.value
 
+    # ------------------------------------------------------ fine-cell sky
+    def fine_sky(self, lmn, FF, NX, one_based=True):
+        """Attach a fine-cell sky: cell directions lmn [n][3], intensities FF [n],
+        parent RING pixel NX [n] (1-based as the reference's NX file by default).
+        The inverse index (countLoc / NXq) is built on the GPU."""
+        lmn = np.ascontiguousarray(lmn, np.float64).reshape(-1, 3) if not _is_torch(lmn) else lmn
+        FF = np.ascontiguousarray(FF, np.float32).reshape(-1) if not _is_torch(FF) else FF
+        NX = np.ascontiguousarray(NX, np.int32).reshape(-1) if not _is_torch(NX) else NX
+        n = lmn.shape[0]
+        if FF.shape[0] != n or NX.shape[0] != n:
+            raise ValueError("lmn, FF and NX must have one row per cell")
+        self.ctx._sync_stream(lmn, FF, NX)
+        self.ctx.check(self.ctx._lib.vrs_geom_fine_sky(self.ctx.handle, self.handle, n, _ptr(lmn),
+                                                       _ptr(FF), _ptr(NX), 1 if one_based else 0))
+        self.n_cells = n
+
+    def fine_index(self):
+        """(countLoc [n_pix], NXq [n_cells], per-pixel mean FF [n_pix]) of the attached sky."""
+        n_cells = getattr(self, "n_cells", 0)
+        if not n_cells:
+            raise RuntimeError("no fine-cell sky attached")
+        cnt = np.empty(self.n_pix, np.int32)
+        nxq = np.empty(n_cells, np.int32)
+        mean = np.empty(self.n_pix, np.float32)
+        self.ctx.check(self.ctx._lib.vrs_fine_index(self.ctx.handle, self.handle, _ptr(cnt),
+                                                    _ptr(nxq), _ptr(mean)))
+        return cnt, nxq, mean
+
+    def visscal_fine(self, k=None, device=False):
+        """V[v] (complex64 as [..., 2] float32) of the attached fine-cell sky with
+        sub-pixel shading."""
+        k = k or knobs()
+        if device:
+            import torch
+            V = torch.empty((self.n_vis, 2), dtype=torch.float32, device=f"cuda:{self.ctx.device}")
+        else:
+            V = np.empty((self.n_vis, 2), np.float32)
+        self.ctx._sync_stream(V)
+        self.ctx.check(self.ctx._lib.vrs_visgen_fine(self.ctx.handle, self.handle, C.byref(k),
+                                                     _ptr(V)))
+        return V
+
     # ------------------------------------------------------------- stage 4
     def imagerecon(self, V, w=None, k=None):
         """F[map][pix] float32 from V[map][v] (complex64 as [..., 2] float32)."""

@@ -68,6 +68,22 @@ struct vrs_geom {
     double4 *d_pix64 = nullptr;
     int32_t *d_ring = nullptr;
     ChunkRec *d_chunk = nullptr;
+    // fine-cell sky (vrs_geom_fine_sky); all null until one is attached
+    int64_t n_cells = 0;
+    float4 *d_cell32 = nullptr;
+    double4 *d_cell64 = nullptr;
+    int32_t *d_fidx = nullptr, *d_fstart = nullptr, *d_fcount = nullptr;
+    float *d_cbar = nullptr;
+    float2 *d_pixcap = nullptr;
+    ChunkRec *d_fchunk = nullptr;
+    void free_fine() {
+        void *ptrs[] = {d_cell32, d_cell64, d_fidx, d_fstart, d_fcount, d_cbar, d_pixcap, d_fchunk};
+        for (void *p : ptrs)
+            if (p) (void)hipFree(p);
+        d_cell32 = nullptr; d_cell64 = nullptr; d_fidx = d_fstart = d_fcount = nullptr;
+        d_cbar = nullptr; d_pixcap = nullptr; d_fchunk = nullptr;
+        n_cells = 0;
+    }
     GeomDev dev() const {
         GeomDev g = {};
         g.n_sat = n_sat; g.n_t = n_t; g.n_base = n_base; g.n_chunk = n_chunk;
@@ -417,6 +433,7 @@ void vrs_geom_destroy(vrs_geom *g) {
                     g->d_pix32, g->d_pix64, g->d_ring, g->d_chunk};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
+    g->free_fine();
     delete g;
 }
 
@@ -1399,4 +1416,141 @@ extern "C" vrs_status vrs_grid_uvw(vrs_ctx *ctx, const vrs_geom *g, int32_t flag
     }
     VRS_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return VRS_OK;
+}
+
+// ------------------------------------------------------------ fine-cell sky
+// The reference's sub-pixel sky (vissGen10M_shade.cu:121-138, inverse index
+// vissGen1M_shade_optim2.cu:59-119): cells with their own direction and FF,
+// each belonging to one sky pixel.
+
+vrs_status vrs_geom_fine_sky(vrs_ctx *ctx, vrs_geom *g, int64_t n_cells, const double *lmn,
+                             const float *ff, const int32_t *nx, int32_t nx_base) {
+    if (vrs_status s = check_ctx(ctx)) return s;
+    if (vrs_status s = check_geom(ctx, g)) return s;
+    if (n_cells < 1 || n_cells >= ((int64_t)1 << 31))
+        return fail(ctx, VRS_E_ARG, "n_cells must be in [1, 2^31)");
+    if (!lmn || !ff || !nx) return fail(ctx, VRS_E_ARG, "lmn, ff and nx must be non-NULL");
+    if (nx_base != 0 && nx_base != 1) return fail(ctx, VRS_E_ARG, "nx_base must be 0 or 1");
+    DeviceGuard dg(ctx->device);
+    VRS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    g->free_fine();
+    Staged sl(ctx, lmn, sizeof(double) * 3 * n_cells, false);
+    Staged sf(ctx, ff, sizeof(float) * n_cells, false);
+    Staged sn(ctx, nx, sizeof(int32_t) * n_cells, false);
+    if (vrs_status s = sl.prepare()) return s;
+    if (vrs_status s = sf.prepare()) return s;
+    if (vrs_status s = sn.prepare()) return s;
+    struct Buf {
+        void *p = nullptr;
+        ~Buf() { if (p) (void)hipFree(p); }
+    } b_keys, b_tmp, b_small;
+    const int64_t n = n_cells, nb = (n + grid_tile() - 1) / grid_tile();
+    vrs_status st = VRS_OK;
+    auto alloc = [&](void **p, size_t bytes) {
+        if (st == VRS_OK && hipMalloc(p, std::max(bytes, (size_t)16)) != hipSuccess) {
+            (void)hipGetLastError();
+            st = fail(ctx, VRS_E_OOM, "fine-cell sky: device allocation of %zu bytes failed", bytes);
+        }
+    };
+    alloc((void **)&g->d_cell32, sizeof(float4) * n);
+    alloc((void **)&g->d_cell64, sizeof(double4) * n);
+    alloc((void **)&g->d_fidx, sizeof(int32_t) * n);
+    alloc((void **)&g->d_fstart, sizeof(int32_t) * g->n_pad);
+    alloc((void **)&g->d_fcount, sizeof(int32_t) * g->n_pad);
+    alloc((void **)&g->d_cbar, sizeof(float) * g->n_pad);
+    alloc((void **)&g->d_pixcap, sizeof(float2) * g->n_pad);
+    alloc((void **)&g->d_fchunk, sizeof(ChunkRec) * g->n_chunk);
+    alloc(&b_keys.p, sizeof(uint64_t) * n);
+    alloc(&b_tmp.p, sizeof(uint64_t) * n);
+    alloc(&b_small.p, sizeof(int) * (nb + 8));
+    if (st != VRS_OK) {
+        g->free_fine();
+        return st;
+    }
+    int *bad = (int *)b_small.p + nb + 4;
+    hipError_t he = hipMemsetAsync(bad, 0, sizeof(int), ctx->stream);
+    if (he == hipSuccess)
+        he = fine_build_index((const double *)sl.dev, (const float *)sf.dev, (const int32_t *)sn.dev,
+                              nx_base, n, g->n_pix, g->d_ring, g->n_pad, g->d_cell32, g->d_cell64,
+                              (uint64_t *)b_keys.p, (uint64_t *)b_tmp.p, (int *)b_small.p, bad,
+                              g->d_fidx, g->d_fstart, g->d_fcount, ctx->stream);
+    int hb = 0;
+    if (he == hipSuccess) he = hipMemcpyAsync(&hb, bad, sizeof hb, hipMemcpyDeviceToHost, ctx->stream);
+    if (he == hipSuccess) he = hipStreamSynchronize(ctx->stream);
+    if (he != hipSuccess) {
+        g->free_fine();
+        return fail(ctx, VRS_E_HIP, "fine-cell sky index: %s", hipGetErrorString(he));
+    }
+    if (hb) {
+        g->free_fine();
+        return fail(ctx, VRS_E_ARG, "fine-cell sky: %s",
+                    (hb & 1)   ? "a parent pixel index is out of range"
+                    : (hb & 2) ? "a sky pixel has no cells"
+                               : "a cell direction is not a unit vector");
+    }
+    g->n_cells = n;
+    GeomDev gd = g->dev();
+    gd.fine = {g->d_fstart, g->d_fcount, g->d_fidx, g->d_cell32, g->d_cell64, nullptr};
+    he = launch_fine_pixels(gd, g->d_cbar, g->d_pixcap, g->d_fchunk, ctx->stream);
+    if (he == hipSuccess) he = hipStreamSynchronize(ctx->stream);
+    if (he != hipSuccess) {
+        g->free_fine();
+        return fail(ctx, VRS_E_HIP, "fine-cell sky bounds: %s", hipGetErrorString(he));
+    }
+    return VRS_OK;
+}
+
+vrs_status vrs_fine_index(vrs_ctx *ctx, const vrs_geom *g, int32_t *count_loc, int32_t *nxq,
+                          float *pixel_mean) {
+    if (vrs_status s = check_ctx(ctx)) return s;
+    if (vrs_status s = check_geom(ctx, g)) return s;
+    if (g->n_cells == 0) return fail(ctx, VRS_E_STATE, "no fine-cell sky attached to the geometry");
+    DeviceGuard dg(ctx->device);
+    Staged sc(ctx, count_loc, sizeof(int32_t) * g->n_pix, true);
+    Staged sq(ctx, nxq, sizeof(int32_t) * g->n_cells, true);
+    Staged sm(ctx, pixel_mean, sizeof(float) * g->n_pix, true);
+    if (vrs_status s = sc.prepare()) return s;
+    if (vrs_status s = sq.prepare()) return s;
+    if (vrs_status s = sm.prepare()) return s;
+    if (count_loc)
+        VRS_HIP(ctx, fine_count_ring(g->d_fcount, g->d_ring, g->n_pad, (int32_t *)sc.dev, ctx->stream));
+    if (nxq)
+        VRS_HIP(ctx, hipMemcpyAsync(sq.dev, g->d_fidx, sizeof(int32_t) * g->n_cells,
+                                    hipMemcpyDeviceToDevice, ctx->stream));
+    if (pixel_mean) {
+        // chunk order -> RING order through the same gather the recon output uses
+        const GeomDev gd = g->dev();
+        VRS_HIP(ctx, launch_from_chunk_order(gd, g->n_pix, g->d_cbar, (float *)sm.dev, ctx->stream));
+    }
+    if (vrs_status s = sc.finish()) return s;
+    if (vrs_status s = sq.finish()) return s;
+    if (vrs_status s = sm.finish()) return s;
+    VRS_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return VRS_OK;
+}
+
+vrs_status vrs_visgen_fine(vrs_ctx *ctx, const vrs_geom *g, const vrs_knobs *k, float *V) {
+    if (vrs_status s = check_ctx(ctx)) return s;
+    if (vrs_status s = check_geom(ctx, g)) return s;
+    if (vrs_status s = check_knobs(ctx, k)) return s;
+    if (!V) return fail(ctx, VRS_E_ARG, "V must be non-NULL");
+    if (g->n_cells == 0) return fail(ctx, VRS_E_STATE, "no fine-cell sky attached to the geometry");
+    if (k->precision == VRS_MFMA_SPLIT16)
+        return fail(ctx, VRS_E_ARG, "the fine-cell sky is one map: use plain or mixed precision");
+    DeviceGuard dg(ctx->device);
+    Staged sV(ctx, V, sizeof(float) * 2 * g->n_vis, true);
+    if (vrs_status s = sV.prepare()) return s;
+    GeomDev gd = g->dev();
+    gd.chunk = g->d_fchunk;
+    gd.fine = {g->d_fstart, g->d_fcount, g->d_fidx, g->d_cell32, g->d_cell64, g->d_pixcap};
+    const bool f64 = k->precision == VRS_MIXED_F64PHASE;
+    const int vt = visgen_tile_vis();
+    const int64_t n_tiles = (g->n_vis + vt - 1) / vt;
+    const int grid = persistent_grid(ctx, visgen_fine_kernel_ptr(f64), n_tiles);
+    {
+        Timer tm(ctx, "visgen_fine");
+        VRS_HIP(ctx, launch_visgen_fine(gd, phase_knobs(k), f64, g->d_cbar, (float2 *)sV.dev, grid,
+                                        ctx->stream));
+    }
+    return sV.finish();
 }

@@ -244,7 +244,114 @@ __global__ void __launch_bounds__(kGB) vis_weight_kernel(const uint64_t *__restr
 
 unsigned blocks_of(int64_t n, int per) { return (unsigned)((n + per - 1) / per); }
 
+// ---- fine-cell sky: inverse index (cells grouped by parent pixel)
+__global__ void __launch_bounds__(kGB) fine_pack_kernel(const double *__restrict__ lmn,
+                                                        const float *__restrict__ ff,
+                                                        const int32_t *__restrict__ nx,
+                                                        int32_t nx_base, int64_t n, int64_t n_pix,
+                                                        float4 *__restrict__ c32,
+                                                        double4 *__restrict__ c64,
+                                                        uint64_t *__restrict__ keys,
+                                                        int *__restrict__ bad) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double x = lmn[3 * i], y = lmn[3 * i + 1], z = lmn[3 * i + 2];
+    c32[i] = make_float4((float)x, (float)y, (float)z, ff[i]);
+    c64[i] = make_double4(x, y, z, 0.0);
+    if (!(fabs(fma(z, z, fma(y, y, x * x)) - 1.0) <= 1e-5)) atomicOr(bad, 4);
+    const int64_t p = (int64_t)nx[i] - nx_base;
+    if (p < 0 || p >= n_pix) atomicOr(bad, 1);
+    keys[i] = ((uint64_t)(uint32_t)(p < 0 ? 0 : p) << 32) | (uint64_t)(uint32_t)i;
+}
+
+__global__ void __launch_bounds__(kGB) fine_idx_kernel(const uint64_t *__restrict__ keys, int64_t n,
+                                                       int32_t *__restrict__ idx) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) idx[i] = (int32_t)(keys[i] & 0xffffffffull);
+}
+
+// per chunk-order pixel: [start, start + count) of its cells in the sorted keys
+__global__ void __launch_bounds__(kGB) fine_ranges_kernel(const uint64_t *__restrict__ keys,
+                                                          int64_t n, const int32_t *__restrict__ ring,
+                                                          int64_t n_pad, int32_t *__restrict__ start,
+                                                          int32_t *__restrict__ count,
+                                                          int *__restrict__ bad) {
+    const int64_t pi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (pi >= n_pad) return;
+    const int r = ring[pi];
+    if (r < 0) {
+        start[pi] = 0;
+        count[pi] = 0;
+        return;
+    }
+    const uint64_t lo_key = (uint64_t)(uint32_t)r << 32, hi_key = (uint64_t)(uint32_t)(r + 1) << 32;
+    int64_t a = 0, b = n;
+    while (a < b) { const int64_t m = (a + b) >> 1; if (keys[m] < lo_key) a = m + 1; else b = m; }
+    int64_t c = a, d = n;
+    while (c < d) { const int64_t m = (c + d) >> 1; if (keys[m] < hi_key) c = m + 1; else d = m; }
+    start[pi] = (int32_t)a;
+    count[pi] = (int32_t)(c - a);
+    if (c == a) atomicOr(bad, 2);
+}
+
+// countLoc in RING order from the chunk-order counts
+__global__ void __launch_bounds__(kGB) fine_count_ring_kernel(const int32_t *__restrict__ count,
+                                                              const int32_t *__restrict__ ring,
+                                                              int64_t n_pad,
+                                                              int32_t *__restrict__ out) {
+    const int64_t pi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (pi >= n_pad) return;
+    const int r = ring[pi];
+    if (r >= 0) out[r] = count[pi];
+}
+
 }  // namespace
+
+hipError_t grid_sort_keys(uint64_t *keys, uint64_t *tmp, int64_t n, int *small, int bit_lo,
+                          int bit_hi, hipStream_t st) {
+    const int64_t nb = (n + kGTile - 1) / kGTile;
+    int *zoff = small, *total = small + nb;
+    uint64_t *a = keys, *b = tmp;
+    for (int bit = bit_lo; bit < bit_hi; ++bit) {
+        hipLaunchKernelGGL(split_count_kernel, dim3((unsigned)nb), dim3(kGB), 0, st, a, n, bit, zoff);
+        hipLaunchKernelGGL(scan_small_kernel, dim3(1), dim3(1024), 0, st, zoff, (int)nb, total);
+        hipLaunchKernelGGL(split_scatter_kernel, dim3((unsigned)nb), dim3(kGB), 0, st, a, n, bit,
+                           (const int *)zoff, (const int *)total, b);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        uint64_t *t = a;
+        a = b;
+        b = t;
+    }
+    if (a == keys) return hipSuccess;
+    return hipMemcpyAsync(keys, a, sizeof(uint64_t) * n, hipMemcpyDeviceToDevice, st);
+}
+
+hipError_t fine_build_index(const double *lmn, const float *ff, const int32_t *nx,
+                            int32_t nx_base, int64_t n, int64_t n_pix, const int32_t *ring,
+                            int64_t n_pad, float4 *c32, double4 *c64, uint64_t *keys,
+                            uint64_t *tmp, int *small, int *bad, int32_t *idx, int32_t *start,
+                            int32_t *count, hipStream_t st) {
+    hipLaunchKernelGGL(fine_pack_kernel, dim3(blocks_of(n, kGB)), dim3(kGB), 0, st, lmn, ff, nx,
+                       nx_base, n, n_pix, c32, c64, keys, bad);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    // the keys start in cell order, so a stable sort on the parent bits alone groups them
+    int bits = 1;
+    while (bits < 31 && ((int64_t)1 << bits) < n_pix) ++bits;
+    if ((e = grid_sort_keys(keys, tmp, n, small, 32, 32 + bits, st)) != hipSuccess) return e;
+    hipLaunchKernelGGL(fine_idx_kernel, dim3(blocks_of(n, kGB)), dim3(kGB), 0, st, keys, n, idx);
+    hipLaunchKernelGGL(fine_ranges_kernel, dim3(blocks_of(n_pad, kGB)), dim3(kGB), 0, st, keys, n,
+                       ring, n_pad, start, count, bad);
+    return hipGetLastError();
+}
+
+hipError_t fine_count_ring(const int32_t *count, const int32_t *ring, int64_t n_pad, int32_t *out,
+                           hipStream_t st) {
+    hipLaunchKernelGGL(fine_count_ring_kernel, dim3(blocks_of(n_pad, kGB)), dim3(kGB), 0, st, count,
+                       ring, n_pad, out);
+    return hipGetLastError();
+}
 
 size_t grid_scratch_bytes(int64_t n_keys) {
     const int64_t nb = (n_keys + kGTile - 1) / kGTile;
@@ -263,21 +370,7 @@ hipError_t grid_sort_unique(uint64_t *keys, uint64_t *tmp, int64_t n, int *iscra
                             uint64_t *ukeys, int32_t *counts, int *n_unique_dev, hipStream_t st) {
     const int64_t nb = (n + kGTile - 1) / kGTile;
     int *zoff = small;            // [nb]
-    int *total = small + nb;      // [1]
-    uint64_t *a = keys, *b = tmp;
-    hipError_t e;
-    for (int bit = 0; bit < 63; ++bit) {
-        hipLaunchKernelGGL(split_count_kernel, dim3((unsigned)nb), dim3(kGB), 0, st, a, n, bit, zoff);
-        hipLaunchKernelGGL(scan_small_kernel, dim3(1), dim3(1024), 0, st, zoff, (int)nb, total);
-        hipLaunchKernelGGL(split_scatter_kernel, dim3((unsigned)nb), dim3(kGB), 0, st, a, n, bit,
-                           (const int *)zoff, (const int *)total, b);
-        if ((e = hipGetLastError()) != hipSuccess) return e;
-        uint64_t *t = a;
-        a = b;
-        b = t;
-    }
-    // 63 passes: the sorted keys are in `tmp`; copy back so that `keys` holds them
-    e = hipMemcpyAsync(keys, a, sizeof(uint64_t) * n, hipMemcpyDeviceToDevice, st);
+    hipError_t e = grid_sort_keys(keys, tmp, n, small, 0, 63, st);
     if (e != hipSuccess) return e;
     int *flags = iscratch;              // [n]
     int *start = iscratch + n;          // [n]

@@ -87,6 +87,43 @@ __device__ __forceinline__ bool occ_pair64(const double4 *__restrict__ pos,
     return m;
 }
 
+// cos(0.5): bounding caps wider than 0.5 rad are not used for whole-pixel decisions
+constexpr float kFineWideCos = 0.87758256f;
+
+// Sub-pixel shading: the pixel's effective intensity for one visibility is the
+// mean over its fine cells of (cell shaded ? t_occ : FF) (vissGen10M_shade.cu:121-138).
+// The pixel's own bounding cap decides wholly shaded / wholly clear pixels;
+// only pixels on a limb test their cells (FP32 with the FP64 guard band).
+__device__ __noinline__ float fine_value(const GeomDev &g, int64_t pi, float px, float py, float pz,
+                            const CapRec &A, const CapRec &B, int ia, int ib, float t_occ,
+                            float cbar) {
+    const float2 pc = g.fine.pixcap[pi];
+    ChunkRec pr;
+    pr.gx = px; pr.gy = py; pr.gz = pz; pr.crho = pc.x; pr.srho = pc.y;
+    pr.flags = (pc.x < kFineWideCos) ? kChunkAlwaysStraddle : 0;  // wide pixel: cell by cell
+    pr.n_valid = 1; pr.hash_all = 0;
+    const int ca = classify_cap(pr, A), cb = classify_cap(pr, B);
+    if (ca == kIn || cb == kIn) return t_occ;
+    if (ca == kOut && cb == kOut) return cbar;
+    const int n = g.fine.count[pi];
+    const int s0 = g.fine.start[pi];
+    float acc = 0.0f;
+    for (int k = 0; k < n; ++k) {
+        const int c = g.fine.idx[s0 + k];
+        const float4 cc = g.fine.cell32[c];
+        bool unc = false;
+        bool sh = occ32(cc.x, cc.y, cc.z, A, unc) | occ32(cc.x, cc.y, cc.z, B, unc);
+        if (unc) {
+            const double4 s64 = g.fine.cell64[c];
+            int na = 0, nb = 0;
+            sh = occ64(g.pos[ia], s64.x, s64.y, s64.z, g.limb, na) |
+                 occ64(g.pos[ib], s64.x, s64.y, s64.z, g.limb, nb);
+        }
+        acc += sh ? t_occ : cc.w;
+    }
+    return n > 0 ? acc / (float)n : 0.0f;
+}
+
 __device__ __forceinline__ float fract_rev(float x) { return __builtin_amdgcn_fractf(x); }
 __device__ __forceinline__ float fract_rev(double x) { return (float)__builtin_amdgcn_fract(x); }
 
@@ -312,7 +349,7 @@ __device__ __forceinline__ void fill_vis_ent(const GeomDev &g, const PhaseKnobs 
     }
 }
 
-template <bool F64, int VT>
+template <bool F64, int VT, bool FINE = false>
 __global__ void __launch_bounds__(kBlock) visgen_kernel(GeomDev g, PhaseKnobs k, int32_t n_maps,
                                                         const float *__restrict__ Ich,
                                                         float2 *__restrict__ V) {
@@ -402,14 +439,19 @@ __global__ void __launch_bounds__(kBlock) visgen_kernel(GeomDev g, PhaseKnobs k,
 #pragma unroll
                 for (int j = 0; j < kPixPerLane; ++j) {
                     const float sx = (float)l[j], sy = (float)m[j], sz = (float)n[j];
-                    bool unc = false;
-                    bool sh = occ32(sx, sy, sz, A, unc) | occ32(sx, sy, sz, B, unc);
-                    if (unc) {
-                        int nl;
-                        sh = occ_pair64(g.pos, g.pix64, (int64_t)c * kChunk + j * kWave + lane,
-                                        ents[q].ia, ents[q].ib, g.limb, &nl);
+                    const int64_t pi = (int64_t)c * kChunk + j * kWave + lane;
+                    float ie;
+                    if (FINE) {
+                        ie = fine_value(g, pi, sx, sy, sz, A, B, ents[q].ia, ents[q].ib, t_occ, I[j]);
+                    } else {
+                        bool unc = false;
+                        bool sh = occ32(sx, sy, sz, A, unc) | occ32(sx, sy, sz, B, unc);
+                        if (unc) {
+                            int nl;
+                            sh = occ_pair64(g.pos, g.pix64, pi, ents[q].ia, ents[q].ib, g.limb, &nl);
+                        }
+                        ie = sh ? t_occ : I[j];
                     }
-                    const float ie = sh ? t_occ : I[j];
                     const T ph = fma(u, l[j], fma(vv, m[j], fma(w, n[j], wo)));
                     const float r = fract_rev(ph);
                     dre = fmaf(ie, __builtin_amdgcn_cosf(r), dre);
@@ -641,9 +683,79 @@ __global__ void __launch_bounds__(256) recon_finalize_kernel(GeomDev g, int32_t 
     F[map * n_pix + r] = s;
 }
 
+// Per chunk-order pixel: mean FF over its cells (same order as fine_value) and
+// the cos / sin of the largest centre-to-cell angle, padded like chunk bounds.
+__global__ void __launch_bounds__(256) fine_pixels_kernel(GeomDev g, float *__restrict__ cbar,
+                                                          float2 *__restrict__ pixcap) {
+    const int64_t pi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (pi >= g.n_pad) return;
+    const int n = g.fine.count[pi], s0 = g.fine.start[pi];
+    const double4 p = g.pix64[pi];
+    float acc = 0.0f;
+    double minc = 1.0;
+    for (int k = 0; k < n; ++k) {
+        const int c = g.fine.idx[s0 + k];
+        acc += g.fine.cell32[c].w;
+        const double4 q = g.fine.cell64[c];
+        minc = fmin(minc, p.x * q.x + p.y * q.y + p.z * q.z);
+    }
+    cbar[pi] = n > 0 ? acc / (float)n : 0.0f;
+    double rho = acos(fmax(-1.0, fmin(1.0, minc)));
+    rho = rho * (1.0 + 1e-6) + 1e-6;
+    pixcap[pi] = make_float2((float)cos(rho), (float)sin(rho));
+}
+
+// Chunk bounds widened to hold every cell of every pixel in the chunk:
+// rho_fine = (rho_chunk + max pixel-cell angle), padded like the chunk bounds.
+__global__ void __launch_bounds__(256) fine_chunks_kernel(GeomDev g, const float2 *__restrict__ pixcap,
+                                                          ChunkRec *__restrict__ out) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= g.n_chunk) return;
+    ChunkRec ch = g.chunk[c];
+    double mp = 0.0;
+    for (int j = 0; j < kChunk; ++j) {
+        const float2 pc = pixcap[(int64_t)c * kChunk + j];
+        mp = fmax(mp, atan2((double)pc.y, (double)pc.x));
+    }
+    double rho = atan2((double)ch.srho, (double)ch.crho) + mp;
+    rho = rho * (1.0 + 1e-6) + 1e-6;
+    if (rho > 0.5) ch.flags |= kChunkAlwaysStraddle;
+    rho = fmin(rho, M_PI);
+    ch.crho = (float)cos(rho);
+    ch.srho = (float)sin(rho);
+    out[c] = ch;
+}
+
+__global__ void __launch_bounds__(256) from_chunk_order_kernel(GeomDev g, const float *__restrict__ in,
+                                                               float *__restrict__ out) {
+    const int64_t pi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (pi >= g.n_pad) return;
+    const int r = g.ring[pi];
+    if (r >= 0) out[r] = in[pi];
+}
+
 inline unsigned blocks_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 }  // namespace
+
+hipError_t launch_from_chunk_order(const GeomDev &g, int64_t n_pix, const float *in, float *out,
+                                   hipStream_t st) {
+    (void)n_pix;
+    hipLaunchKernelGGL(from_chunk_order_kernel, dim3(blocks_for(g.n_pad, 256)), dim3(256), 0, st, g,
+                       in, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_fine_pixels(const GeomDev &g, float *cbar, float2 *pixcap, ChunkRec *fchunk,
+                              hipStream_t st) {
+    hipLaunchKernelGGL(fine_pixels_kernel, dim3(blocks_for(g.n_pad, 256)), dim3(256), 0, st, g,
+                       cbar, pixcap);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(fine_chunks_kernel, dim3(blocks_for(g.n_chunk, 256)), dim3(256), 0, st, g,
+                       (const float2 *)pixcap, fchunk);
+    return hipGetLastError();
+}
 
 hipError_t launch_orbit(const OrbitArgs &a, const GeomDev &g, double4 *pos, CapRec *cap,
                         double4 *uvw64, float4 *uvw32, hipStream_t st) {
@@ -711,6 +823,20 @@ hipError_t launch_recon_finalize(const GeomDev &g, int32_t n_maps, int32_t n_seg
     hipLaunchKernelGGL(recon_finalize_kernel, dim3(blocks_for((int64_t)n_maps * g.n_pad, 256)),
                        dim3(256), 0, st, g, n_maps, n_seg, n_pix, part, F);
     return hipGetLastError();
+}
+
+hipError_t launch_visgen_fine(const GeomDev &g, const PhaseKnobs &k, bool f64, const float *Ich,
+                              float2 *V, int grid, hipStream_t st) {
+    if (f64)
+        hipLaunchKernelGGL((visgen_kernel<true, 16, true>), dim3(grid), dim3(kBlock), 0, st, g, k,
+                           1, Ich, V);
+    else
+        hipLaunchKernelGGL((visgen_kernel<false, 16, true>), dim3(grid), dim3(kBlock), 0, st, g, k,
+                           1, Ich, V);
+    return hipGetLastError();
+}
+const void *visgen_fine_kernel_ptr(bool f64) {
+    return f64 ? (const void *)visgen_kernel<true, 16, true> : (const void *)visgen_kernel<false, 16, true>;
 }
 
 int visgen_tile_vis() { return 16; }

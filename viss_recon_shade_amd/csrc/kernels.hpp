@@ -9,6 +9,17 @@
 
 namespace vrs {
 
+// A fine-cell sky attached to a geometry (sub-pixel shading): cells grouped by
+// their parent pixel, pixel (chunk-order) -> [start, start + count) in idx.
+struct FineDev {
+    const int32_t *start;   // [n_pad]
+    const int32_t *count;   // [n_pad] (0 for padding and empty pixels)
+    const int32_t *idx;     // [n_cells] cell indices, grouped by pixel, ascending within a pixel
+    const float4 *cell32;   // [n_cells] (l, m, n, FF)
+    const double4 *cell64;  // [n_cells] (l, m, n, 0)
+    const float2 *pixcap;   // [n_pad] (cos, sin) of the angle that holds the pixel's cells
+};
+
 // Device view of one geometry; passed to kernels by value.
 struct GeomDev {
     int32_t n_sat, n_t, n_base, n_chunk;
@@ -31,6 +42,7 @@ struct GeomDev {
     const int32_t *map_slot0;  // [n_maps + 1] first slot of each map
     const float *kappa_map;    // [n_maps] recon normalisation of each map
     int64_t n_slots;
+    FineDev fine;              // all null unless a fine-cell vis-gen is running
 };
 
 struct OrbitArgs {
@@ -69,6 +81,22 @@ hipError_t launch_recon(const GeomDev &g, const PhaseKnobs &k, bool f64_phase, i
 // F[map][ring[p]] = sum_seg part[map][seg][p].
 hipError_t launch_recon_finalize(const GeomDev &g, int32_t n_maps, int32_t n_seg, int64_t n_pix,
                                  const float *part, float *F, hipStream_t st);
+
+// Fine-cell sky set-up: per chunk-order pixel, cells' mean FF (cbar) and the
+// angle holding its cells (pixcap); per chunk, bounds widened to hold every
+// cell (fchunk).  g.fine.idx/start/count/cell* come from the caller.
+hipError_t launch_fine_pixels(const GeomDev &g, float *cbar, float2 *pixcap, ChunkRec *fchunk,
+                              hipStream_t st);
+
+// One map, chunk order -> RING order (padding dropped).
+hipError_t launch_from_chunk_order(const GeomDev &g, int64_t n_pix, const float *in, float *out,
+                                   hipStream_t st);
+
+// Vis-gen with the sub-pixel (fine-cell) shade mask; Ich = per-pixel mean FF (chunk order);
+// g.chunk must be the widened fine chunk bounds.
+hipError_t launch_visgen_fine(const GeomDev &g, const PhaseKnobs &k, bool f64, const float *Ich,
+                              float2 *V, int grid, hipStream_t st);
+const void *visgen_fine_kernel_ptr(bool f64);
 
 int visgen_tile_vis();           // default visibilities per vis-gen tile
 int list_pad();                  // list-mode padding granule (entries)

@@ -116,3 +116,27 @@ def make_sky(wl):
 
 def all_pairs(n_sat):
     return np.array([(a, b) for a in range(n_sat) for b in range(a + 1, n_sat)], np.int32)
+
+
+def fine_sky(nside, factor=4, seed=7, sigma=1.0):
+    """A fine-cell sky under an nside sky: the HEALPix pixels of nside * factor
+    as cells (RING order), lognormal intensities, and each cell's parent RING
+    pixel, 1-based as the reference's NX file (NESTED: parent = child >> 2 log2 f).
+    Returns (cell_lmn [n][3] float64, FF [n] float32, NX [n] int32)."""
+    if factor < 1 or factor & (factor - 1):
+        raise ValueError("factor must be a power of two")
+    lib = _lib.load()
+    nf = nside * factor
+    n_cells, n_pix = 12 * nf * nf, 12 * nside * nside
+    lmn = np.empty((n_cells, 3), np.float64)
+    _lib.check(lib.vrs_healpix_lmn(nf, lmn.ctypes.data))
+    ring_of_nest_f = np.empty(n_cells, np.int64)
+    _lib.check(lib.vrs_healpix_nest2ring(nf, ring_of_nest_f.ctypes.data))
+    ring_of_nest_c = np.empty(n_pix, np.int64)
+    _lib.check(lib.vrs_healpix_nest2ring(nside, ring_of_nest_c.ctypes.data))
+    shift = 2 * (factor.bit_length() - 1)
+    nx = np.empty(n_cells, np.int32)
+    nx[ring_of_nest_f] = (ring_of_nest_c[np.arange(n_cells) >> shift] + 1).astype(np.int32)
+    ff = np.empty(n_cells, np.float32)
+    _lib.check(lib.vrs_sky_lognormal(seed, sigma, n_cells, 0, 1.0, 1.0, ff.ctypes.data))
+    return lmn, ff, nx
