@@ -273,6 +273,16 @@ vrs_status vrs_write_shade_txt(const char *path, const int32_t *box4, int64_t n)
 vrs_status vrs_read_shade_txt(const char *path, int32_t *box4, int64_t cap, int64_t *n_read);
 vrs_status vrs_write_image_txt(const char *path, const float *F, int64_t n);
 vrs_status vrs_read_image_txt(const char *path, float *F, int64_t cap, int64_t *n_read);
+/* Image expanded to fine cells, one line per cell c: F[nx[c] - nx_base]
+ * (the Ex files, vissGen1M_shade_accurate.cu:567-588). */
+vrs_status vrs_write_ex_txt(const char *path, const float *F, int64_t n_pix, const int32_t *nx,
+                            int64_t n_cells, int32_t nx_base);
+/* Binary visibilities (the reference never writes V): "VRSV", uint32 version 1,
+ * int64 n_maps, int64 n_vis, then [n_maps][n_vis] complex64 little-endian.
+ * The reader with V == NULL returns the header only; cap counts complex values. */
+vrs_status vrs_write_vis_bin(const char *path, const float *V, int64_t n_maps, int64_t n_vis);
+vrs_status vrs_read_vis_bin(const char *path, float *V, int64_t cap, int64_t *n_maps,
+                            int64_t *n_vis);
 
 /* ------------------------------------------------------- host-side helpers */
 /* Synthetic inputs (no reference counterpart; the reference reads MATLAB files).

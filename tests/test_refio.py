@@ -73,3 +73,35 @@ def test_io_errors(tmp_path):
     assert e.value.code == "VRS_E_IO"
     with pytest.raises(vrs.VrsError):
         refio.write_image(str(tmp_path / "no" / "\0bad"), np.zeros(2, np.float32))
+
+
+def test_ex_expansion_to_fine_cells(tmp_path):
+    # Ex file: one line per fine cell with F of its parent pixel
+    # (vissGen1M_shade_accurate.cu:567-588, NX 1-based)
+    F = np.array([1.5, -2.0, 3.0e-3], np.float32)
+    NX = np.array([3, 1, 1, 2, 3], np.int32)
+    p = refio.paths(str(tmp_path), 1e6, 2)["Ex"]
+    assert p.endswith("F_recon_1M_accu/Ex2frequency1M.txt")
+    refio.write_ex(p, F, NX)
+    assert open(p).read().splitlines() == ["%g" % float(F[q - 1]) for q in NX]
+    with pytest.raises(vrs.VrsError) as e:
+        refio.write_ex(p, F, np.array([4], np.int32))      # parent out of range
+    assert e.value.code == "VRS_E_ARG"
+
+
+def test_binary_visibilities_round_trip(tmp_path):
+    rng = np.random.default_rng(0)
+    V = (rng.standard_normal((3, 17)) + 1j * rng.standard_normal((3, 17))).astype(np.complex64)
+    p = refio.paths(str(tmp_path), 1e7, 1)["V"]
+    refio.write_vis(p, V)
+    assert os.path.getsize(p) == 24 + 3 * 17 * 8
+    with open(p, "rb") as f:
+        head = f.read(24)
+    assert head[:4] == b"VRSV" and np.frombuffer(head[4:8], np.uint32)[0] == 1
+    assert list(np.frombuffer(head[8:24], np.int64)) == [3, 17]
+    assert np.array_equal(refio.read_vis(p), V)
+    with open(p, "r+b") as f:                             # corrupt the magic
+        f.write(b"XXXX")
+    with pytest.raises(vrs.VrsError) as e:
+        refio.read_vis(p)
+    assert e.value.code == "VRS_E_IO"

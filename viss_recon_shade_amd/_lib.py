@@ -112,6 +112,10 @@ SIGNATURES = {
     "vrs_read_shade_txt": (C.c_int, [C.c_char_p, _vp, C.c_int64, C.POINTER(C.c_int64)]),
     "vrs_write_image_txt": (C.c_int, [C.c_char_p, _vp, C.c_int64]),
     "vrs_read_image_txt": (C.c_int, [C.c_char_p, _vp, C.c_int64, C.POINTER(C.c_int64)]),
+    "vrs_write_ex_txt": (C.c_int, [C.c_char_p, _vp, C.c_int64, _vp, C.c_int64, C.c_int32]),
+    "vrs_write_vis_bin": (C.c_int, [C.c_char_p, _vp, C.c_int64, C.c_int64]),
+    "vrs_read_vis_bin": (C.c_int, [C.c_char_p, _vp, C.c_int64, C.POINTER(C.c_int64),
+                                   C.POINTER(C.c_int64)]),
     "vrs_rng_normals": (C.c_int, [C.c_uint64, C.c_int64, _vp]),
     "vrs_rng_uniforms": (C.c_int, [C.c_uint64, C.c_int64, _vp]),
     "vrs_sky_lognormal": (C.c_int, [C.c_uint64, C.c_double, C.c_int64, C.c_int32, C.c_double,

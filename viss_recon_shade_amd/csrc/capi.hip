@@ -1324,6 +1324,34 @@ vrs_status vrs_read_image_txt(const char *path, float *F, int64_t cap, int64_t *
     return io_status(read_image_txt(path, F, cap, n_read), "vrs_read_image_txt", path);
 }
 
+vrs_status vrs_write_ex_txt(const char *path, const float *F, int64_t n_pix, const int32_t *nx,
+                            int64_t n_cells, int32_t nx_base) {
+    if (!path || n_pix < 0 || n_cells < 0 || (n_pix > 0 && !F) || (n_cells > 0 && !nx) ||
+        (nx_base != 0 && nx_base != 1))
+        return fail(nullptr, VRS_E_ARG, "bad path, F, nx, sizes or nx_base");
+    const int rc = write_ex_txt(path, F, n_pix, nx, n_cells, nx_base);
+    if (rc == -2) return fail(nullptr, VRS_E_ARG, "vrs_write_ex_txt: a parent pixel is out of range");
+    return io_status(rc, "vrs_write_ex_txt", path);
+}
+
+vrs_status vrs_write_vis_bin(const char *path, const float *V, int64_t n_maps, int64_t n_vis) {
+    if (!path || n_maps < 0 || n_vis < 0 || (n_maps * n_vis > 0 && !V))
+        return fail(nullptr, VRS_E_ARG, "bad path, V or sizes");
+    return io_status(write_vis_bin(path, V, n_maps, n_vis), "vrs_write_vis_bin", path);
+}
+
+vrs_status vrs_read_vis_bin(const char *path, float *V, int64_t cap, int64_t *n_maps,
+                            int64_t *n_vis) {
+    if (!path || !n_maps || !n_vis || cap < 0) return fail(nullptr, VRS_E_ARG, "bad arguments");
+    *n_maps = 0;
+    *n_vis = 0;
+    const int rc = read_vis_bin(path, V, cap, n_maps, n_vis);
+    if (rc == -2)
+        return fail(nullptr, VRS_E_ARG, "vrs_read_vis_bin: %lld visibilities exceed cap %lld",
+                    (long long)(*n_maps * *n_vis), (long long)cap);
+    return io_status(rc, "vrs_read_vis_bin", path);
+}
+
 }  // extern "C"
 
 // ---------------------------------------------------------------- uvw gridding

@@ -7,9 +7,12 @@
 //   shadeM             "tS tE pS pE" per line     shade10M.cu:108-127, read at
 //                                                 vissGen10M_shade.cu:376-411
 //   F (image)          one real value per line    vissGen10M_shade.cu:563-583
+//   Ex (fine cells)    F of each cell's parent    vissGen1M_shade_accurate.cu:567-588
+// plus a binary complex64 visibility file (the reference never writes V).
 // The readers never write past the caller's capacity (the reference's fixed
 // uvw_presize buffers have no bound check, vissGen10M_shade.cu:56,359).
 #include <cstdint>
+#include <cstring>
 #include <fstream>
 #include <string>
 
@@ -93,6 +96,56 @@ int read_image_txt(const char *path, float *F, int64_t cap, int64_t *n_read) {
     }
     *n_read = n;
     return 0;
+}
+
+int write_ex_txt(const char *path, const float *F, int64_t n_pix, const int32_t *nx,
+                 int64_t n_cells, int32_t nx_base) {
+    for (int64_t c = 0; c < n_cells; ++c) {
+        const int64_t q = (int64_t)nx[c] - nx_base;
+        if (q < 0 || q >= n_pix) return -2;
+    }
+    std::ofstream f(path);
+    if (!f.is_open()) return -1;
+    for (int64_t c = 0; c < n_cells; ++c) f << F[(int64_t)nx[c] - nx_base] << std::endl;
+    return f.good() ? 0 : -1;
+}
+
+// Binary visibilities: "VRSV", u32 version 1, i64 n_maps, i64 n_vis, then
+// n_maps * n_vis complex64 (re, im) little-endian.
+namespace {
+constexpr char kVisMagic[4] = {'V', 'R', 'S', 'V'};
+}
+
+int write_vis_bin(const char *path, const float *V, int64_t n_maps, int64_t n_vis) {
+    std::ofstream f(path, std::ios::binary);
+    if (!f.is_open()) return -1;
+    const uint32_t ver = 1;
+    f.write(kVisMagic, 4);
+    f.write(reinterpret_cast<const char *>(&ver), 4);
+    f.write(reinterpret_cast<const char *>(&n_maps), 8);
+    f.write(reinterpret_cast<const char *>(&n_vis), 8);
+    f.write(reinterpret_cast<const char *>(V), (std::streamsize)(sizeof(float) * 2 * n_maps * n_vis));
+    return f.good() ? 0 : -1;
+}
+
+int read_vis_bin(const char *path, float *V, int64_t cap, int64_t *n_maps, int64_t *n_vis) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f.is_open()) return -1;
+    char magic[4];
+    uint32_t ver = 0;
+    int64_t nm = 0, nv = 0;
+    f.read(magic, 4);
+    f.read(reinterpret_cast<char *>(&ver), 4);
+    f.read(reinterpret_cast<char *>(&nm), 8);
+    f.read(reinterpret_cast<char *>(&nv), 8);
+    if (!f.good() || std::memcmp(magic, kVisMagic, 4) != 0 || ver != 1 || nm < 0 || nv < 0)
+        return -3;
+    *n_maps = nm;
+    *n_vis = nv;
+    if (!V) return 0;                     // header only
+    if (nm * nv > cap) return -2;
+    f.read(reinterpret_cast<char *>(V), (std::streamsize)(sizeof(float) * 2 * nm * nv));
+    return f.good() ? 0 : -3;
 }
 
 }  // namespace vrs

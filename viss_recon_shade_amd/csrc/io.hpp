@@ -10,4 +10,8 @@ int write_shade_txt(const char *path, const int32_t *box4, int64_t n);
 int read_shade_txt(const char *path, int32_t *box4, int64_t cap, int64_t *n_read);
 int write_image_txt(const char *path, const float *F, int64_t n);
 int read_image_txt(const char *path, float *F, int64_t cap, int64_t *n_read);
+int write_ex_txt(const char *path, const float *F, int64_t n_pix, const int32_t *nx,
+                 int64_t n_cells, int32_t nx_base);
+int write_vis_bin(const char *path, const float *V, int64_t n_maps, int64_t n_vis);
+int read_vis_bin(const char *path, float *V, int64_t cap, int64_t *n_maps, int64_t *n_vis);
 }  // namespace vrs

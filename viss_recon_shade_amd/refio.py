@@ -7,6 +7,8 @@ Names (directory per frequency, one file per period p, 1-based):
   frequency_{F}M/updated_uvw{p}frequency{F}M.txt    read at vissGen10M_shade.cu:351
   frequency_{F}M/shadeM{p}frequency{F}M.txt         shade10M.cu:269, vissGen10M_shade.cu:377
   F_recon_{F}M/F{p}period{F}M.txt                   vissGen10M_shade.cu:573
+  F_recon_{F}M_accu/Ex{p}frequency{F}M.txt          vissGen1M_shade_accurate.cu:573-582
+  frequency_{F}M/V{p}frequency{F}M.bin              binary complex64 (no reference file)
 The readers and writers are the C-ABI's (vrs_read_* / vrs_write_*).
 """
 import ctypes as C
@@ -30,6 +32,8 @@ def paths(root, freq_hz, period):
         "updated_uvw": os.path.join(d, f"updated_uvw{period}frequency{f}M.txt"),
         "shadeM": os.path.join(d, f"shadeM{period}frequency{f}M.txt"),
         "F": os.path.join(root, f"F_recon_{f}M", f"F{period}period{f}M.txt"),
+        "Ex": os.path.join(root, f"F_recon_{f}M_accu", f"Ex{period}frequency{f}M.txt"),
+        "V": os.path.join(d, f"V{period}frequency{f}M.bin"),
     }
 
 
@@ -86,3 +90,33 @@ def read_image(path, cap=None):
     n = C.c_int64()
     _lib.check(_lib.load().vrs_read_image_txt(path.encode(), out.ctypes.data, cap, C.byref(n)))
     return out[:n.value]
+
+
+def write_ex(path, F, NX, one_based=True):
+    """One line per fine cell: F of its parent pixel (NX 1-based by default)."""
+    F = np.ascontiguousarray(F, np.float32).reshape(-1)
+    NX = np.ascontiguousarray(NX, np.int32).reshape(-1)
+    _lib.check(_lib.load().vrs_write_ex_txt(_mk(path), F.ctypes.data, len(F), NX.ctypes.data,
+                                            len(NX), 1 if one_based else 0))
+
+
+def write_vis(path, V):
+    """V [maps][n_vis] complex (or [..., 2] float32) -> binary complex64 file."""
+    V = np.asarray(V)
+    if np.iscomplexobj(V):
+        V = np.stack([V.real, V.imag], -1)
+    V = np.ascontiguousarray(V, np.float32)
+    if V.ndim == 2:
+        V = V[None]
+    _lib.check(_lib.load().vrs_write_vis_bin(_mk(path), V.ctypes.data, V.shape[0], V.shape[1]))
+
+
+def read_vis(path):
+    """Binary complex64 visibilities -> complex64 [maps][n_vis]."""
+    lib = _lib.load()
+    nm, nv = C.c_int64(), C.c_int64()
+    _lib.check(lib.vrs_read_vis_bin(path.encode(), None, 0, C.byref(nm), C.byref(nv)))
+    out = np.empty((nm.value, nv.value, 2), np.float32)
+    _lib.check(lib.vrs_read_vis_bin(path.encode(), out.ctypes.data, nm.value * nv.value,
+                                    C.byref(nm), C.byref(nv)))
+    return out[..., 0] + 1j * out[..., 1]
