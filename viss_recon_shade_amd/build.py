@@ -25,6 +25,10 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", "-I", os.path.join(ROOT, "include"), "-I
           "-Wall", "-Wno-unused-function"]
 DEVICE = ["-x", "hip", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
           "-packed-fp32-ops", "-munsafe-fp-atomics"]
+# Per-source device flags.  kernels.hip keeps its per-visibility accumulators
+# in unrolled register arrays updated under wave-uniform branches; promoting
+# those arrays to vector values makes every branch copy the whole array.
+EXTRA = {"kernels.hip": ["-mllvm", "-disable-promote-alloca-to-vector"]}
 
 
 def _compile(src):
@@ -33,7 +37,7 @@ def _compile(src):
     if src.endswith(".cpp"):
         cmd = ["g++"] + COMMON + ["-c", path, "-o", out]
     else:
-        cmd = [HIPCC] + COMMON + DEVICE + ["-c", path, "-o", out]
+        cmd = [HIPCC] + COMMON + DEVICE + EXTRA.get(src, []) + ["-c", path, "-o", out]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")

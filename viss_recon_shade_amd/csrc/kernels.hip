@@ -384,15 +384,6 @@ __global__ void __launch_bounds__(kBlock) visgen_kernel(GeomDev g, PhaseKnobs k,
             // from hoisting all of them into registers.
             asm volatile("" ::: "memory");
             const ChunkRec ch = g.chunk[c];
-            // lane-parallel classification of the tile's visibilities against
-            // this chunk; the results become wave-uniform bit masks
-            uint32_t in_m, out_m;
-            {
-                const int q = lane & (VT - 1);
-                const int cls = classify(ch, ents[q].a, ents[q].b);
-                in_m = (uint32_t)__ballot(cls == kIn) & kTileMask;
-                out_m = (uint32_t)__ballot(cls == kOut) & kTileMask;
-            }
             T l[kPixPerLane], m[kPixPerLane], n[kPixPerLane];
             float I[kPixPerLane];
 #pragma unroll
@@ -407,61 +398,144 @@ __global__ void __launch_bounds__(kBlock) visgen_kernel(GeomDev g, PhaseKnobs k,
                 }
                 I[j] = Im[pi];
             }
-            // Wholly shaded or wholly clear pairs run here; straddling pairs
-            // are skipped and done pixel by pixel below.
-            const uint32_t str_m = ~(in_m | out_m) & kTileMask;
-#pragma unroll
-            for (int q = 0; q < VT; ++q) {
-                if ((str_m >> q) & 1u) continue;
-                const bool sh = (in_m >> q) & 1u;
-                const T u = ents[q].u, vv = ents[q].v, w = ents[q].w, wo = ents[q].wo;
-#pragma unroll
-                for (int j = 0; j < kPixPerLane; ++j) {
-                    const float ie = sh ? t_occ : I[j];
-                    const T ph = fma(u, l[j], fma(vv, m[j], fma(w, n[j], wo)));
-                    const float r = fract_rev(ph);
-                    const float sn = __builtin_amdgcn_sinf(r);
-                    const float cs = __builtin_amdgcn_cosf(r);
-                    are[q] = fmaf(ie, cs, are[q]);
-                    aim[q] = fmaf(ie, sn, aim[q]);
+            if constexpr (!FINE) {
+                // Lane-parallel classification of the tile's pairs against this
+                // chunk, per cap: the results become wave-uniform bit masks.
+                // sa_m / sb_m: pairs whose cap A / B must be tested pixel by pixel.
+                uint32_t in_m, out_m, sa_m, sb_m;
+                {
+                    const int q = lane & (VT - 1);
+                    const int ca = classify_cap(ch, ents[q].a), cb = classify_cap(ch, ents[q].b);
+                    const bool in = (ca == kIn) || (cb == kIn);
+                    in_m = (uint32_t)__ballot(in) & kTileMask;
+                    out_m = (uint32_t)__ballot(ca == kOut && cb == kOut) & kTileMask;
+                    sa_m = (uint32_t)__ballot(!in && ca == kStraddle) & kTileMask;
+                    sb_m = (uint32_t)__ballot(!in && cb == kStraddle) & kTileMask;
                 }
-            }
-            // Straddling pairs: FP32 cap test per pixel; pixels inside the
-            // guard band take the exact FP64 predicate.  The wave's partial
-            // sum goes to the tile's LDS accumulator.
-            uint32_t todo = str_m;
-            while (todo) {
-                const int q = __ffs(todo) - 1;
-                todo &= todo - 1;
-                const CapRec A = ents[q].a, B = ents[q].b;
-                const T u = ents[q].u, vv = ents[q].v, w = ents[q].w, wo = ents[q].wo;
-                float dre = 0.0f, dim = 0.0f;
+                uint32_t unc_m = 0;  // straddling pairs with a pixel inside the FP32 guard band
 #pragma unroll
-                for (int j = 0; j < kPixPerLane; ++j) {
-                    const float sx = (float)l[j], sy = (float)m[j], sz = (float)n[j];
-                    const int64_t pi = (int64_t)c * kChunk + j * kWave + lane;
-                    float ie;
-                    if (FINE) {
-                        ie = fine_value(g, pi, sx, sy, sz, A, B, ents[q].ia, ents[q].ib, t_occ, I[j]);
-                    } else {
+                for (int q = 0; q < VT; ++q) {
+                    const T u = ents[q].u, vv = ents[q].v, w = ents[q].w, wo = ents[q].wo;
+                    if ((in_m >> q) & 1u) {          // wholly shaded: T_occ everywhere
+#pragma unroll
+                        for (int j = 0; j < kPixPerLane; ++j) {
+                            const T ph = fma(u, l[j], fma(vv, m[j], fma(w, n[j], wo)));
+                            const float r = fract_rev(ph);
+                            are[q] = fmaf(t_occ, __builtin_amdgcn_cosf(r), are[q]);
+                            aim[q] = fmaf(t_occ, __builtin_amdgcn_sinf(r), aim[q]);
+                        }
+                    } else if ((out_m >> q) & 1u) {  // wholly clear
+#pragma unroll
+                        for (int j = 0; j < kPixPerLane; ++j) {
+                            const T ph = fma(u, l[j], fma(vv, m[j], fma(w, n[j], wo)));
+                            const float r = fract_rev(ph);
+                            are[q] = fmaf(I[j], __builtin_amdgcn_cosf(r), are[q]);
+                            aim[q] = fmaf(I[j], __builtin_amdgcn_sinf(r), aim[q]);
+                        }
+                    } else {                         // straddling: FP32 cap test per pixel
+                        const bool ta = (sa_m >> q) & 1u, tb = (sb_m >> q) & 1u;
                         bool unc = false;
-                        bool sh = occ32(sx, sy, sz, A, unc) | occ32(sx, sy, sz, B, unc);
+#pragma unroll
+                        for (int j = 0; j < kPixPerLane; ++j) {
+                            const float sx = (float)l[j], sy = (float)m[j], sz = (float)n[j];
+                            bool sh = false;
+                            if (ta) sh = occ32(sx, sy, sz, ents[q].a, unc);
+                            if (tb) sh = occ32(sx, sy, sz, ents[q].b, unc) || sh;
+                            const float ie = sh ? t_occ : I[j];
+                            const T ph = fma(u, l[j], fma(vv, m[j], fma(w, n[j], wo)));
+                            const float r = fract_rev(ph);
+                            are[q] = fmaf(ie, __builtin_amdgcn_cosf(r), are[q]);
+                            aim[q] = fmaf(ie, __builtin_amdgcn_sinf(r), aim[q]);
+                        }
+                        if (__ballot(unc)) unc_m |= 1u << q;
+                    }
+                }
+                // Pixels inside the guard band: replace the FP32 decision by the
+                // exact FP64 predicate; the difference goes to the wave's LDS
+                // accumulator (only this wave touches corr[wave]: deterministic).
+                while (unc_m) {
+                    const int q = __ffs(unc_m) - 1;
+                    unc_m &= unc_m - 1;
+                    const bool ta = (sa_m >> q) & 1u, tb = (sb_m >> q) & 1u;
+                    const CapRec A = ents[q].a, B = ents[q].b;
+                    const T u = ents[q].u, vv = ents[q].v, w = ents[q].w, wo = ents[q].wo;
+                    float dre = 0.0f, dim = 0.0f;
+#pragma unroll
+                    for (int j = 0; j < kPixPerLane; ++j) {
+                        const float sx = (float)l[j], sy = (float)m[j], sz = (float)n[j];
+                        bool unc = false, sh = false;
+                        if (ta) sh = occ32(sx, sy, sz, A, unc);
+                        if (tb) sh = occ32(sx, sy, sz, B, unc) || sh;
                         if (unc) {
                             int nl;
-                            sh = occ_pair64(g.pos, g.pix64, pi, ents[q].ia, ents[q].ib, g.limb, &nl);
+                            const int64_t pi = (int64_t)c * kChunk + j * kWave + lane;
+                            const bool ex = occ_pair64(g.pos, g.pix64, pi, ents[q].ia, ents[q].ib,
+                                                       g.limb, &nl);
+                            if (ex != sh) {
+                                const float d = ex ? (t_occ - I[j]) : (I[j] - t_occ);
+                                const T ph = fma(u, l[j], fma(vv, m[j], fma(w, n[j], wo)));
+                                const float r = fract_rev(ph);
+                                dre = fmaf(d, __builtin_amdgcn_cosf(r), dre);
+                                dim = fmaf(d, __builtin_amdgcn_sinf(r), dim);
+                            }
                         }
-                        ie = sh ? t_occ : I[j];
                     }
-                    const T ph = fma(u, l[j], fma(vv, m[j], fma(w, n[j], wo)));
-                    const float r = fract_rev(ph);
-                    dre = fmaf(ie, __builtin_amdgcn_cosf(r), dre);
-                    dim = fmaf(ie, __builtin_amdgcn_sinf(r), dim);
+                    dre = wave_sum(dre);
+                    dim = wave_sum(dim);
+                    if (lane == 0) {
+                        corr[wave][q][0] += dre;
+                        corr[wave][q][1] += dim;
+                    }
                 }
-                dre = wave_sum(dre);
-                dim = wave_sum(dim);
-                if (lane == 0) {  // only this wave touches corr[wave]: deterministic order
-                    corr[wave][q][0] += dre;
-                    corr[wave][q][1] += dim;
+            } else {
+                // Fine-cell sky: chunk bounds cover every cell; straddling pairs
+                // go pixel by pixel through the pixel's own bounding cap and cells.
+                uint32_t in_m, out_m;
+                {
+                    const int q = lane & (VT - 1);
+                    const int cls = classify(ch, ents[q].a, ents[q].b);
+                    in_m = (uint32_t)__ballot(cls == kIn) & kTileMask;
+                    out_m = (uint32_t)__ballot(cls == kOut) & kTileMask;
+                }
+                const uint32_t str_m = ~(in_m | out_m) & kTileMask;
+#pragma unroll
+                for (int q = 0; q < VT; ++q) {
+                    if ((str_m >> q) & 1u) continue;
+                    const bool sh = (in_m >> q) & 1u;
+                    const T u = ents[q].u, vv = ents[q].v, w = ents[q].w, wo = ents[q].wo;
+#pragma unroll
+                    for (int j = 0; j < kPixPerLane; ++j) {
+                        const float ie = sh ? t_occ : I[j];
+                        const T ph = fma(u, l[j], fma(vv, m[j], fma(w, n[j], wo)));
+                        const float r = fract_rev(ph);
+                        are[q] = fmaf(ie, __builtin_amdgcn_cosf(r), are[q]);
+                        aim[q] = fmaf(ie, __builtin_amdgcn_sinf(r), aim[q]);
+                    }
+                }
+                uint32_t todo = str_m;
+                while (todo) {
+                    const int q = __ffs(todo) - 1;
+                    todo &= todo - 1;
+                    const CapRec A = ents[q].a, B = ents[q].b;
+                    const T u = ents[q].u, vv = ents[q].v, w = ents[q].w, wo = ents[q].wo;
+                    float dre = 0.0f, dim = 0.0f;
+#pragma unroll
+                    for (int j = 0; j < kPixPerLane; ++j) {
+                        const float sx = (float)l[j], sy = (float)m[j], sz = (float)n[j];
+                        const int64_t pi = (int64_t)c * kChunk + j * kWave + lane;
+                        const float ie = fine_value(g, pi, sx, sy, sz, A, B, ents[q].ia,
+                                                    ents[q].ib, t_occ, I[j]);
+                        const T ph = fma(u, l[j], fma(vv, m[j], fma(w, n[j], wo)));
+                        const float r = fract_rev(ph);
+                        dre = fmaf(ie, __builtin_amdgcn_cosf(r), dre);
+                        dim = fmaf(ie, __builtin_amdgcn_sinf(r), dim);
+                    }
+                    dre = wave_sum(dre);
+                    dim = wave_sum(dim);
+                    if (lane == 0) {
+                        corr[wave][q][0] += dre;
+                        corr[wave][q][1] += dim;
+                    }
                 }
             }
         }
