@@ -33,8 +33,6 @@ constexpr int kKS = 16;            // K per MFMA
 constexpr int kStage = 64;         // K per LDS stage
 constexpr int kWaves = 4;
 constexpr int kThreads = kWaves * 64;
-constexpr int kAPad = 8;           // halves of padding per LDS row (bank spread)
-constexpr int kARow = kStage + kAPad;
 
 static_assert(kChunk % kStage == 0, "stages must tile chunks");
 
@@ -206,11 +204,57 @@ __device__ __forceinline__ int acc_row(int r, int lane) {
 
 // ------------------------------------------------------------------ vis-gen
 // Block: 4 waves x 32 visibilities; all waves share the staged map tile.
+// Stages of kStage pixels are double-buffered: the next stage's map tile and
+// pixel vectors go straight from global memory into the other LDS buffer
+// while the current stage computes (one barrier per stage).
+// LDS tiles of the map operand are [NH][kMaps rows][8 segments of 8 halves],
+// unpadded; segment s of row r sits in slot s ^ (r & 7), so that the 32 rows
+// one A-operand read touches spread over all banks.
+__device__ __forceinline__ int a_slot(int row, int seg) { return seg ^ (row & 7); }
+
+typedef __attribute__((address_space(3))) void *lds_ptr_t;
+
+// Direct global -> LDS load of stage st (gfx950 global_load_lds_dwordx4):
+// every wave moves whole 1 KiB blocks, lane l writing bytes [16 l, 16 l + 16)
+// of its block.  NH halves-planes of the map operand, plus kStage FP64 pixel
+// vectors when pix64 is given.  Completion: s_waitcnt vmcnt(0) + barrier.
+template <int NH>
+__device__ __forceinline__ void stage_load_lds(const _Float16 *__restrict__ Ag, int64_t ld,
+                                               const double4 *__restrict__ pix64, int64_t st,
+                                               _Float16 *sA, double4 *sP) {
+    constexpr int kBlocks = NH * kMaps * (kStage / 8) / 64;   // 1 KiB blocks of the map tile
+    static_assert(kBlocks % kWaves == 0, "map tile splits evenly over the waves");
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t p0 = st * kStage;
+#pragma unroll
+    for (int r = 0; r < kBlocks / kWaves; ++r) {
+        const int blk = wave * (kBlocks / kWaves) + r;
+        const int i = blk * 64 + lane;                       // 16-byte slot index in the tile
+        const int hl = i / (kMaps * (kStage / 8));
+        const int rem = i % (kMaps * (kStage / 8));
+        const int row = rem / (kStage / 8), slot = rem % (kStage / 8);
+        const int seg = a_slot(row, slot);                   // the swizzle is an involution
+        const _Float16 *src = Ag + ((int64_t)hl * kMaps + row) * ld + p0 + seg * 8;
+        __builtin_amdgcn_global_load_lds(src, (lds_ptr_t)(sA + blk * 512), 16, 0, 0);
+    }
+    if (pix64) {
+        // kStage double4 = kStage / 32 blocks of 1 KiB, one per wave
+        static_assert(kStage / 32 <= kWaves, "pixel blocks fit the waves");
+        if (wave < kStage / 32) {
+            const char *src = reinterpret_cast<const char *>(pix64 + p0) + wave * 1024 + lane * 16;
+            __builtin_amdgcn_global_load_lds(src, (lds_ptr_t)(reinterpret_cast<char *>(sP) + wave * 1024),
+                                             16, 0, 0);
+        }
+    }
+}
+
+__device__ __forceinline__ void wait_vm() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
 __global__ void __launch_bounds__(kThreads, 2) mfma_visgen_kernel(
     GeomDev g, PhaseKnobs k, int32_t n_maps, const _Float16 *__restrict__ A16,
     const float *__restrict__ inv_scale, float2 *__restrict__ V) {
-    __shared__ _Float16 sA[2][kMaps][kARow];   // [hi/lo][map][pixel]
-    __shared__ double4 sP[kStage];             // pixel (l, m, n) FP64
+    __shared__ _Float16 sA[2][2][kMaps][kStage];  // [buffer][hi/lo][map][pixel slot]
+    __shared__ double4 sP[2][kStage];             // [buffer] pixel (l, m, n) FP64
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int col = lane & 31;                 // this lane's visibility within the wave tile
@@ -218,6 +262,7 @@ __global__ void __launch_bounds__(kThreads, 2) mfma_visgen_kernel(
     const int64_t n_vtiles = (g.n_vis + kWaves * kNT - 1) / (kWaves * kNT);
     const int64_t n_groups = (n_maps + kMaps - 1) / kMaps;
     const int64_t n_tiles = n_vtiles * n_groups;
+    const int64_t n_stages = g.n_pad / kStage;
     const float t_occ = k.t_occ;
 
     for (int64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
@@ -248,20 +293,23 @@ __global__ void __launch_bounds__(kThreads, 2) mfma_visgen_kernel(
         int cls = kOut;
 
         const _Float16 *Ag = A16 + grp * 2 * kMaps * g.n_pad;
-        for (int64_t p0 = 0; p0 < g.n_pad; p0 += kStage) {
-            __syncthreads();
-            // stage the map tile (2 x 64 maps x 64 pixels) and the pixel vectors
-            for (int i = threadIdx.x; i < 2 * kMaps * (kStage / 8); i += kThreads) {
-                const int hl = i / (kMaps * (kStage / 8));
-                const int rem = i % (kMaps * (kStage / 8));
-                const int row = rem / (kStage / 8), seg = rem % (kStage / 8);
-                const uint4 val = *reinterpret_cast<const uint4 *>(
-                    Ag + ((int64_t)hl * kMaps + row) * g.n_pad + p0 + seg * 8);
-                *reinterpret_cast<uint4 *>(&sA[hl][row][seg * 8]) = val;
-            }
-            if (threadIdx.x < kStage) sP[threadIdx.x] = g.pix64[p0 + threadIdx.x];
-            if ((p0 % kChunk) == 0) cls = classify_m(g.chunk[p0 / kChunk], A, B);
-            __syncthreads();
+        // Both buffers are free here: every wave passed the last barrier of
+        // the previous tile after its final reads.
+        stage_load_lds<2>(Ag, g.n_pad, g.pix64, 0, &sA[0][0][0][0], &sP[0][0]);
+        wait_vm();
+        __syncthreads();
+        for (int64_t st = 0; st < n_stages; ++st) {
+            const int buf = (int)(st & 1);
+            const int64_t p0 = st * kStage;
+            // chunk record first, so that waiting for it does not wait for the prefetch
+            const bool new_chunk = (p0 % kChunk) == 0;
+            ChunkRec ch;
+            if (new_chunk) ch = g.chunk[p0 / kChunk];
+            if (new_chunk) cls = classify_m(ch, A, B);
+            // the other buffer was last read before the previous barrier
+            if (st + 1 < n_stages)
+                stage_load_lds<2>(Ag, g.n_pad, g.pix64, st + 1, &sA[buf ^ 1][0][0][0],
+                                  &sP[buf ^ 1][0]);
 #pragma unroll
             for (int ks = 0; ks < kStage / kKS; ++ks) {
                 H8 bre_h, bre_l, bim_h, bim_l;
@@ -271,7 +319,7 @@ __global__ void __launch_bounds__(kThreads, 2) mfma_visgen_kernel(
 #pragma unroll
                     for (int e = 0; e < 2; ++e) {
                         const int kk = ks * kKS + kg * 8 + t2 * 2 + e;
-                        const double4 s = sP[kk];
+                        const double4 s = sP[buf][kk];
                         const double ph = fma(u, s.x, fma(vv, s.y, fma(w, s.z, wo)));
                         const float r = (float)__builtin_amdgcn_fract(ph);
                         float c = __builtin_amdgcn_cosf(r);
@@ -292,9 +340,9 @@ __global__ void __launch_bounds__(kThreads, 2) mfma_visgen_kernel(
                 }
 #pragma unroll
                 for (int mt = 0; mt < 2; ++mt) {
-                    half8 ah, al;
-                    ah = *reinterpret_cast<const half8 *>(&sA[0][mt * kMT + col][ks * kKS + kg * 8]);
-                    al = *reinterpret_cast<const half8 *>(&sA[1][mt * kMT + col][ks * kKS + kg * 8]);
+                    const int row = mt * kMT + col, so = a_slot(row, ks * 2 + kg) * 8;
+                    const half8 ah = *reinterpret_cast<const half8 *>(&sA[buf][0][row][so]);
+                    const half8 al = *reinterpret_cast<const half8 *>(&sA[buf][1][row][so]);
                     acc[mt][0] = mfma(ah, bre_h.v, acc[mt][0]);
                     acc[mt][0] = mfma(ah, bre_l.v, acc[mt][0]);
                     acc[mt][0] = mfma(al, bre_h.v, acc[mt][0]);
@@ -303,6 +351,8 @@ __global__ void __launch_bounds__(kThreads, 2) mfma_visgen_kernel(
                     acc[mt][1] = mfma(al, bim_h.v, acc[mt][1]);
                 }
             }
+            wait_vm();
+            __syncthreads();
         }
         // shaded-pixel term: both K halves of this column
         gre += __shfl_xor(gre, 32, 64);
@@ -325,16 +375,70 @@ __global__ void __launch_bounds__(kThreads, 2) mfma_visgen_kernel(
 
 // ------------------------------------------------------------------ recon
 // Block: 4 waves x 32 pixels; K runs over visibilities, staged 64 at a time.
+// The weighted-V tile of the next stage moves straight into the other LDS
+// buffer while the current stage computes; wave 0 reads the next stage's
+// visibility geometry into registers at the top of a stage and classifies
+// and stores it at the bottom.
+struct CapLite {  // the cap fields a pixel test reads (CapRec without its 32-byte alignment)
+    float dx, dy, dz, c, g;
+};
+// 88 bytes: lanes 0-31 and 32-63 read entries 8 apart, 704 bytes = 176 dwords,
+// so their 16-byte reads land on disjoint banks.
+struct VisS {
+    double u, v, w, wo;
+    int32_t cls, ia, ib, pad0;
+    CapLite a, b;
+};
+static_assert(sizeof(VisS) == 88, "staged visibility record layout");
+
+__device__ __forceinline__ CapLite cap_lite(const CapRec &r) { return {r.dx, r.dy, r.dz, r.c, r.g}; }
+__device__ __forceinline__ CapRec cap_full(const CapLite &l) {
+    return {l.dx, l.dy, l.dz, l.c, 0.f, l.g, 0.f, 0.f};
+}
+
+struct VisRaw {  // what wave 0 holds between the two halves of staging
+    int2 pr;
+    int t;
+    bool ok;
+    double4 q;
+};
+
+__device__ __forceinline__ void vis_raw_load(const GeomDev &g, int64_t vv, VisRaw &r) {
+    r.ok = vv < g.n_vis;
+    if (r.ok) {
+        const int b = (int)(vv / g.n_t);
+        r.t = (int)(vv % g.n_t);
+        r.pr = g.pairs[b];
+        r.q = g.uvw64[vv];
+    }
+}
+
+__device__ __forceinline__ void vis_stage_store(const GeomDev &g, const PhaseKnobs &k,
+                                                const ChunkRec &ch, const VisRaw &r, VisS &e) {
+    if (r.ok) {
+        e.ia = r.pr.x * g.n_t + r.t;
+        e.ib = r.pr.y * g.n_t + r.t;
+        const CapRec a = g.cap[e.ia], b = g.cap[e.ib];
+        e.a = cap_lite(a);
+        e.b = cap_lite(b);
+        e.u = r.q.x * k.s; e.v = r.q.y * k.s; e.w = r.q.z * k.s;
+        e.wo = k.wfac ? 0.0 : -e.w;
+        e.cls = classify_m(ch, a, b);
+    } else {
+        e.ia = e.ib = 0;
+        const CapLite none = {0.f, 0.f, 0.f, 2.0f, 0.f};
+        e.a = none;
+        e.b = none;
+        e.u = e.v = e.w = e.wo = 0.0;
+        e.cls = kIn;  // padding: contributes nothing
+    }
+}
+
 __global__ void __launch_bounds__(kThreads, 2) mfma_recon_kernel(
     GeomDev g, PhaseKnobs k, int32_t n_maps, int64_t n_vis_pad, const _Float16 *__restrict__ A16,
     const float *__restrict__ inv_scale, int32_t n_seg, float *__restrict__ part) {
-    __shared__ _Float16 sA[4][kMaps][kARow];   // [re hi, re lo, im hi, im lo][map][vis]
-    struct VisS {
-        double u, v, w, wo;
-        CapRec a, b;
-        int ia, ib, cls, pad0;
-    };
-    __shared__ VisS sV[kStage];
+    __shared__ _Float16 sA[2][4][kMaps][kStage];  // [buffer][re hi, re lo, im hi, im lo][map][vis slot]
+    __shared__ VisS sV[2][kStage];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int col = lane & 31;
@@ -360,40 +464,27 @@ __global__ void __launch_bounds__(kThreads, 2) mfma_recon_kernel(
         const _Float16 *Ag = A16 + grp * 4 * kMaps * n_vis_pad;
         const int64_t vb = seg * seg_len;
         const int64_t ve = (vb + seg_len < n_vis_pad) ? vb + seg_len : n_vis_pad;
-        for (int64_t v0 = vb; v0 < ve; v0 += kStage) {
-            __syncthreads();
-            for (int i = threadIdx.x; i < 4 * kMaps * (kStage / 8); i += kThreads) {
-                const int hl = i / (kMaps * (kStage / 8));
-                const int rem = i % (kMaps * (kStage / 8));
-                const int row = rem / (kStage / 8), sg = rem % (kStage / 8);
-                const uint4 val = *reinterpret_cast<const uint4 *>(
-                    Ag + ((int64_t)hl * kMaps + row) * n_vis_pad + v0 + sg * 8);
-                *reinterpret_cast<uint4 *>(&sA[hl][row][sg * 8]) = val;
-            }
+        const int64_t n_st = ve > vb ? (ve - vb) / kStage : 0;   // both multiples of kStage
+        VisRaw raw;
+        if (n_st > 0) {
+            // Both buffers are free: every wave passed the previous item's last barrier.
+            stage_load_lds<4>(Ag, n_vis_pad, nullptr, vb / kStage, &sA[0][0][0][0], nullptr);
             if (threadIdx.x < kStage) {
-                const int64_t vv = v0 + threadIdx.x;
-                VisS &e = sV[threadIdx.x];
-                if (vv < g.n_vis) {
-                    const int b = (int)(vv / g.n_t), t = (int)(vv % g.n_t);
-                    const int2 pr = g.pairs[b];
-                    e.ia = pr.x * g.n_t + t;
-                    e.ib = pr.y * g.n_t + t;
-                    e.a = g.cap[e.ia];
-                    e.b = g.cap[e.ib];
-                    const double4 q = g.uvw64[vv];
-                    e.u = q.x * k.s; e.v = q.y * k.s; e.w = q.z * k.s;
-                    e.wo = k.wfac ? 0.0 : -e.w;
-                    e.cls = classify_m(ch, e.a, e.b);
-                } else {
-                    e.ia = e.ib = 0;
-                    const CapRec none = {0.f, 0.f, 0.f, 2.0f, 0.f, 0.f, 0.f, 0.f};
-                    e.a = none;
-                    e.b = none;
-                    e.u = e.v = e.w = e.wo = 0.0;
-                    e.cls = kIn;  // padding: contributes nothing
-                }
+                vis_raw_load(g, vb + threadIdx.x, raw);
+                vis_stage_store(g, k, ch, raw, sV[0][threadIdx.x]);
             }
+            wait_vm();
             __syncthreads();
+        }
+        for (int64_t st = 0; st < n_st; ++st) {
+            const int buf = (int)(st & 1);
+            const bool more = st + 1 < n_st;
+            const int64_t v1 = vb + (st + 1) * kStage;
+            if (more) {
+                if (threadIdx.x < kStage) vis_raw_load(g, v1 + threadIdx.x, raw);
+                stage_load_lds<4>(Ag, n_vis_pad, nullptr, v1 / kStage, &sA[buf ^ 1][0][0][0],
+                                  nullptr);
+            }
 #pragma unroll
             for (int ks = 0; ks < kStage / kKS; ++ks) {
                 H8 bre_h, bre_l, bim_h, bim_l;
@@ -402,11 +493,12 @@ __global__ void __launch_bounds__(kThreads, 2) mfma_recon_kernel(
                     float cs[2], sn[2];
 #pragma unroll
                     for (int e = 0; e < 2; ++e) {
-                        const VisS &q = sV[ks * kKS + kg * 8 + t2 * 2 + e];
+                        const VisS &q = sV[buf][ks * kKS + kg * 8 + t2 * 2 + e];
                         const int cls = q.cls;
                         float c = 0.0f, sv = 0.0f;
                         bool sh = (cls == kIn);
-                        if (cls == kStraddle) sh = pair_shaded(g, s, q.a, q.b, q.ia, q.ib);
+                        if (cls == kStraddle)
+                            sh = pair_shaded(g, s, cap_full(q.a), cap_full(q.b), q.ia, q.ib);
                         if (!sh) {
                             const double ph = fma(q.u, s.x, fma(q.v, s.y, fma(q.w, s.z, q.wo)));
                             const float r = (float)__builtin_amdgcn_fract(ph);
@@ -421,12 +513,11 @@ __global__ void __launch_bounds__(kThreads, 2) mfma_recon_kernel(
                 }
 #pragma unroll
                 for (int mt = 0; mt < 2; ++mt) {
-                    const int row = mt * kMT + col;
-                    const int kc = ks * kKS + kg * 8;
-                    const half8 arh = *reinterpret_cast<const half8 *>(&sA[0][row][kc]);
-                    const half8 arl = *reinterpret_cast<const half8 *>(&sA[1][row][kc]);
-                    const half8 aih = *reinterpret_cast<const half8 *>(&sA[2][row][kc]);
-                    const half8 ail = *reinterpret_cast<const half8 *>(&sA[3][row][kc]);
+                    const int row = mt * kMT + col, so = a_slot(row, ks * 2 + kg) * 8;
+                    const half8 arh = *reinterpret_cast<const half8 *>(&sA[buf][0][row][so]);
+                    const half8 arl = *reinterpret_cast<const half8 *>(&sA[buf][1][row][so]);
+                    const half8 aih = *reinterpret_cast<const half8 *>(&sA[buf][2][row][so]);
+                    const half8 ail = *reinterpret_cast<const half8 *>(&sA[buf][3][row][so]);
                     acc[mt] = mfma(arh, bre_h.v, acc[mt]);
                     acc[mt] = mfma(arh, bre_l.v, acc[mt]);
                     acc[mt] = mfma(arl, bre_h.v, acc[mt]);
@@ -435,6 +526,10 @@ __global__ void __launch_bounds__(kThreads, 2) mfma_recon_kernel(
                     acc[mt] = mfma(ail, bim_h.v, acc[mt]);
                 }
             }
+            // the other buffer was last read before the previous barrier
+            if (more && threadIdx.x < kStage) vis_stage_store(g, k, ch, raw, sV[buf ^ 1][threadIdx.x]);
+            wait_vm();
+            __syncthreads();
         }
 #pragma unroll
         for (int mt = 0; mt < 2; ++mt)
