@@ -95,6 +95,31 @@ __device__ __forceinline__ bool pair_shaded(const GeomDev &g, const double4 s64,
     return sh;
 }
 
+// Shade bits of 8 consecutive staged pixels for one pair (bit t: pixel t):
+// FP32 cap tests, the exact FP64 predicate for pixels in the guard band.
+__device__ __forceinline__ uint32_t straddle_mask8(const GeomDev &g, const double4 *sp,
+                                                const CapRec &A, const CapRec &B, int ia, int ib) {
+    uint32_t m = 0, um = 0;
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+        const double4 s = sp[t];
+        const float sx = (float)s.x, sy = (float)s.y, sz = (float)s.z;
+        bool unc = false;
+        const bool sh = occ32m(sx, sy, sz, A, unc) | occ32m(sx, sy, sz, B, unc);
+        m |= (uint32_t)sh << t;
+        um |= (uint32_t)unc << t;
+    }
+    while (um) {
+        const int t = __ffs(um) - 1;
+        um &= um - 1;
+        const double4 s = sp[t];
+        const bool ex = occ64m(g.pos[ia], s.x, s.y, s.z, g.limb) |
+                        occ64m(g.pos[ib], s.x, s.y, s.z, g.limb);
+        m = (m & ~(1u << t)) | ((uint32_t)ex << t);
+    }
+    return m;
+}
+
 // ------------------------------------------------------------------ prep
 __global__ void zero_u32_kernel(unsigned int *__restrict__ x, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -208,9 +233,11 @@ __device__ __forceinline__ int acc_row(int r, int lane) {
 // pixel vectors go straight from global memory into the other LDS buffer
 // while the current stage computes (one barrier per stage).
 // LDS tiles of the map operand are [NH][kMaps rows][8 segments of 8 halves],
-// unpadded; segment s of row r sits in slot s ^ (r & 7), so that the 32 rows
-// one A-operand read touches spread over all banks.
-__device__ __forceinline__ int a_slot(int row, int seg) { return seg ^ (row & 7); }
+// unpadded (128-byte rows).  Segment s of row r sits in slot s ^ ((r >> 1) & 7):
+// with 64 four-byte banks, even and odd rows start on opposite bank halves and
+// the slot swizzle spreads the 8 rows of each half over its 8 16-byte groups,
+// so the 16 rows read per LDS cycle by an A-operand load hit 64 distinct banks.
+__device__ __forceinline__ int a_slot(int row, int seg) { return seg ^ ((row >> 1) & 7); }
 
 typedef __attribute__((address_space(3))) void *lds_ptr_t;
 
@@ -255,6 +282,7 @@ __global__ void __launch_bounds__(kThreads, 2) mfma_visgen_kernel(
     const float *__restrict__ inv_scale, float2 *__restrict__ V) {
     __shared__ _Float16 sA[2][2][kMaps][kStage];  // [buffer][hi/lo][map][pixel slot]
     __shared__ double4 sP[2][kStage];             // [buffer] pixel (l, m, n) FP64
+    __shared__ CapRec sCap[kWaves * kNT][2];      // caps of the block's visibilities
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int col = lane & 31;                 // this lane's visibility within the wave tile
@@ -272,16 +300,21 @@ __global__ void __launch_bounds__(kThreads, 2) mfma_visgen_kernel(
         // this lane's visibility
         double u = 0.0, vv = 0.0, w = 0.0;
         int ia = 0, ib = 0;
-        CapRec A = {0.f, 0.f, 0.f, 2.0f, 0.f, 0.f, 0.f, 0.f}, B = A;
+        const int vc = wave * kNT + col;        // this lane's visibility within the block
         if (vok) {
             const int b = (int)(v / g.n_t), t = (int)(v % g.n_t);
             const int2 pr = g.pairs[b];
             ia = pr.x * g.n_t + t;
             ib = pr.y * g.n_t + t;
-            A = g.cap[ia];
-            B = g.cap[ib];
             const double4 q = g.uvw64[v];
             u = q.x * k.s; vv = q.y * k.s; w = q.z * k.s;
+        }
+        // the caps are read once per chunk and in straddling K-steps: keep them in LDS
+        // (sCap is free: every wave passed the previous tile's last barrier)
+        if (kg == 0) {
+            const CapRec none = {0.f, 0.f, 0.f, 2.0f, 0.f, 0.f, 0.f, 0.f};
+            sCap[vc][0] = vok ? g.cap[ia] : none;
+            sCap[vc][1] = vok ? g.cap[ib] : none;
         }
         const double wo = k.wfac ? 0.0 : -w;
         floatx16 acc[2][2];  // [m tile][re, im]
@@ -305,38 +338,41 @@ __global__ void __launch_bounds__(kThreads, 2) mfma_visgen_kernel(
             const bool new_chunk = (p0 % kChunk) == 0;
             ChunkRec ch;
             if (new_chunk) ch = g.chunk[p0 / kChunk];
-            if (new_chunk) cls = classify_m(ch, A, B);
+            if (new_chunk) cls = classify_m(ch, sCap[vc][0], sCap[vc][1]);
             // the other buffer was last read before the previous barrier
             if (st + 1 < n_stages)
                 stage_load_lds<2>(Ag, g.n_pad, g.pix64, st + 1, &sA[buf ^ 1][0][0][0],
                                   &sP[buf ^ 1][0]);
-#pragma unroll
+#pragma unroll 2
             for (int ks = 0; ks < kStage / kKS; ++ks) {
+                // E for this lane's 8 pixels: straight-line phase and sin/cos
+                // first, then the shade mask, so that the 8 chains interleave.
+                const int kb = ks * kKS + kg * 8;
+                float cs[8], sn[8];
+#pragma unroll
+                for (int t = 0; t < 8; ++t) {
+                    const double4 sp = sP[buf][kb + t];
+                    const double ph = fma(u, sp.x, fma(vv, sp.y, fma(w, sp.z, wo)));
+                    const float r = (float)__builtin_amdgcn_fract(ph);
+                    cs[t] = __builtin_amdgcn_cosf(r);
+                    sn[t] = __builtin_amdgcn_sinf(r);
+                }
+                uint32_t shm = (cls == kIn) ? 0xffu : 0u;
+                if (cls == kStraddle)
+                    shm = straddle_mask8(g, &sP[buf][kb], sCap[vc][0], sCap[vc][1], ia, ib);
                 H8 bre_h, bre_l, bim_h, bim_l;
 #pragma unroll
-                for (int t2 = 0; t2 < 4; ++t2) {
-                    float cs[2], sn[2];
+                for (int t = 0; t < 8; ++t) {
+                    const bool sh = (shm >> t) & 1u;
+                    gre += sh ? cs[t] : 0.0f;
+                    gim += sh ? sn[t] : 0.0f;
+                    cs[t] = sh ? 0.0f : cs[t];
+                    sn[t] = sh ? 0.0f : -sn[t];   // E = cos - i sin
+                }
 #pragma unroll
-                    for (int e = 0; e < 2; ++e) {
-                        const int kk = ks * kKS + kg * 8 + t2 * 2 + e;
-                        const double4 s = sP[buf][kk];
-                        const double ph = fma(u, s.x, fma(vv, s.y, fma(w, s.z, wo)));
-                        const float r = (float)__builtin_amdgcn_fract(ph);
-                        float c = __builtin_amdgcn_cosf(r);
-                        float sv = __builtin_amdgcn_sinf(r);
-                        bool sh = (cls == kIn);
-                        if (cls == kStraddle) sh = pair_shaded(g, s, A, B, ia, ib);
-                        if (sh) {
-                            gre += c;
-                            gim += sv;
-                            c = 0.0f;
-                            sv = 0.0f;
-                        }
-                        cs[e] = c;
-                        sn[e] = -sv;  // E = cos - i sin
-                    }
-                    split2(cs[0], cs[1], bre_h.u[t2], bre_l.u[t2]);
-                    split2(sn[0], sn[1], bim_h.u[t2], bim_l.u[t2]);
+                for (int t2 = 0; t2 < 4; ++t2) {
+                    split2(cs[2 * t2], cs[2 * t2 + 1], bre_h.u[t2], bre_l.u[t2]);
+                    split2(sn[2 * t2], sn[2 * t2 + 1], bim_h.u[t2], bim_l.u[t2]);
                 }
 #pragma unroll
                 for (int mt = 0; mt < 2; ++mt) {
@@ -487,29 +523,38 @@ __global__ void __launch_bounds__(kThreads, 2) mfma_recon_kernel(
             }
 #pragma unroll
             for (int ks = 0; ks < kStage / kKS; ++ks) {
+                // E for this lane's pixel and 8 staged visibilities: straight-line
+                // phase and sin/cos, then the (wave-uniform per visibility) classes.
+                const int kb = ks * kKS + kg * 8;
+                float cs[8], sn[8];
+                uint32_t shm = 0, stm = 0;
+#pragma unroll
+                for (int t = 0; t < 8; ++t) {
+                    const VisS &q = sV[buf][kb + t];
+                    const double ph = fma(q.u, s.x, fma(q.v, s.y, fma(q.w, s.z, q.wo)));
+                    const float r = (float)__builtin_amdgcn_fract(ph);
+                    cs[t] = __builtin_amdgcn_cosf(r);
+                    sn[t] = __builtin_amdgcn_sinf(r);
+                    shm |= (uint32_t)(q.cls == kIn) << t;
+                    stm |= (uint32_t)(q.cls == kStraddle) << t;
+                }
+                while (stm) {
+                    const int t = __ffs(stm) - 1;
+                    stm &= stm - 1;
+                    const VisS &q = sV[buf][kb + t];
+                    if (pair_shaded(g, s, cap_full(q.a), cap_full(q.b), q.ia, q.ib)) shm |= 1u << t;
+                }
                 H8 bre_h, bre_l, bim_h, bim_l;
 #pragma unroll
-                for (int t2 = 0; t2 < 4; ++t2) {
-                    float cs[2], sn[2];
+                for (int t = 0; t < 8; ++t) {
+                    const bool sh = (shm >> t) & 1u;
+                    cs[t] = sh ? 0.0f : cs[t];
+                    sn[t] = sh ? 0.0f : -sn[t];   // Re(V e^{+i theta}) = Vr cos - Vi sin
+                }
 #pragma unroll
-                    for (int e = 0; e < 2; ++e) {
-                        const VisS &q = sV[buf][ks * kKS + kg * 8 + t2 * 2 + e];
-                        const int cls = q.cls;
-                        float c = 0.0f, sv = 0.0f;
-                        bool sh = (cls == kIn);
-                        if (cls == kStraddle)
-                            sh = pair_shaded(g, s, cap_full(q.a), cap_full(q.b), q.ia, q.ib);
-                        if (!sh) {
-                            const double ph = fma(q.u, s.x, fma(q.v, s.y, fma(q.w, s.z, q.wo)));
-                            const float r = (float)__builtin_amdgcn_fract(ph);
-                            c = __builtin_amdgcn_cosf(r);
-                            sv = __builtin_amdgcn_sinf(r);
-                        }
-                        cs[e] = c;
-                        sn[e] = -sv;  // Re(V e^{+i theta}) = Vr cos - Vi sin
-                    }
-                    split2(cs[0], cs[1], bre_h.u[t2], bre_l.u[t2]);
-                    split2(sn[0], sn[1], bim_h.u[t2], bim_l.u[t2]);
+                for (int t2 = 0; t2 < 4; ++t2) {
+                    split2(cs[2 * t2], cs[2 * t2 + 1], bre_h.u[t2], bre_l.u[t2]);
+                    split2(sn[2 * t2], sn[2 * t2 + 1], bim_h.u[t2], bim_l.u[t2]);
                 }
 #pragma unroll
                 for (int mt = 0; mt < 2; ++mt) {
