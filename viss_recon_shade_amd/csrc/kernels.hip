@@ -361,7 +361,7 @@ __global__ void __launch_bounds__(kBlock) visgen_kernel(GeomDev g, PhaseKnobs k,
     __shared__ float corr[kWavesPerBlock][VT][2];  // straddling pairs, per wave
     constexpr uint32_t kTileMask = (VT == 32) ? 0xffffffffu : ((1u << VT) - 1u);
     const int lane = threadIdx.x & (kWave - 1);
-    const int wave = threadIdx.x >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
     const bool list = g.vis_sat != nullptr;
     const int64_t n_vtiles = (g.n_vis + VT - 1) / VT;
     const int64_t n_tiles = list ? n_vtiles : n_vtiles * n_maps;
@@ -600,7 +600,7 @@ __global__ void __launch_bounds__(kBlock) recon_kernel(GeomDev g, PhaseKnobs k, 
     __shared__ int wave_live[kWavesPerBlock];
     __shared__ float red[kWavesPerBlock][kChunk];
     const int lane = threadIdx.x & (kWave - 1);
-    const int wave = threadIdx.x >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
     const bool list = g.vis_sat != nullptr;
     const int64_t seg_len = (g.n_vis + n_seg - 1) / n_seg;
     const int64_t n_slots = list ? g.n_slots : (int64_t)n_maps * n_seg;
@@ -664,6 +664,9 @@ __global__ void __launch_bounds__(kBlock) recon_kernel(GeomDev g, PhaseKnobs k, 
                 base += (q < wave) ? wave_live[q] : 0;
                 n_live += wave_live[q];
             }
+            // both are wave-uniform; saying so keeps the pixel loop below a scalar loop
+            base = __builtin_amdgcn_readfirstlane(base);
+            n_live = __builtin_amdgcn_readfirstlane(n_live);
             if (live) {
                 const int pos = base + __popcll(bal & ((1ull << lane) - 1ull));
                 RecHot<F64> &e = hot[pos];
@@ -690,8 +693,8 @@ __global__ void __launch_bounds__(kBlock) recon_kernel(GeomDev g, PhaseKnobs k, 
 #pragma unroll 2
             for (int q = wave; q < n_live; q += kWavesPerBlock) {
                 const RecHot<F64> e = hot[q];
-                float vr[kPixPerLane], vi[kPixPerLane];
                 if (__builtin_amdgcn_readfirstlane(e.cls) == kStraddle) {
+                    // the pair straddles the chunk: test each pixel
                     const CapRec A2 = cold[q].a, B2 = cold[q].b;
 #pragma unroll
                     for (int j = 0; j < kPixPerLane; ++j) {
@@ -704,25 +707,22 @@ __global__ void __launch_bounds__(kBlock) recon_kernel(GeomDev g, PhaseKnobs k, 
                                             (int64_t)c * kChunk + j * kWave + lane, cold[q].ia,
                                             cold[q].ib, g.limb, &nl);
                         }
-                        vr[j] = sh ? 0.0f : e.vr;
-                        vi[j] = sh ? 0.0f : e.vi;
+                        const float vr = sh ? 0.0f : e.vr, vi = sh ? 0.0f : e.vi;
+                        const T ph = fma(e.u, l[j], fma(e.v, m[j], fma(e.w, n[j], e.wo)));
+                        const float r = fract_rev(ph);
+                        // Re(V exp(+i theta)) = Vr cos - Vi sin
+                        acc[j] = fmaf(vr, __builtin_amdgcn_cosf(r), acc[j]);
+                        acc[j] = fmaf(-vi, __builtin_amdgcn_sinf(r), acc[j]);
                     }
                 } else {
+                    // wholly clear for the chunk (wholly shaded pairs were compacted away)
 #pragma unroll
                     for (int j = 0; j < kPixPerLane; ++j) {
-                        vr[j] = e.vr;
-                        vi[j] = e.vi;
+                        const T ph = fma(e.u, l[j], fma(e.v, m[j], fma(e.w, n[j], e.wo)));
+                        const float r = fract_rev(ph);
+                        acc[j] = fmaf(e.vr, __builtin_amdgcn_cosf(r), acc[j]);
+                        acc[j] = fmaf(-e.vi, __builtin_amdgcn_sinf(r), acc[j]);
                     }
-                }
-#pragma unroll
-                for (int j = 0; j < kPixPerLane; ++j) {
-                    const T ph = fma(e.u, l[j], fma(e.v, m[j], fma(e.w, n[j], e.wo)));
-                    const float r = fract_rev(ph);
-                    const float sn = __builtin_amdgcn_sinf(r);
-                    const float cs = __builtin_amdgcn_cosf(r);
-                    // Re(V exp(+i theta)) = Vr cos - Vi sin
-                    acc[j] = fmaf(vr[j], cs, acc[j]);
-                    acc[j] = fmaf(-vi[j], sn, acc[j]);
                 }
             }
             __syncthreads();

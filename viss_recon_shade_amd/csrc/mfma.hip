@@ -284,7 +284,7 @@ __global__ void __launch_bounds__(kThreads, 2) mfma_visgen_kernel(
     __shared__ double4 sP[2][kStage];             // [buffer] pixel (l, m, n) FP64
     __shared__ CapRec sCap[kWaves * kNT][2];      // caps of the block's visibilities
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
     const int col = lane & 31;                 // this lane's visibility within the wave tile
     const int kg = lane >> 5;                  // which 8 of the 16 K values this lane supplies
     const int64_t n_vtiles = (g.n_vis + kWaves * kNT - 1) / (kWaves * kNT);
@@ -476,7 +476,7 @@ __global__ void __launch_bounds__(kThreads, 2) mfma_recon_kernel(
     __shared__ _Float16 sA[2][4][kMaps][kStage];  // [buffer][re hi, re lo, im hi, im lo][map][vis slot]
     __shared__ VisS sV[2][kStage];
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
     const int col = lane & 31;
     const int kg = lane >> 5;
     const int64_t n_ptiles = g.n_pad / (kWaves * kNT);  // n_pad is a multiple of 128
