@@ -26,6 +26,12 @@ sys.path.insert(0, ROOT)
 PEAK_FP32_TFLOPS = 157.3          # MI355X published FP32 vector/matrix peak
 TRANS_PEAK = 256 * 2.4e9 * 16     # v_sin/v_cos issue: 16 lanes/clk/CU (quarter rate)
 FLOPS_PER_TERM = 10               # 3 FMA phase + 2 FMA accumulate
+N_CU, N_XCD = 256, 8              # SQ counters sum over CUs, GRBM_GUI_ACTIVE over XCDs
+
+
+def visgen_name(f64, vt):
+    """Kernel name of a vis-gen tile configuration."""
+    return f"visgen_kernel<{'true' if f64 else 'false'}, {vt}, false>"
 
 
 def parse():
@@ -227,14 +233,26 @@ def main():
 
     value = terms / (ms_plain * 1e-3) / 1e9
     traffic = None
+    issue = None
+    dom = max(("visgen", "recon"), key=lambda n: kt_plain[n])
+    dom_ms = kt_plain[dom]
+    kname = {"visgen": visgen_name(False, tile_plain[0]), "recon": "recon_kernel<false>"}[dom]
     tfile = os.path.join(ROOT, "profiles", "traffic_c2.json")
     if wl.name == "c2" and os.path.exists(tfile):
         kern = json.load(open(tfile))["kernels"]
-        key = {"visgen": f"visgen_kernel<false, {tile_plain[0]}>", "recon": "recon_kernel<false>"}
-    dom = max(("visgen", "recon"), key=lambda n: kt_plain[n])
-    dom_ms = kt_plain[dom]
-    if wl.name == "c2" and os.path.exists(tfile) and key[dom] in kern:
-        traffic = kern[key[dom]]["traffic_bytes"]
+        if kname in kern:
+            traffic = kern[kname]["traffic_bytes"]
+    mfile = os.path.join(ROOT, "profiles", "instmix_c2.json")
+    if wl.name == "c2" and os.path.exists(mfile):
+        mix = json.load(open(mfile)).get(kname)
+        if mix and "SQ_ACTIVE_INST_VALU" in mix and "GRBM_GUI_ACTIVE" in mix:
+            per_term = 64.0 / wl.terms_per_direction
+            issue = {"valu_busy": round(mix["SQ_ACTIVE_INST_VALU"] / N_CU
+                                        / (mix["GRBM_GUI_ACTIVE"] / N_XCD), 3),
+                     "valu_insts_per_term": round(mix["SQ_INSTS_VALU"] * per_term, 2),
+                     "trans_insts_per_term": round(mix["SQ_INSTS_VALU_TRANS_F32"] * per_term, 2),
+                     "source": "profiles/instmix_c2.json (rocprofv3 SQ_ACTIVE_INST_VALU, "
+                               "GRBM_GUI_ACTIVE, SQ_INSTS_VALU[_TRANS_F32])"}
     achieved = FLOPS_PER_TERM * wl.terms_per_direction / (dom_ms * 1e-3) / 1e12
     out = {
         "metric": "Gterms/s (vis x pix) vis-gen + recon",
@@ -260,7 +278,7 @@ def main():
                      "algorithmic_bytes": 33_000_000 if dom == "visgen" else 30_000_000,
                      "flops_per_term": FLOPS_PER_TERM,
                      "trans_frac": round(2 * wl.terms_per_direction / (dom_ms * 1e-3) / TRANS_PEAK, 4),
-                     "kernel_ms": round(dom_ms, 3)},
+                     "kernel_ms": round(dom_ms, 3), "kernel_name": kname, "issue": issue},
         "kernels_ms": {k: round(v, 3) for k, v in kt_plain.items()},
         "visgen_tile": {"plain": tile_plain[0], "mixed": tile_mixed[0]},
         "mixed_f64phase": {"value": round(terms / (ms_mixed * 1e-3) / 1e9, 2),
