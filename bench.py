@@ -34,6 +34,16 @@ def visgen_name(f64, vt):
     return f"visgen_kernel<{'true' if f64 else 'false'}, {vt}, false>"
 
 
+def algorithmic_bytes(wl, kernel):
+    """Minimum bytes one plain-FP32 launch moves: uvw (FP64, 24 B) and the
+    satellites' caps (32 B per satellite and step) once, V (8 B) once, the sky
+    (l, m, n FP32 + I for vis-gen; l, m, n + F + 8 partial images for recon)."""
+    base = wl.n_vis * 24 + wl.n_sat * wl.n_t * 32 + wl.n_vis * 8
+    if kernel == "visgen":
+        return base + wl.n_pix * 16
+    return base + wl.n_pix * (12 + 4 + 8 * 4)
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -232,6 +242,16 @@ def main():
     Vm, Fm = state["V"].cpu().numpy(), state["F"].cpu().numpy()
 
     value = terms / (ms_plain * 1e-3) / 1e9
+    # the same step with host (numpy) buffers at the boundary: PCIe staging included
+    def host_step():
+        g.MIncline(export=False)
+        Vh = g.visscal(I, kp)
+        g.imagerecon(Vh, None, kp)
+    host_step()
+    t0 = time.perf_counter()
+    for _ in range(2):
+        host_step()
+    ms_host = (time.perf_counter() - t0) / 2 * 1e3
     traffic = None
     issue = None
     dom = max(("visgen", "recon"), key=lambda n: kt_plain[n])
@@ -275,12 +295,14 @@ def main():
                      "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": traffic,
                      "traffic_source": "profiles/traffic_c2.json (rocprofv3 FETCH_SIZE/WRITE_SIZE, "
                                        "copy-calibrated)" if traffic else None,
-                     "algorithmic_bytes": 33_000_000 if dom == "visgen" else 30_000_000,
+                     "algorithmic_bytes": algorithmic_bytes(wl, dom),
                      "flops_per_term": FLOPS_PER_TERM,
                      "trans_frac": round(2 * wl.terms_per_direction / (dom_ms * 1e-3) / TRANS_PEAK, 4),
                      "kernel_ms": round(dom_ms, 3), "kernel_name": kname, "issue": issue},
         "kernels_ms": {k: round(v, 3) for k, v in kt_plain.items()},
         "visgen_tile": {"plain": tile_plain[0], "mixed": tile_mixed[0]},
+        "host_buffers": {"value": round(terms / (ms_host * 1e-3) / 1e9, 2), "ms_per_step": round(ms_host, 3),
+                         "note": "same step with numpy I, V, F at the C-ABI (staged over PCIe)"},
         "mixed_f64phase": {"value": round(terms / (ms_mixed * 1e-3) / 1e9, 2),
                            "ms_per_step": round(ms_mixed, 3),
                            "kernels_ms": {k: round(v, 3) for k, v in kt_mixed.items()}},
