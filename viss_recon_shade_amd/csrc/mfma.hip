@@ -324,6 +324,7 @@ __global__ void __launch_bounds__(kThreads, 2) mfma_visgen_kernel(
             for (int c = 0; c < 2; ++c) acc[a][c] = floatx16{};
         float gre = 0.0f, gim = 0.0f;  // sum of E over shaded pixels
         int cls = kOut;
+        bool any_shade = false;
 
         const _Float16 *Ag = A16 + grp * 2 * kMaps * g.n_pad;
         // Both buffers are free here: every wave passed the last barrier of
@@ -338,7 +339,10 @@ __global__ void __launch_bounds__(kThreads, 2) mfma_visgen_kernel(
             const bool new_chunk = (p0 % kChunk) == 0;
             ChunkRec ch;
             if (new_chunk) ch = g.chunk[p0 / kChunk];
-            if (new_chunk) cls = classify_m(ch, sCap[vc][0], sCap[vc][1]);
+            if (new_chunk) {
+                cls = classify_m(ch, sCap[vc][0], sCap[vc][1]);
+                any_shade = __ballot(cls != kOut) != 0;
+            }
             // the other buffer was last read before the previous barrier
             if (st + 1 < n_stages)
                 stage_load_lds<2>(Ag, g.n_pad, g.pix64, st + 1, &sA[buf ^ 1][0][0][0],
@@ -357,17 +361,22 @@ __global__ void __launch_bounds__(kThreads, 2) mfma_visgen_kernel(
                     cs[t] = __builtin_amdgcn_cosf(r);
                     sn[t] = __builtin_amdgcn_sinf(r);
                 }
-                uint32_t shm = (cls == kIn) ? 0xffu : 0u;
-                if (cls == kStraddle)
-                    shm = straddle_mask8(g, &sP[buf][kb], sCap[vc][0], sCap[vc][1], ia, ib);
                 H8 bre_h, bre_l, bim_h, bim_l;
+                if (any_shade) {  // wave-uniform: some lane's pair is not wholly clear
+                    uint32_t shm = (cls == kIn) ? 0xffu : 0u;
+                    if (cls == kStraddle)
+                        shm = straddle_mask8(g, &sP[buf][kb], sCap[vc][0], sCap[vc][1], ia, ib);
 #pragma unroll
-                for (int t = 0; t < 8; ++t) {
-                    const bool sh = (shm >> t) & 1u;
-                    gre += sh ? cs[t] : 0.0f;
-                    gim += sh ? sn[t] : 0.0f;
-                    cs[t] = sh ? 0.0f : cs[t];
-                    sn[t] = sh ? 0.0f : -sn[t];   // E = cos - i sin
+                    for (int t = 0; t < 8; ++t) {
+                        const bool sh = (shm >> t) & 1u;
+                        gre += sh ? cs[t] : 0.0f;
+                        gim += sh ? sn[t] : 0.0f;
+                        cs[t] = sh ? 0.0f : cs[t];
+                        sn[t] = sh ? 0.0f : -sn[t];   // E = cos - i sin
+                    }
+                } else {
+#pragma unroll
+                    for (int t = 0; t < 8; ++t) sn[t] = -sn[t];
                 }
 #pragma unroll
                 for (int t2 = 0; t2 < 4; ++t2) {
@@ -538,18 +547,24 @@ __global__ void __launch_bounds__(kThreads, 2) mfma_recon_kernel(
                     shm |= (uint32_t)(q.cls == kIn) << t;
                     stm |= (uint32_t)(q.cls == kStraddle) << t;
                 }
-                while (stm) {
-                    const int t = __ffs(stm) - 1;
-                    stm &= stm - 1;
-                    const VisS &q = sV[buf][kb + t];
-                    if (pair_shaded(g, s, cap_full(q.a), cap_full(q.b), q.ia, q.ib)) shm |= 1u << t;
-                }
                 H8 bre_h, bre_l, bim_h, bim_l;
+                if (__builtin_amdgcn_readfirstlane(shm | stm)) {  // the classes are wave-uniform
+                    while (stm) {
+                        const int t = __ffs(stm) - 1;
+                        stm &= stm - 1;
+                        const VisS &q = sV[buf][kb + t];
+                        if (pair_shaded(g, s, cap_full(q.a), cap_full(q.b), q.ia, q.ib))
+                            shm |= 1u << t;
+                    }
 #pragma unroll
-                for (int t = 0; t < 8; ++t) {
-                    const bool sh = (shm >> t) & 1u;
-                    cs[t] = sh ? 0.0f : cs[t];
-                    sn[t] = sh ? 0.0f : -sn[t];   // Re(V e^{+i theta}) = Vr cos - Vi sin
+                    for (int t = 0; t < 8; ++t) {
+                        const bool sh = (shm >> t) & 1u;
+                        cs[t] = sh ? 0.0f : cs[t];
+                        sn[t] = sh ? 0.0f : -sn[t];   // Re(V e^{+i theta}) = Vr cos - Vi sin
+                    }
+                } else {
+#pragma unroll
+                    for (int t = 0; t < 8; ++t) sn[t] = -sn[t];
                 }
 #pragma unroll
                 for (int t2 = 0; t2 < 4; ++t2) {
