@@ -37,12 +37,18 @@ def fixture(name, fine_factor=2):
     count, hsh, near = oracle.mask_stats(pos, pairs, lmn, W.R_MOON_KM)
     V = oracle.visgen(pos, pairs, uvw, lmn, I, W.R_MOON_KM, s=S, wfac=WFAC, t_occ=T_OCC)[0]
     F = oracle.recon(pos, pairs, uvw, lmn, V[None], W.R_MOON_KM, s=S, wfac=WFAC)[0]
+    # adjoint (gradient of sum |V(I) - V_obs|^2) for a fixed residual, canonical knobs
+    rng = np.random.default_rng(17)
+    resid = (rng.standard_normal(len(V)) + 1j * rng.standard_normal(len(V))).astype(np.complex64)
+    grad = oracle.recon(pos, pairs, uvw, lmn, resid.astype(np.complex128)[None], W.R_MOON_KM,
+                        kappa=2.0)[0]
     cl, ff, nx = W.fine_sky(wl.nside, factor=fine_factor, seed=21)
     Vf = oracle.visgen_fine(pos, pairs, uvw, lmn, cl, ff, nx, W.R_MOON_KM, s=S, wfac=WFAC,
                             t_occ=T_OCC)
     return dict(n_sat=wl.n_sat, n_t=wl.n_t, freq_hz=wl.freq_hz, nside=wl.nside, raan=raan,
                 m0=m0, fine_factor=fine_factor, I=I, pos=pos, pairs=pairs, uvw=uvw,
-                mask_count=count, mask_hash=hsh, near_limb=near, V=V, F=F, V_fine=Vf)
+                mask_count=count, mask_hash=hsh, near_limb=near, V=V, F=F, V_fine=Vf,
+                resid=resid, grad=grad)
 
 
 def main():

@@ -1,7 +1,8 @@
 """Committed oracle fixtures (tests/golden/oracle_*.npz, written by
 tests/golden/make_fixtures.py): the oracle must reproduce them (CPU), and the
 GPU path must match them through the C-ABI -- shade counts and hashes bit for
-bit, V and F within the mixed-precision tolerance (1e-5 relative L2)."""
+bit, V, F, the fine-sky V and the adjoint gradient within the mixed-precision
+tolerance (1e-5 relative L2)."""
 import os
 
 import numpy as np
@@ -33,7 +34,8 @@ def test_oracle_reproduces_fixture(oracle, name):
     fx, now = load(name), mk.fixture(name)
     for key in ("pairs", "mask_count", "mask_hash", "near_limb", "I", "raan", "m0"):
         assert np.array_equal(np.asarray(now[key]), fx[key]), key
-    for key in ("pos", "uvw", "V", "F", "V_fine"):
+    assert np.array_equal(now["resid"], fx["resid"])
+    for key in ("pos", "uvw", "V", "F", "V_fine", "grad"):
         scale = np.abs(fx[key]).max()
         assert np.abs(np.asarray(now[key]) - fx[key]).max() <= 1e-12 * scale, key
 
@@ -58,6 +60,10 @@ def test_gpu_matches_fixture(ctx, name):
     Vin = np.stack([fx["V"].real, fx["V"].imag], -1).astype(np.float32)[None]
     F = g.imagerecon(Vin, None, k)[0]
     assert rel(F, fx["F"]) <= TOL_MIXED
+    # adjoint: FP64 gradient of the committed residual (canonical knobs, kappa = 2)
+    r32 = np.stack([fx["resid"].real, fx["resid"].imag], -1).astype(np.float32)
+    G = g.adjoint(r32, vrs.knobs(precision="mixed"))
+    assert rel(G, fx["grad"]) <= TOL_MIXED
     cl, ff, nx = W.fine_sky(int(fx["nside"]), factor=int(fx["fine_factor"]), seed=21)
     g.fine_sky(cl, ff, nx)
     Vf = g.visscal_fine(k)
