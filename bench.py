@@ -189,17 +189,44 @@ def batch_leg(torch, ctx, vrs, W, args):
     return out
 
 
-def accuracy(g, wl, pos, uvw, lmn, I, V, F):
+def _errs(x, ref):
+    d = np.abs(np.asarray(x) - ref)
+    scale = np.abs(ref).max()
+    return {"rel_l2": float(np.linalg.norm(d) / np.linalg.norm(ref)),
+            "max_abs_over_inf": float(d.max() / scale),
+            "p50_abs_over_inf": float(np.quantile(d, 0.5) / scale),
+            "p99_abs_over_inf": float(np.quantile(d, 0.99) / scale)}
+
+
+def accuracy(g, wl, pos, uvw, lmn, I, V, F, n_vis_sample=4096, n_pix_sample=256):
+    """Errors against the FP64 oracle on deterministic subsamples: evenly spaced
+    visibilities x all pixels (V), evenly spaced pixels x all visibilities (F)."""
     import oracle as o
-    sel = np.linspace(0, wl.n_vis - 1, 32).astype(np.int64)
+    sel = np.linspace(0, wl.n_vis - 1, n_vis_sample).astype(np.int64)
     Vo = o.visgen(pos, g.pairs, uvw, lmn, I, g.r_moon_km, vis_sel=sel)[0]
     Vc = V[0, sel, 0].astype(np.float64) + 1j * V[0, sel, 1]
-    psel = np.linspace(0, wl.n_pix - 1, 8).astype(np.int64)
+    psel = np.linspace(0, wl.n_pix - 1, n_pix_sample).astype(np.int64)
     Vfull = V[0, :, 0].astype(np.float64) + 1j * V[0, :, 1]
     Fo = o.recon(pos, g.pairs, uvw, lmn, Vfull, g.r_moon_km, pix_sel=psel)[0]
-    return {"V_rel_l2": float(np.linalg.norm(Vc - Vo) / np.linalg.norm(Vo)),
-            "F_rel_l2": float(np.linalg.norm(F[0, psel] - Fo) / np.linalg.norm(Fo)),
-            "sample": "32 visibilities x all pixels; 8 pixels x all visibilities"}
+    ev, ef = _errs(Vc, Vo), _errs(F[0, psel], Fo)
+    return {"V_rel_l2": ev["rel_l2"], "F_rel_l2": ef["rel_l2"], "V": ev, "F": ef,
+            "sample": f"{n_vis_sample} visibilities x all pixels; {n_pix_sample} pixels x all "
+                      f"visibilities (F from the GPU's own V)"}
+
+
+def mask_check(g, wl, pos, lmn, stride=32):
+    """Shade counts and XOR hashes of every stride-th visibility against the oracle."""
+    import oracle as o
+    st = g.ShadeMatrix()
+    sel = np.arange(0, wl.n_vis, stride, dtype=np.int64)
+    cnt, h, near = o.mask_stats(pos, g.pairs, lmn, g.r_moon_km, vis_sel=sel)
+    return {"visibilities_compared": int(len(sel)),
+            "pixel_pairs_compared": int(len(sel)) * wl.n_pix,
+            "count_mismatches": int((st["count"][sel] != cnt).sum()),
+            "hash_mismatches": int((st["hash"][sel] != h).sum()),
+            "near_limb_pairs_sample": int(near),
+            "near_limb_pairs_all": int(st["near_limb"]),
+            "fp64_tests_all": int(st["fp64_tests"])}
 
 
 def main():
@@ -313,7 +340,8 @@ def main():
         out["batch_c4"] = batch_leg(torch, ctx, vrs, W, args)
     if not args.no_accuracy:
         out["accuracy"] = {"plain": accuracy(g, wl, pos, uvw, lmn, I, Vp, Fp),
-                           "mixed": accuracy(g, wl, pos, uvw, lmn, I, Vm, Fm)}
+                           "mixed": accuracy(g, wl, pos, uvw, lmn, I, Vm, Fm),
+                           "mask": mask_check(g, wl, pos, lmn)}
     if not args.no_cpu:
         Vc = Vp[0, :, 0].astype(np.float64) + 1j * Vp[0, :, 1]
         out["cpu_baseline"] = cpu_baseline(g, wl, pos, uvw, lmn, I[0], Vc, args.cpu_stride)
