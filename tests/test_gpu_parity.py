@@ -217,3 +217,26 @@ def test_c2_full_size_sampled_parity(ctx, oracle):
     ssel = np.arange(0, wl.n_vis, 4099, dtype=np.int64)
     cnt, h, _ = oracle.mask_stats(pos, g.pairs, lmn, g.r_moon_km, vis_sel=ssel)
     assert np.array_equal(st["count"][ssel], cnt) and np.array_equal(st["hash"][ssel], h)
+
+
+def test_c2_full_size_linearity_and_adjoint(ctx):
+    """Size-independent properties on every c2 output: with T_occ = 0 vis-gen is
+    linear in I (dyadic intensities keep 2 I1 - 3 I2 exact in FP32), and the
+    adjoint satisfies <A I, y> = <I, A^H y> with A^H = vrs_adjoint / 2."""
+    wl, g, pos, uvw, lmn, I = geom(ctx, "c2")
+    rng = np.random.default_rng(31)
+    I1 = (rng.integers(0, 1024, wl.n_pix) / 256).astype(np.float32)
+    I2 = (rng.integers(0, 1024, wl.n_pix) / 256).astype(np.float32)
+    k = vrs.knobs(precision="mixed", t_occ=0.0)
+    V = cplx(g.visscal(np.stack([I1, I2, 2 * I1 - 3 * I2]), k))
+    e_lin = rel(V[2], 2 * V[0] - 3 * V[1])
+    print(f"[c2 linearity] rel {e_lin:.2e}", flush=True)
+    assert e_lin <= 1e-5
+    y = (rng.standard_normal(wl.n_vis) + 1j * rng.standard_normal(wl.n_vis)).astype(np.complex64)
+    y2 = np.stack([y.real, y.imag], -1).astype(np.float32)
+    AhY = g.adjoint(y2, k).astype(np.float64) / 2.0   # gradient = 2 Re(A^H r)
+    lhs = np.real(np.vdot(y.astype(np.complex128), V[0]))
+    rhs = float(np.dot(I1.astype(np.float64), AhY))
+    e_adj = abs(lhs - rhs) / abs(lhs)
+    print(f"[c2 adjoint identity] rel {e_adj:.2e}", flush=True)
+    assert e_adj <= 1e-5
