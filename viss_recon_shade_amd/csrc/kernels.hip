@@ -577,11 +577,17 @@ template <bool F64> struct RecHot {
     float vr, vi;
     int32_t cls, pad;
 };
-// ... and what it reads only when the chunk straddles a limb.
-struct RecCold {
-    CapRec a, b;
-    int32_t ia, ib, pad0, pad1;
+// ... and what it reads only when the chunk straddles a limb: the fields of
+// the two caps a pixel test reads (not CapRec itself: its 32-byte alignment
+// would take LDS that the fifth block per CU needs).
+struct CapTest {
+    float dx, dy, dz, c, g;
 };
+struct RecCold {
+    CapTest a, b;
+    int32_t ia, ib;
+};
+static_assert(sizeof(RecCold) == 48, "recon cold record");
 
 // Work item = (slot, chunk); a slot is a visibility range of one map.  In
 // geometry mode slot = map * n_seg + seg over equal segments; in list mode
@@ -683,8 +689,8 @@ __global__ void __launch_bounds__(kBlock) recon_kernel(GeomDev g, PhaseKnobs k, 
                 e.vi = q.y * sc;
                 e.cls = cls;
                 RecCold &o = cold[pos];
-                o.a = A;
-                o.b = B;
+                o.a = {A.dx, A.dy, A.dz, A.c, A.g};
+                o.b = {B.dx, B.dy, B.dz, B.c, B.g};
                 o.ia = ia;
                 o.ib = ib;
             }
@@ -695,7 +701,9 @@ __global__ void __launch_bounds__(kBlock) recon_kernel(GeomDev g, PhaseKnobs k, 
                 const RecHot<F64> e = hot[q];
                 if (__builtin_amdgcn_readfirstlane(e.cls) == kStraddle) {
                     // the pair straddles the chunk: test each pixel
-                    const CapRec A2 = cold[q].a, B2 = cold[q].b;
+                    const CapTest ta = cold[q].a, tb = cold[q].b;
+                    const CapRec A2 = {ta.dx, ta.dy, ta.dz, ta.c, 0.f, ta.g, 0.f, 0.f};
+                    const CapRec B2 = {tb.dx, tb.dy, tb.dz, tb.c, 0.f, tb.g, 0.f, 0.f};
 #pragma unroll
                     for (int j = 0; j < kPixPerLane; ++j) {
                         bool unc = false;
