@@ -595,11 +595,10 @@ static_assert(sizeof(RecCold) == 48, "recon cold record");
 // (one per thread), classifies them against the block's chunk once, and
 // compacts away those that shade the whole chunk.
 template <bool F64>
-__global__ void __launch_bounds__(kBlock) recon_kernel(GeomDev g, PhaseKnobs k, int32_t n_maps,
-                                                       const float2 *__restrict__ V,
-                                                       const float *__restrict__ wgt,
-                                                       float kappa, int32_t n_seg,
-                                                       float *__restrict__ part) {
+__device__ __forceinline__ void recon_body(const GeomDev &g, const PhaseKnobs &k, int32_t n_maps,
+                                           const float2 *__restrict__ V,
+                                           const float *__restrict__ wgt, float kappa,
+                                           int32_t n_seg, float *__restrict__ part) {
     using T = typename PhaseT<F64>::T;
     __shared__ RecHot<F64> hot[kReconStage];
     __shared__ RecCold cold[kReconStage];
@@ -747,6 +746,27 @@ __global__ void __launch_bounds__(kBlock) recon_kernel(GeomDev g, PhaseKnobs k, 
     }
 }
 static_assert(kBlock == kChunk, "recon reduction assumes one thread per chunk pixel");
+
+// The FP32 recon runs five blocks per CU at its natural 86 registers; the
+// FP64-phase one is held to five waves per SIMD too (a few spilled registers
+// cost less than a fifth of the waves).
+template <bool F64>
+__global__ void __launch_bounds__(kBlock) recon_kernel(GeomDev g, PhaseKnobs k, int32_t n_maps,
+                                                       const float2 *__restrict__ V,
+                                                       const float *__restrict__ wgt,
+                                                       float kappa, int32_t n_seg,
+                                                       float *__restrict__ part) {
+    recon_body<F64>(g, k, n_maps, V, wgt, kappa, n_seg, part);
+}
+template <>
+__global__ void __launch_bounds__(kBlock, 5) recon_kernel<true>(GeomDev g, PhaseKnobs k,
+                                                                int32_t n_maps,
+                                                                const float2 *__restrict__ V,
+                                                                const float *__restrict__ wgt,
+                                                                float kappa, int32_t n_seg,
+                                                                float *__restrict__ part) {
+    recon_body<true>(g, k, n_maps, V, wgt, kappa, n_seg, part);
+}
 
 // F[map][ring[p]] = sum over the map's slots of part[slot][p].
 __global__ void __launch_bounds__(256) recon_finalize_kernel(GeomDev g, int32_t n_maps,
