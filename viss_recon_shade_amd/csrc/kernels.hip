@@ -349,10 +349,11 @@ __device__ __forceinline__ void fill_vis_ent(const GeomDev &g, const PhaseKnobs 
     }
 }
 
-// Four waves per SIMD for every variant: the FP64-phase and 32-visibility
-// tiles then spill a few registers, which costs less than a quarter of the waves.
+// Four waves per SIMD: the FP64-phase tiles then spill a few registers, which
+// costs less than a quarter of the waves.  The FP32 32-visibility tile keeps
+// its registers (three waves) rather than spill at the same speed.
 template <bool F64, int VT, bool FINE = false>
-__global__ void __launch_bounds__(kBlock, 4) visgen_kernel(GeomDev g, PhaseKnobs k, int32_t n_maps,
+__global__ void __launch_bounds__(kBlock, (!F64 && VT == 32) ? 3 : 4) visgen_kernel(GeomDev g, PhaseKnobs k, int32_t n_maps,
                                                         const float *__restrict__ Ich,
                                                         float2 *__restrict__ V) {
     using T = typename PhaseT<F64>::T;
