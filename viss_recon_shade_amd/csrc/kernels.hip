@@ -52,6 +52,16 @@ __device__ __forceinline__ int classify_cap(const ChunkRec &ch, const CapRec &a)
     return kStraddle;
 }
 
+// The same test as two flags (no class encoding): in = classify_cap == kIn,
+// out = classify_cap == kOut; `always` = the chunk's always-straddle flag.
+__device__ __forceinline__ void cap_in_out(const ChunkRec &ch, bool always, const CapRec &a,
+                                           bool &in, bool &out) {
+    const float x = fmaf(ch.gz, a.dz, fmaf(ch.gy, a.dy, ch.gx * a.dx));
+    const float mg = a.g + kChunkMargin;
+    in = !always && (x >= fmaf(a.c, ch.crho, a.sa * ch.srho) + mg) && (a.c < ch.crho);
+    out = !always && !in && (x < fmaf(a.c, ch.crho, -a.sa * ch.srho) - mg);
+}
+
 // Classify a chunk against the pair: shaded by either satellite, by neither, or mixed.
 __device__ __forceinline__ int classify(const ChunkRec &ch, const CapRec &a, const CapRec &b) {
     const int ca = classify_cap(ch, a), cb = classify_cap(ch, b);
@@ -408,12 +418,16 @@ __global__ void __launch_bounds__(kBlock, (!F64 && VT == 32) ? 3 : 4) visgen_ker
                 uint32_t in_m, out_m, sa_m, sb_m;
                 {
                     const int q = lane & (VT - 1);
-                    const int ca = classify_cap(ch, ents[q].a), cb = classify_cap(ch, ents[q].b);
-                    const bool in = (ca == kIn) || (cb == kIn);
+                    const bool always = (__builtin_amdgcn_readfirstlane(ch.flags) &
+                                         kChunkAlwaysStraddle) != 0;
+                    bool ai, ao, bi, bo;
+                    cap_in_out(ch, always, ents[q].a, ai, ao);
+                    cap_in_out(ch, always, ents[q].b, bi, bo);
+                    const bool in = ai || bi;
                     in_m = (uint32_t)__ballot(in) & kTileMask;
-                    out_m = (uint32_t)__ballot(ca == kOut && cb == kOut) & kTileMask;
-                    sa_m = (uint32_t)__ballot(!in && ca == kStraddle) & kTileMask;
-                    sb_m = (uint32_t)__ballot(!in && cb == kStraddle) & kTileMask;
+                    out_m = (uint32_t)__ballot(ao && bo) & kTileMask;
+                    sa_m = (uint32_t)__ballot(!in && !ao) & kTileMask;
+                    sb_m = (uint32_t)__ballot(!in && !bo) & kTileMask;
                 }
                 uint32_t unc_m = 0;  // straddling pairs with a pixel inside the FP32 guard band
 #pragma unroll
