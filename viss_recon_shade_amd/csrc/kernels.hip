@@ -417,17 +417,20 @@ __global__ void __launch_bounds__(kBlock, (!F64 && VT == 32) ? 3 : 4) visgen_ker
                 // sa_m / sb_m: pairs whose cap A / B must be tested pixel by pixel.
                 uint32_t in_m, out_m, sa_m, sb_m;
                 {
-                    const int q = lane & (VT - 1);
-                    const bool always = (__builtin_amdgcn_readfirstlane(ch.flags) &
-                                         kChunkAlwaysStraddle) != 0;
-                    bool ai, ao, bi, bo;
-                    cap_in_out(ch, always, ents[q].a, ai, ao);
-                    cap_in_out(ch, always, ents[q].b, bi, bo);
-                    const bool in = ai || bi;
-                    in_m = (uint32_t)__ballot(in) & kTileMask;
-                    out_m = (uint32_t)__ballot(ao && bo) & kTileMask;
-                    sa_m = (uint32_t)__ballot(!in && !ao) & kTileMask;
-                    sb_m = (uint32_t)__ballot(!in && !bo) & kTileMask;
+                    // lane q + VT * s tests cap s (0 = A, 1 = B) of pair q
+                    const int q = lane & (VT - 1), s = (lane / VT) & 1;
+                    bool cin, cout;
+                    cap_in_out(ch, (ch.flags & kChunkAlwaysStraddle) != 0, (&ents[q].a)[s], cin,
+                               cout);
+                    const uint64_t bin = __ballot(cin), bout = __ballot(cout);
+                    const uint32_t ain = (uint32_t)bin & kTileMask;
+                    const uint32_t bin_ = (uint32_t)(bin >> VT) & kTileMask;
+                    const uint32_t aout = (uint32_t)bout & kTileMask;
+                    const uint32_t bout_ = (uint32_t)(bout >> VT) & kTileMask;
+                    in_m = ain | bin_;
+                    out_m = aout & bout_;
+                    sa_m = ~(in_m | aout) & kTileMask;
+                    sb_m = ~(in_m | bout_) & kTileMask;
                 }
                 uint32_t unc_m = 0;  // straddling pairs with a pixel inside the FP32 guard band
 #pragma unroll
