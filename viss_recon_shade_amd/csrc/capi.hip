@@ -62,7 +62,7 @@ struct vrs_geom {
     double4 *d_pos = nullptr;
     CapRec *d_cap = nullptr;
     double4 *d_uvw64 = nullptr;
-    float4 *d_uvw32 = nullptr;
+    int2 *d_viscap = nullptr;
     int2 *d_pairs = nullptr;
     float4 *d_pix32 = nullptr;
     double4 *d_pix64 = nullptr;
@@ -88,7 +88,7 @@ struct vrs_geom {
         GeomDev g = {};
         g.n_sat = n_sat; g.n_t = n_t; g.n_base = n_base; g.n_chunk = n_chunk;
         g.n_vis = n_vis; g.n_pad = n_pad;
-        g.pos = d_pos; g.cap = d_cap; g.uvw64 = d_uvw64; g.uvw32 = d_uvw32; g.pairs = d_pairs;
+        g.pos = d_pos; g.cap = d_cap; g.uvw64 = d_uvw64; g.vis_cap = d_viscap; g.pairs = d_pairs;
         g.pix32 = d_pix32; g.pix64 = d_pix64; g.ring = d_ring; g.chunk = d_chunk;
         const double e = kNearLimbRel * r_moon;
         g.limb.R2 = r_moon * r_moon;
@@ -429,7 +429,7 @@ void vrs_geom_destroy(vrs_geom *g) {
     if (!g) return;
     DeviceGuard dg(g->device);
     (void)hipDeviceSynchronize();
-    void *ptrs[] = {g->d_orbit, g->d_pos,  g->d_cap,  g->d_uvw64, g->d_uvw32, g->d_pairs,
+    void *ptrs[] = {g->d_orbit, g->d_pos,  g->d_cap,  g->d_uvw64, g->d_viscap, g->d_pairs,
                     g->d_pix32, g->d_pix64, g->d_ring, g->d_chunk};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -516,7 +516,7 @@ static vrs_status geom_build(vrs_ctx *ctx, const vrs_geom_desc *d, vrs_geom *g) 
     VRS_HIP(ctx, hipMalloc(&g->d_pos, sizeof(double4) * nst));
     VRS_HIP(ctx, hipMalloc(&g->d_cap, sizeof(CapRec) * nst));
     VRS_HIP(ctx, hipMalloc(&g->d_uvw64, sizeof(double4) * std::max<int64_t>(g->n_vis, 1)));
-    VRS_HIP(ctx, hipMalloc(&g->d_uvw32, sizeof(float4) * std::max<int64_t>(g->n_vis, 1)));
+    VRS_HIP(ctx, hipMalloc(&g->d_viscap, sizeof(int2) * std::max<int64_t>(g->n_vis, 1)));
     VRS_HIP(ctx, hipMalloc(&g->d_pairs, sizeof(int2) * g->n_base));
     VRS_HIP(ctx, hipMalloc(&g->d_pix32, sizeof(float4) * g->n_pad));
     VRS_HIP(ctx, hipMalloc(&g->d_pix64, sizeof(double4) * g->n_pad));
@@ -628,7 +628,7 @@ vrs_status vrs_orbit_uvw(vrs_ctx *ctx, vrs_geom *g, double *sat_pos, double *uvw
     DeviceGuard dg(ctx->device);
     {
         Timer tm(ctx, "orbit");
-        VRS_HIP(ctx, launch_orbit(g->orb, g->dev(), g->d_pos, g->d_cap, g->d_uvw64, g->d_uvw32,
+        VRS_HIP(ctx, launch_orbit(g->orb, g->dev(), g->d_pos, g->d_cap, g->d_uvw64, g->d_viscap,
                                   ctx->stream));
     }
     // strided exports (x, y, z) out of the padded 4-wide device records

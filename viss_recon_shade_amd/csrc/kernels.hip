@@ -170,7 +170,7 @@ __global__ void __launch_bounds__(256) orbit_kernel(OrbitArgs a, double4 *__rest
 
 __global__ void __launch_bounds__(256) uvw_kernel(GeomDev g, double inv_lambda_km,
                                                   double4 *__restrict__ uvw64,
-                                                  float4 *__restrict__ uvw32) {
+                                                  int2 *__restrict__ vis_cap) {
     const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= g.n_vis) return;
     const int b = (int)(v / g.n_t), t = (int)(v % g.n_t);
@@ -180,7 +180,7 @@ __global__ void __launch_bounds__(256) uvw_kernel(GeomDev g, double inv_lambda_k
     const double u = (rb.x - ra.x) * inv_lambda_km, vv = (rb.y - ra.y) * inv_lambda_km,
                  w = (rb.z - ra.z) * inv_lambda_km;
     uvw64[v] = make_double4(u, vv, w, 0.0);
-    uvw32[v] = make_float4((float)u, (float)vv, (float)w, 0.0f);
+    vis_cap[v] = make_int2(pr.x * g.n_t + t, pr.y * g.n_t + t);
 }
 
 // ------------------------------------------------------------------ stage 2
@@ -196,9 +196,8 @@ __global__ void __launch_bounds__(kBlock) shade_kernel(GeomDev g, int32_t *__res
     const int64_t nwaves = (int64_t)gridDim.x * kWavesPerBlock;
     unsigned long long t_shaded = 0, t_near = 0, t_f64 = 0;
     for (int64_t v = wave0; v < g.n_vis; v += nwaves) {
-        const int b = (int)(v / g.n_t), t = (int)(v % g.n_t);
-        const int2 pr = g.pairs[b];
-        const int ia = pr.x * g.n_t + t, ib = pr.y * g.n_t + t;
+        const int2 vc = g.vis_cap[v];
+        const int ia = vc.x, ib = vc.y;
         const CapRec A = g.cap[ia], B = g.cap[ib];
         int cnt = 0, near = 0, f64 = 0;
         uint32_t h = 0;
@@ -316,10 +315,9 @@ __device__ __forceinline__ bool vis_geometry(const GeomDev &g, int64_t v, int64_
         vv = (rb.y - ra.y) * il;
         w = (rb.z - ra.z) * il;
     } else {
-        const int b = (int)(v / g.n_t), t = (int)(v % g.n_t);
-        const int2 pr = g.pairs[b];
-        ia = pr.x * g.n_t + t;
-        ib = pr.y * g.n_t + t;
+        const int2 vc = g.vis_cap[v];
+        ia = vc.x;
+        ib = vc.y;
         const double4 q = g.uvw64[v];
         u = q.x; vv = q.y; w = q.z;
     }
@@ -342,7 +340,7 @@ __device__ __forceinline__ void fill_vis_ent(const GeomDev &g, const PhaseKnobs 
             e.v = (T)(vv * k.s);
             e.w = (T)(w * k.s);
         } else {
-            // the FP32 path rounds uvw to FP32 first, as the stored uvw32 array is
+            // the FP32 path rounds uvw to FP32 first
             const float s = (float)k.s;
             e.u = (T)((float)u * s);
             e.v = (T)((float)vv * s);
@@ -595,7 +593,7 @@ constexpr int kReconStage = kBlock;  // visibilities staged in LDS per step (one
 template <bool F64> struct RecHot {
     typename PhaseT<F64>::T u, v, w, wo;
     float vr, vi;
-    int32_t cls, pad;
+    int32_t pad0, pad1;
 };
 // ... and what it reads only when the chunk straddles a limb: the fields of
 // the two caps a pixel test reads (not CapRec itself: its 32-byte alignment
@@ -613,7 +611,8 @@ static_assert(sizeof(RecCold) == 48, "recon cold record");
 // geometry mode slot = map * n_seg + seg over equal segments; in list mode
 // the slots come from g.slots.  Each step stages kReconStage visibilities
 // (one per thread), classifies them against the block's chunk once, and
-// compacts away those that shade the whole chunk.
+// compacts away those that shade the whole chunk: wholly clear ones first,
+// then the straddling ones, so that each kind runs its own branch-free loop.
 template <bool F64>
 __device__ __forceinline__ void recon_body(const GeomDev &g, const PhaseKnobs &k, int32_t n_maps,
                                            const float2 *__restrict__ V,
@@ -622,7 +621,7 @@ __device__ __forceinline__ void recon_body(const GeomDev &g, const PhaseKnobs &k
     using T = typename PhaseT<F64>::T;
     __shared__ RecHot<F64> hot[kReconStage];
     __shared__ RecCold cold[kReconStage];
-    __shared__ int wave_live[kWavesPerBlock];
+    __shared__ int2 wave_cnt[kWavesPerBlock];  // (clear, straddling) staged per wave
     __shared__ float red[kWavesPerBlock][kChunk];
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
@@ -679,21 +678,28 @@ __device__ __forceinline__ void recon_body(const GeomDev &g, const PhaseKnobs &k
                 B = g.cap[ib];
                 cls = classify(ch, A, B);
             }
-            const bool live = cls != kIn;
-            const uint64_t bal = __ballot(live);
-            if (lane == 0) wave_live[wave] = __popcll(bal);
+            const bool clear = cls == kOut, strad = cls == kStraddle;
+            const uint64_t bc = __ballot(clear), bs = __ballot(strad);
+            if (lane == 0) wave_cnt[wave] = make_int2(__popcll(bc), __popcll(bs));
             __syncthreads();
-            int base = 0, n_live = 0;
+            int base_c = 0, base_s = 0, n_clear = 0, n_live = 0;
 #pragma unroll
             for (int q = 0; q < kWavesPerBlock; ++q) {
-                base += (q < wave) ? wave_live[q] : 0;
-                n_live += wave_live[q];
+                const int2 wc = wave_cnt[q];
+                base_c += (q < wave) ? wc.x : 0;
+                base_s += (q < wave) ? wc.y : 0;
+                n_clear += wc.x;
+                n_live += wc.x + wc.y;
             }
-            // both are wave-uniform; saying so keeps the pixel loop below a scalar loop
-            base = __builtin_amdgcn_readfirstlane(base);
+            // all wave-uniform; saying so keeps the pixel loops below scalar loops
+            base_c = __builtin_amdgcn_readfirstlane(base_c);
+            base_s = __builtin_amdgcn_readfirstlane(base_s);
+            n_clear = __builtin_amdgcn_readfirstlane(n_clear);
             n_live = __builtin_amdgcn_readfirstlane(n_live);
-            if (live) {
-                const int pos = base + __popcll(bal & ((1ull << lane) - 1ull));
+            if (clear || strad) {
+                const uint64_t below = (1ull << lane) - 1ull;
+                const int pos = clear ? base_c + __popcll(bc & below)
+                                      : n_clear + base_s + __popcll(bs & below);
                 RecHot<F64> &e = hot[pos];
                 if (F64) {
                     e.u = (T)(u * k.s); e.v = (T)(vv * k.s); e.w = (T)(w * k.s);
@@ -706,7 +712,6 @@ __device__ __forceinline__ void recon_body(const GeomDev &g, const PhaseKnobs &k
                 const float sc = kap * (wgt ? wgt[v] : 1.0f);
                 e.vr = q.x * sc;
                 e.vi = q.y * sc;
-                e.cls = cls;
                 RecCold &o = cold[pos];
                 o.a = {A.dx, A.dy, A.dz, A.c, A.g};
                 o.b = {B.dx, B.dy, B.dz, B.c, B.g};
@@ -714,42 +719,39 @@ __device__ __forceinline__ void recon_body(const GeomDev &g, const PhaseKnobs &k
                 o.ib = ib;
             }
             __syncthreads();
-            // ---- pixel loop over the live visibilities
-#pragma unroll 2
-            for (int q = wave; q < n_live; q += kWavesPerBlock) {
+            // ---- pixel loops: the wholly clear visibilities, then the straddling ones
+            for (int q = wave; q < n_clear; q += kWavesPerBlock) {
                 const RecHot<F64> e = hot[q];
-                if (__builtin_amdgcn_readfirstlane(e.cls) == kStraddle) {
-                    // the pair straddles the chunk: test each pixel
-                    const CapTest ta = cold[q].a, tb = cold[q].b;
-                    const CapRec A2 = {ta.dx, ta.dy, ta.dz, ta.c, 0.f, ta.g, 0.f, 0.f};
-                    const CapRec B2 = {tb.dx, tb.dy, tb.dz, tb.c, 0.f, tb.g, 0.f, 0.f};
 #pragma unroll
-                    for (int j = 0; j < kPixPerLane; ++j) {
-                        bool unc = false;
-                        bool sh = occ32(sx[j], sy[j], sz[j], A2, unc) |
-                                  occ32(sx[j], sy[j], sz[j], B2, unc);
-                        if (unc) {
-                            int nl;
-                            sh = occ_pair64(g.pos, g.pix64,
-                                            (int64_t)c * kChunk + j * kWave + lane, cold[q].ia,
-                                            cold[q].ib, g.limb, &nl);
-                        }
-                        const float vr = sh ? 0.0f : e.vr, vi = sh ? 0.0f : e.vi;
-                        const T ph = fma(e.u, l[j], fma(e.v, m[j], fma(e.w, n[j], e.wo)));
-                        const float r = fract_rev(ph);
-                        // Re(V exp(+i theta)) = Vr cos - Vi sin
-                        acc[j] = fmaf(vr, __builtin_amdgcn_cosf(r), acc[j]);
-                        acc[j] = fmaf(-vi, __builtin_amdgcn_sinf(r), acc[j]);
-                    }
-                } else {
-                    // wholly clear for the chunk (wholly shaded pairs were compacted away)
+                for (int j = 0; j < kPixPerLane; ++j) {
+                    const T ph = fma(e.u, l[j], fma(e.v, m[j], fma(e.w, n[j], e.wo)));
+                    const float r = fract_rev(ph);
+                    // Re(V exp(+i theta)) = Vr cos - Vi sin
+                    acc[j] = fmaf(e.vr, __builtin_amdgcn_cosf(r), acc[j]);
+                    acc[j] = fmaf(-e.vi, __builtin_amdgcn_sinf(r), acc[j]);
+                }
+            }
+            for (int q = n_clear + wave; q < n_live; q += kWavesPerBlock) {
+                // the pair straddles the chunk: test each pixel
+                const RecHot<F64> e = hot[q];
+                const CapTest ta = cold[q].a, tb = cold[q].b;
+                const CapRec A2 = {ta.dx, ta.dy, ta.dz, ta.c, 0.f, ta.g, 0.f, 0.f};
+                const CapRec B2 = {tb.dx, tb.dy, tb.dz, tb.c, 0.f, tb.g, 0.f, 0.f};
 #pragma unroll
-                    for (int j = 0; j < kPixPerLane; ++j) {
-                        const T ph = fma(e.u, l[j], fma(e.v, m[j], fma(e.w, n[j], e.wo)));
-                        const float r = fract_rev(ph);
-                        acc[j] = fmaf(e.vr, __builtin_amdgcn_cosf(r), acc[j]);
-                        acc[j] = fmaf(-e.vi, __builtin_amdgcn_sinf(r), acc[j]);
+                for (int j = 0; j < kPixPerLane; ++j) {
+                    bool unc = false;
+                    bool sh = occ32(sx[j], sy[j], sz[j], A2, unc) |
+                              occ32(sx[j], sy[j], sz[j], B2, unc);
+                    if (unc) {
+                        int nl;
+                        sh = occ_pair64(g.pos, g.pix64, (int64_t)c * kChunk + j * kWave + lane,
+                                        cold[q].ia, cold[q].ib, g.limb, &nl);
                     }
+                    const float vr = sh ? 0.0f : e.vr, vi = sh ? 0.0f : e.vi;
+                    const T ph = fma(e.u, l[j], fma(e.v, m[j], fma(e.w, n[j], e.wo)));
+                    const float r = fract_rev(ph);
+                    acc[j] = fmaf(vr, __builtin_amdgcn_cosf(r), acc[j]);
+                    acc[j] = fmaf(-vi, __builtin_amdgcn_sinf(r), acc[j]);
                 }
             }
             __syncthreads();
@@ -880,14 +882,14 @@ hipError_t launch_fine_pixels(const GeomDev &g, float *cbar, float2 *pixcap, Chu
 }
 
 hipError_t launch_orbit(const OrbitArgs &a, const GeomDev &g, double4 *pos, CapRec *cap,
-                        double4 *uvw64, float4 *uvw32, hipStream_t st) {
+                        double4 *uvw64, int2 *vis_cap, hipStream_t st) {
     const int64_t n = (int64_t)a.n_sat * a.n_t;
     hipLaunchKernelGGL(orbit_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, st, a, pos, cap);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     if (g.n_vis > 0)
         hipLaunchKernelGGL(uvw_kernel, dim3(blocks_for(g.n_vis, 256)), dim3(256), 0, st, g,
-                           a.inv_lambda_km, uvw64, uvw32);
+                           a.inv_lambda_km, uvw64, vis_cap);
     return hipGetLastError();
 }
 

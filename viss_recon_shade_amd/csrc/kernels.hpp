@@ -27,7 +27,7 @@ struct GeomDev {
     const double4 *pos;    // [n_sat * n_t]  (x, y, z, |r|^2)
     const CapRec *cap;     // [n_sat * n_t]
     const double4 *uvw64;  // [n_vis]
-    const float4 *uvw32;   // [n_vis]
+    const int2 *vis_cap;   // [n_vis] cap rows (sat_a * n_t + t, sat_b * n_t + t)
     const int2 *pairs;     // [n_base]
     const float4 *pix32;   // [n_pad]
     const double4 *pix64;  // [n_pad]
@@ -59,7 +59,7 @@ struct PhaseKnobs {
 
 // Stage 1: positions, caps, uvw.
 hipError_t launch_orbit(const OrbitArgs &a, const GeomDev &g, double4 *pos, CapRec *cap,
-                        double4 *uvw64, float4 *uvw32, hipStream_t st);
+                        double4 *uvw64, int2 *vis_cap, hipStream_t st);
 
 // Stage 2: per-visibility shade statistics.  bits may be null; tot = [shaded, near, fp64].
 hipError_t launch_shade(const GeomDev &g, int32_t *count, uint32_t *hash, uint32_t *bits,

@@ -302,10 +302,9 @@ __global__ void __launch_bounds__(kThreads, 2) mfma_visgen_kernel(
         int ia = 0, ib = 0;
         const int vc = wave * kNT + col;        // this lane's visibility within the block
         if (vok) {
-            const int b = (int)(v / g.n_t), t = (int)(v % g.n_t);
-            const int2 pr = g.pairs[b];
-            ia = pr.x * g.n_t + t;
-            ib = pr.y * g.n_t + t;
+            const int2 vcap = g.vis_cap[v];
+            ia = vcap.x;
+            ib = vcap.y;
             const double4 q = g.uvw64[v];
             u = q.x * k.s; vv = q.y * k.s; w = q.z * k.s;
         }
@@ -442,8 +441,7 @@ __device__ __forceinline__ CapRec cap_full(const CapLite &l) {
 }
 
 struct VisRaw {  // what wave 0 holds between the two halves of staging
-    int2 pr;
-    int t;
+    int2 cap;
     bool ok;
     double4 q;
 };
@@ -451,9 +449,7 @@ struct VisRaw {  // what wave 0 holds between the two halves of staging
 __device__ __forceinline__ void vis_raw_load(const GeomDev &g, int64_t vv, VisRaw &r) {
     r.ok = vv < g.n_vis;
     if (r.ok) {
-        const int b = (int)(vv / g.n_t);
-        r.t = (int)(vv % g.n_t);
-        r.pr = g.pairs[b];
+        r.cap = g.vis_cap[vv];
         r.q = g.uvw64[vv];
     }
 }
@@ -461,8 +457,8 @@ __device__ __forceinline__ void vis_raw_load(const GeomDev &g, int64_t vv, VisRa
 __device__ __forceinline__ void vis_stage_store(const GeomDev &g, const PhaseKnobs &k,
                                                 const ChunkRec &ch, const VisRaw &r, VisS &e) {
     if (r.ok) {
-        e.ia = r.pr.x * g.n_t + r.t;
-        e.ib = r.pr.y * g.n_t + r.t;
+        e.ia = r.cap.x;
+        e.ib = r.cap.y;
         const CapRec a = g.cap[e.ia], b = g.cap[e.ib];
         e.a = cap_lite(a);
         e.b = cap_lite(b);

@@ -4,7 +4,7 @@
 //   pos    double4 [n_sat][n_t]      satellite position (km) and |r|^2, FP64
 //   cap    CapRec  [n_sat][n_t]      shade cap of that satellite at that step (FP32)
 //   uvw64  double4 [n_vis]           (u, v, w, 0) wavelengths, v = b * n_t + t
-//   uvw32  float4  [n_vis]           the same rounded to FP32
+//   vis_cap int2   [n_vis]           cap rows of the pair, (sat_a * n_t + t, sat_b * n_t + t)
 //   pairs  int2    [n_base]          satellite indices of each baseline
 //   pix32  float4  [n_pad]           (l, m, n, 0) in chunk order, FP32
 //   pix64  double4 [n_pad]           (l, m, n, 0) in chunk order, FP64
