@@ -629,7 +629,43 @@ static_assert(sizeof(RecCold) == 48, "recon cold record");
 // the slots come from g.slots.  Each step stages kReconStage visibilities
 // (one per thread), classifies them against the block's chunk once, and
 // compacts away those that shade the whole chunk: wholly clear ones first,
-// then the straddling ones, so that each kind runs its own branch-free loop.
+// then those with one straddling cap (stored as cap a), then those with two,
+// so that each kind runs its own branch-free loop.
+
+// One staged straddling visibility over this lane's pixels: the FP32 test of
+// cap a (and of cap b when TWO), the exact FP64 pair predicate for pixels in a
+// guard band, shaded pixels dropped.
+template <bool F64, bool TWO>
+__device__ __forceinline__ void recon_straddle(const GeomDev &g, int c, int lane,
+                                               const RecHot<F64> &e, const RecCold &o,
+                                               const typename PhaseT<F64>::T (&l)[kPixPerLane],
+                                               const typename PhaseT<F64>::T (&m)[kPixPerLane],
+                                               const typename PhaseT<F64>::T (&n)[kPixPerLane],
+                                               const float (&sx)[kPixPerLane],
+                                               const float (&sy)[kPixPerLane],
+                                               const float (&sz)[kPixPerLane],
+                                               float (&acc)[kPixPerLane]) {
+    using T = typename PhaseT<F64>::T;
+    const CapRec A2 = {o.a.dx, o.a.dy, o.a.dz, o.a.c, 0.f, o.a.g, 0.f, 0.f};
+    const CapRec B2 = {o.b.dx, o.b.dy, o.b.dz, o.b.c, 0.f, o.b.g, 0.f, 0.f};
+#pragma unroll
+    for (int j = 0; j < kPixPerLane; ++j) {
+        bool unc = false;
+        bool sh = occ32(sx[j], sy[j], sz[j], A2, unc);
+        if (TWO) sh = occ32(sx[j], sy[j], sz[j], B2, unc) | sh;
+        if (unc) {
+            int nl;
+            sh = occ_pair64(g.pos, g.pix64, (int64_t)c * kChunk + j * kWave + lane, o.ia, o.ib,
+                            g.limb, &nl);
+        }
+        const float vr = sh ? 0.0f : e.vr, vi = sh ? 0.0f : e.vi;
+        const T ph = fma(e.u, l[j], fma(e.v, m[j], fma(e.w, n[j], e.wo)));
+        const float r = fract_rev(ph);
+        acc[j] = fmaf(vr, __builtin_amdgcn_cosf(r), acc[j]);
+        acc[j] = fmaf(-vi, __builtin_amdgcn_sinf(r), acc[j]);
+    }
+}
+
 template <bool F64>
 __device__ __forceinline__ void recon_body(const GeomDev &g, const PhaseKnobs &k, int32_t n_maps,
                                            const float2 *__restrict__ V,
@@ -638,7 +674,7 @@ __device__ __forceinline__ void recon_body(const GeomDev &g, const PhaseKnobs &k
     using T = typename PhaseT<F64>::T;
     __shared__ RecHot<F64> hot[kReconStage];
     __shared__ RecCold cold[kReconStage];
-    __shared__ int2 wave_cnt[kWavesPerBlock];  // (clear, straddling) staged per wave
+    __shared__ int4 wave_cnt[kWavesPerBlock];  // (clear, one cap, two caps) staged per wave
     __shared__ float red[kWavesPerBlock][kChunk];
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
@@ -687,36 +723,46 @@ __device__ __forceinline__ void recon_body(const GeomDev &g, const PhaseKnobs &k
         for (int64_t vt = vb; vt < ve; vt += kReconStage) {
             // ---- stage: one visibility per thread, classified, compacted
             const int64_t v = vt + threadIdx.x;
-            int ia = 0, ib = 0, cls = kIn;
+            int ia = 0, ib = 0, ca = kIn, cb = kIn;
             double u = 0, vv = 0, w = 0;
             CapRec A, B;
             if (v < ve && vis_geometry(g, v, map, ia, ib, u, vv, w)) {
                 A = g.cap[ia];
                 B = g.cap[ib];
-                cls = classify(ch, A, B);
+                ca = classify_cap(ch, A);
+                cb = classify_cap(ch, B);
             }
-            const bool clear = cls == kOut, strad = cls == kStraddle;
-            const uint64_t bc = __ballot(clear), bs = __ballot(strad);
-            if (lane == 0) wave_cnt[wave] = make_int2(__popcll(bc), __popcll(bs));
+            const bool live = ca != kIn && cb != kIn;
+            const bool clear = ca == kOut && cb == kOut;
+            const bool two = ca == kStraddle && cb == kStraddle;
+            const bool one = live && !clear && !two;
+            const uint64_t bc = __ballot(clear), b1 = __ballot(one), b2 = __ballot(two);
+            if (lane == 0)
+                wave_cnt[wave] = make_int4(__popcll(bc), __popcll(b1), __popcll(b2), 0);
             __syncthreads();
-            int base_c = 0, base_s = 0, n_clear = 0, n_live = 0;
+            int base_c = 0, base_1 = 0, base_2 = 0, n_clear = 0, n_one = 0, n_live = 0;
 #pragma unroll
             for (int q = 0; q < kWavesPerBlock; ++q) {
-                const int2 wc = wave_cnt[q];
+                const int4 wc = wave_cnt[q];
                 base_c += (q < wave) ? wc.x : 0;
-                base_s += (q < wave) ? wc.y : 0;
+                base_1 += (q < wave) ? wc.y : 0;
+                base_2 += (q < wave) ? wc.z : 0;
                 n_clear += wc.x;
-                n_live += wc.x + wc.y;
+                n_one += wc.y;
+                n_live += wc.x + wc.y + wc.z;
             }
             // all wave-uniform; saying so keeps the pixel loops below scalar loops
             base_c = __builtin_amdgcn_readfirstlane(base_c);
-            base_s = __builtin_amdgcn_readfirstlane(base_s);
+            base_1 = __builtin_amdgcn_readfirstlane(base_1);
+            base_2 = __builtin_amdgcn_readfirstlane(base_2);
             n_clear = __builtin_amdgcn_readfirstlane(n_clear);
+            n_one = __builtin_amdgcn_readfirstlane(n_one);
             n_live = __builtin_amdgcn_readfirstlane(n_live);
-            if (clear || strad) {
+            if (live) {
                 const uint64_t below = (1ull << lane) - 1ull;
                 const int pos = clear ? base_c + __popcll(bc & below)
-                                      : n_clear + base_s + __popcll(bs & below);
+                                : one ? n_clear + base_1 + __popcll(b1 & below)
+                                      : n_clear + n_one + base_2 + __popcll(b2 & below);
                 RecHot<F64> &e = hot[pos];
                 if (F64) {
                     e.u = (T)(u * k.s); e.v = (T)(vv * k.s); e.w = (T)(w * k.s);
@@ -730,7 +776,9 @@ __device__ __forceinline__ void recon_body(const GeomDev &g, const PhaseKnobs &k
                 e.vr = q.x * sc;
                 e.vi = q.y * sc;
                 RecCold &o = cold[pos];
-                o.a = {A.dx, A.dy, A.dz, A.c, A.g};
+                // one straddling cap goes first (cap b is then wholly clear of the chunk)
+                const CapRec &t0 = (ca == kOut) ? B : A;
+                o.a = {t0.dx, t0.dy, t0.dz, t0.c, t0.g};
                 o.b = {B.dx, B.dy, B.dz, B.c, B.g};
                 o.ia = ia;
                 o.ib = ib;
@@ -748,29 +796,10 @@ __device__ __forceinline__ void recon_body(const GeomDev &g, const PhaseKnobs &k
                     acc[j] = fmaf(-e.vi, __builtin_amdgcn_sinf(r), acc[j]);
                 }
             }
-            for (int q = n_clear + wave; q < n_live; q += kWavesPerBlock) {
-                // the pair straddles the chunk: test each pixel
-                const RecHot<F64> e = hot[q];
-                const CapTest ta = cold[q].a, tb = cold[q].b;
-                const CapRec A2 = {ta.dx, ta.dy, ta.dz, ta.c, 0.f, ta.g, 0.f, 0.f};
-                const CapRec B2 = {tb.dx, tb.dy, tb.dz, tb.c, 0.f, tb.g, 0.f, 0.f};
-#pragma unroll
-                for (int j = 0; j < kPixPerLane; ++j) {
-                    bool unc = false;
-                    bool sh = occ32(sx[j], sy[j], sz[j], A2, unc) |
-                              occ32(sx[j], sy[j], sz[j], B2, unc);
-                    if (unc) {
-                        int nl;
-                        sh = occ_pair64(g.pos, g.pix64, (int64_t)c * kChunk + j * kWave + lane,
-                                        cold[q].ia, cold[q].ib, g.limb, &nl);
-                    }
-                    const float vr = sh ? 0.0f : e.vr, vi = sh ? 0.0f : e.vi;
-                    const T ph = fma(e.u, l[j], fma(e.v, m[j], fma(e.w, n[j], e.wo)));
-                    const float r = fract_rev(ph);
-                    acc[j] = fmaf(vr, __builtin_amdgcn_cosf(r), acc[j]);
-                    acc[j] = fmaf(-vi, __builtin_amdgcn_sinf(r), acc[j]);
-                }
-            }
+            for (int q = n_clear + wave; q < n_clear + n_one; q += kWavesPerBlock)
+                recon_straddle<F64, false>(g, c, lane, hot[q], cold[q], l, m, n, sx, sy, sz, acc);
+            for (int q = n_clear + n_one + wave; q < n_live; q += kWavesPerBlock)
+                recon_straddle<F64, true>(g, c, lane, hot[q], cold[q], l, m, n, sx, sy, sz, acc);
             __syncthreads();
         }
 #pragma unroll
