@@ -301,6 +301,14 @@ def main():
                      "source": "profiles/instmix_c2.json (rocprofv3 SQ_ACTIVE_INST_VALU, "
                                "GRBM_GUI_ACTIVE, SQ_INSTS_VALU[_TRANS_F32])"}
     achieved = FLOPS_PER_TERM * wl.terms_per_direction / (dom_ms * 1e-3) / 1e12
+    # measured VALU/transcendental ceiling: the bare term, registers only, full occupancy
+    ceil = ctx.term_ceiling()
+    ceiling = {"value": round(ceil, 1), "unit": "Gterms/s",
+               "frac": {k: round(wl.terms_per_direction / (v * 1e-3) / 1e9 / ceil, 4)
+                        for k, v in kt_plain.items() if k in ("visgen", "recon")},
+               "note": "vrs_term_ceiling: 3 FMA + fract + v_cos + v_sin + 2 FMA per term, no "
+                       "shading, no memory; recon's figure counts all (vis, pix) pairs, "
+                       "including the wholly shaded ones it skips"}
     out = {
         "metric": "Gterms/s (vis x pix) vis-gen + recon",
         "value": round(value, 2),
@@ -328,7 +336,8 @@ def main():
                      "algorithmic_bytes": algorithmic_bytes(wl, dom),
                      "flops_per_term": FLOPS_PER_TERM,
                      "trans_frac": round(2 * wl.terms_per_direction / (dom_ms * 1e-3) / TRANS_PEAK, 4),
-                     "kernel_ms": round(dom_ms, 3), "kernel_name": kname, "issue": issue},
+                     "kernel_ms": round(dom_ms, 3), "kernel_name": kname, "issue": issue,
+                     "valu_trans_ceiling": ceiling},
         "kernels_ms": {k: round(v, 3) for k, v in kt_plain.items()},
         "visgen_tile": {"plain": tile_plain[0], "mixed": tile_mixed[0]},
         "host_buffers": {"value": round(terms / (ms_host * 1e-3) / 1e9, 2), "ms_per_step": round(ms_host, 3),

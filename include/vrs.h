@@ -113,6 +113,12 @@ vrs_status vrs_timing_enable(vrs_ctx *ctx, int on);
 /* Milliseconds of the last launch of `kernel` ("orbit", "shade", "visgen", "recon", "mfma_visgen",
  * "mfma_recon", "batch"); -1 when unknown.  Synchronizes the stream. */
 double vrs_timing_last_ms(vrs_ctx *ctx, const char *kernel);
+/* Measured VALU/transcendental ceiling of the plain FP32 term on this device:
+ * a register-only kernel runs the bare term (3 FMA phase, fract, v_cos, v_sin,
+ * 2 FMA accumulate; no shading, no memory) at full occupancy.  *gterms_per_s =
+ * terms per second / 1e9, best of two timed launches (~2 ms each).  Measurement
+ * only: it has no counterpart in the reference. */
+vrs_status vrs_term_ceiling(vrs_ctx *ctx, double *gterms_per_s);
 
 /* --------------------------------------------------------------- geometry */
 /* Stage 1 and the static sky: propagates the orbits on the GPU, forms uvw and the

@@ -113,3 +113,24 @@ def test_map_count_across_matrix_core_groups(ctx, oracle):
     for i, m in enumerate(sel):
         assert rel(F[m], Fo[i]) <= 1e-4, m
     g.close()
+
+
+def test_term_ceiling_bounds(ctx):
+    """The measured bare-term rate: positive, below the VALU issue bound (one
+    wave64 VALU instruction per 4 clocks per SIMD at 2.4 GHz: 1024 SIMDs x
+    2.4e9 x 16 lanes / 6 non-transcendental instructions), and above the
+    plain vis-gen kernel's own rate, which runs the same term plus shading."""
+    print("[edges] term ceiling", flush=True)
+    c = ctx.term_ceiling()
+    assert 0 < c < 1024 * 2.4e9 * 16 / 6 / 1e9, c
+    wl = W.CONFIGS["c1"]
+    g = ctx.workload_geometry(wl)
+    I = W.make_sky(wl)
+    k = vrs.knobs(precision="plain")
+    ctx.timing(True)
+    g.visscal(I, k)
+    ms = ctx.last_ms("visgen")
+    ctx.timing(False)
+    rate = wl.n_vis * wl.n_pix / (ms * 1e-3) / 1e9
+    assert rate < 1.05 * c, (rate, c)
+    g.close()

@@ -82,6 +82,7 @@ SIGNATURES = {
     "vrs_knobs_reference": (None, [C.POINTER(Knobs)]),
     "vrs_timing_enable": (C.c_int, [_ctx, C.c_int]),
     "vrs_timing_last_ms": (C.c_double, [_ctx, C.c_char_p]),
+    "vrs_term_ceiling": (C.c_int, [_ctx, C.POINTER(C.c_double)]),
     "vrs_geom_create": (C.c_int, [_ctx, C.POINTER(GeomDesc), C.POINTER(C.c_void_p)]),
     "vrs_geom_destroy": (None, [_vp]),
     "vrs_geom_info_get": (C.c_int, [_vp, C.POINTER(GeomInfo)]),

@@ -89,6 +89,13 @@ class Context:
     def last_ms(self, kernel):
         return self._lib.vrs_timing_last_ms(self.handle, kernel.encode())
 
+    def term_ceiling(self):
+        """Measured Gterms/s of the bare plain-FP32 term (registers only, no
+        shading): the VALU/transcendental ceiling of the vis-gen and recon terms."""
+        out = C.c_double(0.0)
+        self.check(self._lib.vrs_term_ceiling(self.handle, C.byref(out)))
+        return out.value
+
     def geometry(self, n_sat, n_t, freq_hz, nside, raan=None, m0=None, dt_s=W.DT_S,
                  a_km=W.A_KM, incl_rad=W.INCL_RAD, mu=W.MU_MOON, r_moon_km=W.R_MOON_KM,
                  pairs=None):

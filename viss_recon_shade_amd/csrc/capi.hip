@@ -374,6 +374,45 @@ double vrs_timing_last_ms(vrs_ctx *ctx, const char *kernel) {
     return ms;
 }
 
+vrs_status vrs_term_ceiling(vrs_ctx *ctx, double *gterms_per_s) {
+    if (vrs_status s = check_ctx(ctx)) return s;
+    if (!gterms_per_s) return fail(ctx, VRS_E_ARG, "gterms_per_s is required");
+    DeviceGuard dg(ctx->device);
+    // eight blocks of 256 per CU; about 2 ms per launch on an MI355X
+    const int32_t blocks = 8 * std::max(ctx->n_cu, 1), iters = 2048;
+    struct Buf {
+        void *p = nullptr;
+        ~Buf() { if (p) (void)hipFree(p); }
+    } b_uvw, b_out;
+    struct Ev {
+        hipEvent_t e = nullptr;
+        ~Ev() { if (e) (void)hipEventDestroy(e); }
+    } e0, e1;
+    float4 tab[16];
+    for (int i = 0; i < 16; ++i)  // (u, v, w, w offset) in revolutions, |phase| up to ~1e5
+        tab[i] = make_float4(1.0e5f - 977.0f * i, -3.1e4f + 411.0f * i, 812.5f + 3.0f * i,
+                             -812.5f - 3.0f * i);
+    VRS_HIP(ctx, hipMalloc(&b_uvw.p, sizeof(tab)));
+    VRS_HIP(ctx, hipMalloc(&b_out.p, sizeof(float) * 256 * (size_t)blocks));
+    VRS_HIP(ctx, hipMemcpyAsync(b_uvw.p, tab, sizeof(tab), hipMemcpyHostToDevice, ctx->stream));
+    VRS_HIP(ctx, hipEventCreate(&e0.e));
+    VRS_HIP(ctx, hipEventCreate(&e1.e));
+    double best = 0.0;
+    for (int rep = 0; rep < 3; ++rep) {  // the first launch warms the clocks
+        VRS_HIP(ctx, hipEventRecord(e0.e, ctx->stream));
+        VRS_HIP(ctx, launch_term_ceiling((const float4 *)b_uvw.p, iters, blocks, (float *)b_out.p,
+                                         ctx->stream));
+        VRS_HIP(ctx, hipEventRecord(e1.e, ctx->stream));
+        VRS_HIP(ctx, hipEventSynchronize(e1.e));
+        float ms = 0.0f;
+        VRS_HIP(ctx, hipEventElapsedTime(&ms, e0.e, e1.e));
+        if (rep > 0 && ms > 0.0f)
+            best = std::max(best, (double)term_ceiling_terms(iters, blocks) / (ms * 1e-3) / 1e9);
+    }
+    *gterms_per_s = best;
+    return VRS_OK;
+}
+
 uint32_t vrs_pix_hash(int64_t p) { return pix_hash_u64((uint64_t)p); }
 
 // ---------------------------------------------------------------- host helpers

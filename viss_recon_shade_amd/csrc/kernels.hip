@@ -1009,6 +1009,50 @@ const void *visgen_fine_kernel_ptr(bool f64) {
     return f64 ? (const void *)visgen_kernel<true, 16, true> : (const void *)visgen_kernel<false, 16, true>;
 }
 
+// ------------------------------------------------------------------ ceiling
+// The bare plain-FP32 term at full occupancy, registers only: per (visibility,
+// pixel) 3 FMA (phase), fract, v_cos, v_sin, 2 FMA (accumulate) and nothing
+// else; the visibilities come from a 16-entry table through the scalar cache.
+// Its rate is the VALU/transcendental ceiling of the vis-gen and recon terms.
+constexpr int kCeilPix = 8;
+__global__ void __launch_bounds__(256) term_ceiling_kernel(const float4 *__restrict__ uvw,
+                                                           int32_t iters,
+                                                           float *__restrict__ out) {
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    float l[kCeilPix], m[kCeilPix], n[kCeilPix], I[kCeilPix], ar[kCeilPix], ai[kCeilPix];
+#pragma unroll
+    for (int j = 0; j < kCeilPix; ++j) {
+        const float a = 0.37f * (float)((gid * kCeilPix + j) & 1023) * (1.0f / 1024.0f);
+        l[j] = 0.6f * a;
+        m[j] = 0.8f - 0.5f * a;
+        n[j] = 0.1f + 0.3f * a;
+        I[j] = 1.0f + 0.01f * (float)j;
+        ar[j] = ai[j] = 0.0f;
+    }
+    for (int it = 0; it < iters; ++it) {
+        const float4 q = uvw[it & 15];
+#pragma unroll
+        for (int j = 0; j < kCeilPix; ++j) {
+            const float r = __builtin_amdgcn_fractf(fmaf(q.x, l[j], fmaf(q.y, m[j], fmaf(q.z, n[j], q.w))));
+            ar[j] = fmaf(I[j], __builtin_amdgcn_cosf(r), ar[j]);
+            ai[j] = fmaf(I[j], __builtin_amdgcn_sinf(r), ai[j]);
+        }
+    }
+    float s = 0.0f;
+#pragma unroll
+    for (int j = 0; j < kCeilPix; ++j) s += ar[j] + ai[j];
+    out[gid] = s;
+}
+
+hipError_t launch_term_ceiling(const float4 *uvw, int32_t iters, int32_t blocks, float *out,
+                               hipStream_t st) {
+    hipLaunchKernelGGL(term_ceiling_kernel, dim3(blocks), dim3(256), 0, st, uvw, iters, out);
+    return hipGetLastError();
+}
+int64_t term_ceiling_terms(int32_t iters, int32_t blocks) {
+    return (int64_t)blocks * 256 * iters * kCeilPix;
+}
+
 int visgen_tile_vis() { return 16; }
 int list_pad() { return kListPad; }
 int recon_stage_vis() { return kReconStage; }

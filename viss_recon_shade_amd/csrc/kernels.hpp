@@ -94,6 +94,10 @@ hipError_t launch_from_chunk_order(const GeomDev &g, int64_t n_pix, const float 
 
 // Vis-gen with the sub-pixel (fine-cell) shade mask; Ich = per-pixel mean FF (chunk order);
 // g.chunk must be the widened fine chunk bounds.
+// Bare-term ceiling kernel: `blocks` x 256 threads, each writes out[thread] once.
+hipError_t launch_term_ceiling(const float4 *uvw, int32_t iters, int32_t blocks, float *out,
+                               hipStream_t st);
+int64_t term_ceiling_terms(int32_t iters, int32_t blocks);
 hipError_t launch_visgen_fine(const GeomDev &g, const PhaseKnobs &k, bool f64, const float *Ich,
                               float2 *V, int grid, hipStream_t st);
 const void *visgen_fine_kernel_ptr(bool f64);
