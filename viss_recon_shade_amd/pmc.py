@@ -72,7 +72,7 @@ def short_name(kernel):
                 k = k[:i]
                 break
     k = re.sub(r"^void\s+", "", k)
-    k = re.sub(r"^vrs::\(anonymous namespace\)::", "", k)
+    k = re.sub(r"^vrs::(\(anonymous namespace\)::)?", "", k)
     return k
 
 
