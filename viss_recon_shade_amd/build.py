@@ -28,7 +28,14 @@ DEVICE = ["-x", "hip", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "
 # Per-source device flags.  kernels.hip keeps its per-visibility accumulators
 # in unrolled register arrays updated under wave-uniform branches; promoting
 # those arrays to vector values makes every branch copy the whole array.
-EXTRA = {"kernels.hip": ["-mllvm", "-disable-promote-alloca-to-vector"]}
+# Machine scheduler strategies, chosen by timing on an MI355X (c2 plain, c2
+# FP64-phase and c3 64-map kernels; every default-strategy kernel was 0.4-2.4 %
+# slower): max-ilp for the VALU-bound DFT kernels, iterative-ilp for the
+# matrix-core kernels (two waves per SIMD: instruction-level parallelism inside
+# a wave is what overlaps their VALU work with the MFMA chains).
+EXTRA = {"kernels.hip": ["-mllvm", "-disable-promote-alloca-to-vector",
+                         "-mllvm", "--amdgpu-sched-strategy=max-ilp"],
+         "mfma.hip": ["-mllvm", "--amdgpu-sched-strategy=iterative-ilp"]}
 
 
 def _compile(src):
