@@ -687,8 +687,10 @@ vrs_status vrs_geom_lmn(vrs_ctx *ctx, const vrs_geom *g, double *lmn) {
     if (!g || !lmn) return fail(ctx, VRS_E_ARG, "geom and lmn must be non-NULL");
     if (is_device_ptr(lmn)) {
         DeviceGuard dg(g->ctx->device);
-        VRS_HIP(ctx, hipMemcpy(lmn, g->lmn_ring.data(), sizeof(double) * g->lmn_ring.size(),
-                               hipMemcpyHostToDevice));
+        // on the context stream, finished before the host vector can change
+        VRS_HIP(ctx, hipMemcpyAsync(lmn, g->lmn_ring.data(), sizeof(double) * g->lmn_ring.size(),
+                                    hipMemcpyHostToDevice, g->ctx->stream));
+        VRS_HIP(ctx, hipStreamSynchronize(g->ctx->stream));
     } else {
         std::memcpy(lmn, g->lmn_ring.data(), sizeof(double) * g->lmn_ring.size());
     }
@@ -1005,8 +1007,12 @@ vrs_status batch_set_seg(vrs_batch *b, int seg) {
     b->d_part = nullptr;
     VRS_HIP(ctx, hipMalloc(&b->d_slots, sizeof(int4) * std::max<size_t>(slots.size(), 1)));
     VRS_HIP(ctx, hipMalloc(&b->d_part, sizeof(float) * std::max<size_t>(slots.size(), 1) * b->g->n_pad));
-    VRS_HIP(ctx, hipMemcpy(b->d_slots, slots.data(), sizeof(int4) * slots.size(), hipMemcpyHostToDevice));
-    VRS_HIP(ctx, hipMemcpy(b->d_map_slot0, slot0.data(), sizeof(int32_t) * slot0.size(), hipMemcpyHostToDevice));
+    // on the batch stream (the one its graph runs on); the host vectors end here
+    VRS_HIP(ctx, hipMemcpyAsync(b->d_slots, slots.data(), sizeof(int4) * slots.size(),
+                                hipMemcpyHostToDevice, b->stream));
+    VRS_HIP(ctx, hipMemcpyAsync(b->d_map_slot0, slot0.data(), sizeof(int32_t) * slot0.size(),
+                                hipMemcpyHostToDevice, b->stream));
+    VRS_HIP(ctx, hipStreamSynchronize(b->stream));
     b->n_slots = (int64_t)slots.size();
     b->seg = seg;
     return VRS_OK;
@@ -1111,8 +1117,10 @@ vrs_status vrs_batch_create(vrs_ctx *ctx, const vrs_geom *g, const vrs_batch_des
     DeviceGuard dg(ctx->device);
     // positions for the line-of-sight test (FP64, exported from stage 1)
     std::vector<double> pos((size_t)g->n_sat * g->n_t * 3);
-    VRS_HIP(ctx, hipMemcpy2D(pos.data(), 3 * sizeof(double), g->d_pos, sizeof(double4),
-                             3 * sizeof(double), (size_t)g->n_sat * g->n_t, hipMemcpyDeviceToHost));
+    VRS_HIP(ctx, hipMemcpy2DAsync(pos.data(), 3 * sizeof(double), g->d_pos, sizeof(double4),
+                                  3 * sizeof(double), (size_t)g->n_sat * g->n_t,
+                                  hipMemcpyDeviceToHost, ctx->stream));
+    VRS_HIP(ctx, hipStreamSynchronize(ctx->stream));
     vrs_batch *b = new vrs_batch;
     b->ctx = ctx;
     b->g = g;
@@ -1175,11 +1183,17 @@ vrs_status vrs_batch_create(vrs_ctx *ctx, const vrs_geom *g, const vrs_batch_des
     VRS_BHIP(hipMalloc(&b->d_Ich, sizeof(float) * b->n_ch * g->n_pad));
     VRS_BHIP(hipMalloc(&b->d_F, sizeof(float) * b->n_ch * g->n_pix));
     VRS_BHIP(hipMalloc(&b->d_V, sizeof(float2) * b->n_pad_vis));
-    VRS_BHIP(hipMemset(b->d_V, 0, sizeof(float2) * b->n_pad_vis));  // padding entries stay zero
-    VRS_BHIP(hipMemset(b->d_I, 0, sizeof(float) * b->n_ch * g->n_pix));
-    VRS_BHIP(hipMemcpy(b->d_vis_sat, vis_sat.data(), sizeof(int2) * vis_sat.size(), hipMemcpyHostToDevice));
-    VRS_BHIP(hipMemcpy(b->d_tile_map, tile_map.data(), sizeof(int32_t) * tile_map.size(), hipMemcpyHostToDevice));
-    VRS_BHIP(hipMemcpy(b->d_inv_lambda, b->inv_lambda.data(), sizeof(double) * b->n_ch, hipMemcpyHostToDevice));
+    // set-up on the batch stream (not captured: the graph is built later), finished
+    // before the host vectors go away
+    VRS_BHIP(hipMemsetAsync(b->d_V, 0, sizeof(float2) * b->n_pad_vis, b->stream));  // padding stays zero
+    VRS_BHIP(hipMemsetAsync(b->d_I, 0, sizeof(float) * b->n_ch * g->n_pix, b->stream));
+    VRS_BHIP(hipMemcpyAsync(b->d_vis_sat, vis_sat.data(), sizeof(int2) * vis_sat.size(),
+                            hipMemcpyHostToDevice, b->stream));
+    VRS_BHIP(hipMemcpyAsync(b->d_tile_map, tile_map.data(), sizeof(int32_t) * tile_map.size(),
+                            hipMemcpyHostToDevice, b->stream));
+    VRS_BHIP(hipMemcpyAsync(b->d_inv_lambda, b->inv_lambda.data(), sizeof(double) * b->n_ch,
+                            hipMemcpyHostToDevice, b->stream));
+    VRS_BHIP(hipStreamSynchronize(b->stream));
     if (vrs_status s = batch_set_seg(b, 16384)) return cleanup(s);
 
     // autotune: vis-gen tile (8, 16, 32 visibilities) and recon range length
