@@ -480,6 +480,7 @@ __global__ void __launch_bounds__(kThreads, 2) mfma_recon_kernel(
     const float *__restrict__ inv_scale, int32_t n_seg, float *__restrict__ part) {
     __shared__ _Float16 sA[2][4][kMaps][kStage];  // [buffer][re hi, re lo, im hi, im lo][map][vis slot]
     __shared__ VisS sV[2][kStage];
+    __shared__ int sAllIn[2];  // [buffer] every staged visibility shades the block's whole chunk
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
     const int col = lane & 31;
@@ -510,9 +511,11 @@ __global__ void __launch_bounds__(kThreads, 2) mfma_recon_kernel(
         if (n_st > 0) {
             // Both buffers are free: every wave passed the previous item's last barrier.
             stage_load_lds<4>(Ag, n_vis_pad, nullptr, vb / kStage, &sA[0][0][0][0], nullptr);
-            if (threadIdx.x < kStage) {
+            if (threadIdx.x < kStage) {  // wave 0
                 vis_raw_load(g, vb + threadIdx.x, raw);
                 vis_stage_store(g, k, ch, raw, sV[0][threadIdx.x]);
+                const bool all_in = __ballot(sV[0][threadIdx.x].cls == kIn) == ~0ull;
+                if (threadIdx.x == 0) sAllIn[0] = all_in;
             }
             wait_vm();
             __syncthreads();
@@ -526,64 +529,71 @@ __global__ void __launch_bounds__(kThreads, 2) mfma_recon_kernel(
                 stage_load_lds<4>(Ag, n_vis_pad, nullptr, v1 / kStage, &sA[buf ^ 1][0][0][0],
                                   nullptr);
             }
+            // a stage whose visibilities all shade the chunk adds nothing: skip it
+            if (!__builtin_amdgcn_readfirstlane(sAllIn[buf])) {
 #pragma unroll
-            for (int ks = 0; ks < kStage / kKS; ++ks) {
-                // E for this lane's pixel and 8 staged visibilities: straight-line
-                // phase and sin/cos, then the (wave-uniform per visibility) classes.
-                const int kb = ks * kKS + kg * 8;
-                float cs[8], sn[8];
-                uint32_t shm = 0, stm = 0;
-#pragma unroll
-                for (int t = 0; t < 8; ++t) {
-                    const VisS &q = sV[buf][kb + t];
-                    const double ph = fma(q.u, s.x, fma(q.v, s.y, fma(q.w, s.z, q.wo)));
-                    const float r = (float)__builtin_amdgcn_fract(ph);
-                    cs[t] = __builtin_amdgcn_cosf(r);
-                    sn[t] = __builtin_amdgcn_sinf(r);
-                    shm |= (uint32_t)(q.cls == kIn) << t;
-                    stm |= (uint32_t)(q.cls == kStraddle) << t;
-                }
-                H8 bre_h, bre_l, bim_h, bim_l;
-                if (__builtin_amdgcn_readfirstlane(shm | stm)) {  // the classes are wave-uniform
-                    while (stm) {
-                        const int t = __ffs(stm) - 1;
-                        stm &= stm - 1;
-                        const VisS &q = sV[buf][kb + t];
-                        if (pair_shaded(g, s, cap_full(q.a), cap_full(q.b), q.ia, q.ib))
-                            shm |= 1u << t;
-                    }
+                for (int ks = 0; ks < kStage / kKS; ++ks) {
+                    // E for this lane's pixel and 8 staged visibilities: straight-line
+                    // phase and sin/cos, then the (wave-uniform per visibility) classes.
+                    const int kb = ks * kKS + kg * 8;
+                    float cs[8], sn[8];
+                    uint32_t shm = 0, stm = 0;
 #pragma unroll
                     for (int t = 0; t < 8; ++t) {
-                        const bool sh = (shm >> t) & 1u;
-                        cs[t] = sh ? 0.0f : cs[t];
-                        sn[t] = sh ? 0.0f : -sn[t];   // Re(V e^{+i theta}) = Vr cos - Vi sin
+                        const VisS &q = sV[buf][kb + t];
+                        const double ph = fma(q.u, s.x, fma(q.v, s.y, fma(q.w, s.z, q.wo)));
+                        const float r = (float)__builtin_amdgcn_fract(ph);
+                        cs[t] = __builtin_amdgcn_cosf(r);
+                        sn[t] = __builtin_amdgcn_sinf(r);
+                        shm |= (uint32_t)(q.cls == kIn) << t;
+                        stm |= (uint32_t)(q.cls == kStraddle) << t;
                     }
-                } else {
+                    H8 bre_h, bre_l, bim_h, bim_l;
+                    if (__builtin_amdgcn_readfirstlane(shm | stm)) {  // the classes are wave-uniform
+                        while (stm) {
+                            const int t = __ffs(stm) - 1;
+                            stm &= stm - 1;
+                            const VisS &q = sV[buf][kb + t];
+                            if (pair_shaded(g, s, cap_full(q.a), cap_full(q.b), q.ia, q.ib))
+                                shm |= 1u << t;
+                        }
 #pragma unroll
-                    for (int t = 0; t < 8; ++t) sn[t] = -sn[t];
-                }
+                        for (int t = 0; t < 8; ++t) {
+                            const bool sh = (shm >> t) & 1u;
+                            cs[t] = sh ? 0.0f : cs[t];
+                            sn[t] = sh ? 0.0f : -sn[t];   // Re(V e^{+i theta}) = Vr cos - Vi sin
+                        }
+                    } else {
 #pragma unroll
-                for (int t2 = 0; t2 < 4; ++t2) {
-                    split2(cs[2 * t2], cs[2 * t2 + 1], bre_h.u[t2], bre_l.u[t2]);
-                    split2(sn[2 * t2], sn[2 * t2 + 1], bim_h.u[t2], bim_l.u[t2]);
-                }
+                        for (int t = 0; t < 8; ++t) sn[t] = -sn[t];
+                    }
 #pragma unroll
-                for (int mt = 0; mt < 2; ++mt) {
-                    const int row = mt * kMT + col, so = a_slot(row, ks * 2 + kg) * 8;
-                    const half8 arh = *reinterpret_cast<const half8 *>(&sA[buf][0][row][so]);
-                    const half8 arl = *reinterpret_cast<const half8 *>(&sA[buf][1][row][so]);
-                    const half8 aih = *reinterpret_cast<const half8 *>(&sA[buf][2][row][so]);
-                    const half8 ail = *reinterpret_cast<const half8 *>(&sA[buf][3][row][so]);
-                    acc[mt] = mfma(arh, bre_h.v, acc[mt]);
-                    acc[mt] = mfma(arh, bre_l.v, acc[mt]);
-                    acc[mt] = mfma(arl, bre_h.v, acc[mt]);
-                    acc[mt] = mfma(aih, bim_h.v, acc[mt]);
-                    acc[mt] = mfma(aih, bim_l.v, acc[mt]);
-                    acc[mt] = mfma(ail, bim_h.v, acc[mt]);
+                    for (int t2 = 0; t2 < 4; ++t2) {
+                        split2(cs[2 * t2], cs[2 * t2 + 1], bre_h.u[t2], bre_l.u[t2]);
+                        split2(sn[2 * t2], sn[2 * t2 + 1], bim_h.u[t2], bim_l.u[t2]);
+                    }
+#pragma unroll
+                    for (int mt = 0; mt < 2; ++mt) {
+                        const int row = mt * kMT + col, so = a_slot(row, ks * 2 + kg) * 8;
+                        const half8 arh = *reinterpret_cast<const half8 *>(&sA[buf][0][row][so]);
+                        const half8 arl = *reinterpret_cast<const half8 *>(&sA[buf][1][row][so]);
+                        const half8 aih = *reinterpret_cast<const half8 *>(&sA[buf][2][row][so]);
+                        const half8 ail = *reinterpret_cast<const half8 *>(&sA[buf][3][row][so]);
+                        acc[mt] = mfma(arh, bre_h.v, acc[mt]);
+                        acc[mt] = mfma(arh, bre_l.v, acc[mt]);
+                        acc[mt] = mfma(arl, bre_h.v, acc[mt]);
+                        acc[mt] = mfma(aih, bim_h.v, acc[mt]);
+                        acc[mt] = mfma(aih, bim_l.v, acc[mt]);
+                        acc[mt] = mfma(ail, bim_h.v, acc[mt]);
+                    }
                 }
             }
             // the other buffer was last read before the previous barrier
-            if (more && threadIdx.x < kStage) vis_stage_store(g, k, ch, raw, sV[buf ^ 1][threadIdx.x]);
+            if (more && threadIdx.x < kStage) {  // wave 0
+                vis_stage_store(g, k, ch, raw, sV[buf ^ 1][threadIdx.x]);
+                const bool all_in = __ballot(sV[buf ^ 1][threadIdx.x].cls == kIn) == ~0ull;
+                if (threadIdx.x == 0) sAllIn[buf ^ 1] = all_in;
+            }
             wait_vm();
             __syncthreads();
         }
