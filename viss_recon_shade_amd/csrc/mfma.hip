@@ -324,6 +324,7 @@ __global__ void __launch_bounds__(kThreads, 2) mfma_visgen_kernel(
         float gre = 0.0f, gim = 0.0f;  // sum of E over shaded pixels
         int cls = kOut;
         bool any_shade = false;
+        bool all_in = false;  // every visibility of the wave shades the whole chunk: E' = 0
 
         const _Float16 *Ag = A16 + grp * 2 * kMaps * g.n_pad;
         // Both buffers are free here: every wave passed the last barrier of
@@ -341,6 +342,7 @@ __global__ void __launch_bounds__(kThreads, 2) mfma_visgen_kernel(
             if (new_chunk) {
                 cls = classify_m(ch, sCap[vc][0], sCap[vc][1]);
                 any_shade = __ballot(cls != kOut) != 0;
+                all_in = __ballot(cls == kIn) == ~0ull;
             }
             // the other buffer was last read before the previous barrier
             if (st + 1 < n_stages)
@@ -377,22 +379,24 @@ __global__ void __launch_bounds__(kThreads, 2) mfma_visgen_kernel(
 #pragma unroll
                     for (int t = 0; t < 8; ++t) sn[t] = -sn[t];
                 }
+                if (!all_in) {  // wave-uniform
 #pragma unroll
-                for (int t2 = 0; t2 < 4; ++t2) {
-                    split2(cs[2 * t2], cs[2 * t2 + 1], bre_h.u[t2], bre_l.u[t2]);
-                    split2(sn[2 * t2], sn[2 * t2 + 1], bim_h.u[t2], bim_l.u[t2]);
-                }
+                    for (int t2 = 0; t2 < 4; ++t2) {
+                        split2(cs[2 * t2], cs[2 * t2 + 1], bre_h.u[t2], bre_l.u[t2]);
+                        split2(sn[2 * t2], sn[2 * t2 + 1], bim_h.u[t2], bim_l.u[t2]);
+                    }
 #pragma unroll
-                for (int mt = 0; mt < 2; ++mt) {
-                    const int row = mt * kMT + col, so = a_slot(row, ks * 2 + kg) * 8;
-                    const half8 ah = *reinterpret_cast<const half8 *>(&sA[buf][0][row][so]);
-                    const half8 al = *reinterpret_cast<const half8 *>(&sA[buf][1][row][so]);
-                    acc[mt][0] = mfma(ah, bre_h.v, acc[mt][0]);
-                    acc[mt][0] = mfma(ah, bre_l.v, acc[mt][0]);
-                    acc[mt][0] = mfma(al, bre_h.v, acc[mt][0]);
-                    acc[mt][1] = mfma(ah, bim_h.v, acc[mt][1]);
-                    acc[mt][1] = mfma(ah, bim_l.v, acc[mt][1]);
-                    acc[mt][1] = mfma(al, bim_h.v, acc[mt][1]);
+                    for (int mt = 0; mt < 2; ++mt) {
+                        const int row = mt * kMT + col, so = a_slot(row, ks * 2 + kg) * 8;
+                        const half8 ah = *reinterpret_cast<const half8 *>(&sA[buf][0][row][so]);
+                        const half8 al = *reinterpret_cast<const half8 *>(&sA[buf][1][row][so]);
+                        acc[mt][0] = mfma(ah, bre_h.v, acc[mt][0]);
+                        acc[mt][0] = mfma(ah, bre_l.v, acc[mt][0]);
+                        acc[mt][0] = mfma(al, bre_h.v, acc[mt][0]);
+                        acc[mt][1] = mfma(ah, bim_h.v, acc[mt][1]);
+                        acc[mt][1] = mfma(ah, bim_l.v, acc[mt][1]);
+                        acc[mt][1] = mfma(al, bim_h.v, acc[mt][1]);
+                    }
                 }
             }
             wait_vm();
