@@ -31,10 +31,11 @@ constexpr int kMaps = 64;          // maps per pass (two M tiles)
 constexpr int kNT = 32;            // columns per wave (visibilities or pixels)
 constexpr int kKS = 16;            // K per MFMA
 constexpr int kStage = 64;         // K per LDS stage
-constexpr int kWaves = 4;
+constexpr int kWaves = 8;         // one block per CU: two waves per SIMD
 constexpr int kThreads = kWaves * 64;
 
 static_assert(kChunk % kStage == 0, "stages must tile chunks");
+static_assert(kWaves * kNT == kChunk, "a recon block covers one chunk");
 
 __device__ __forceinline__ floatx16 mfma(half8 a, half8 b, floatx16 c) {
     return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
@@ -228,7 +229,8 @@ __device__ __forceinline__ int acc_row(int r, int lane) {
 }
 
 // ------------------------------------------------------------------ vis-gen
-// Block: 4 waves x 32 visibilities; all waves share the staged map tile.
+// Block: 8 waves x 32 visibilities; all waves share the staged map tile (each
+// staged tile serves 256 visibilities, halving the map operand's re-reads).
 // Stages of kStage pixels are double-buffered: the next stage's map tile and
 // pixel vectors go straight from global memory into the other LDS buffer
 // while the current stage computes (one barrier per stage).
@@ -277,7 +279,7 @@ __device__ __forceinline__ void stage_load_lds(const _Float16 *__restrict__ Ag, 
 
 __device__ __forceinline__ void wait_vm() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 
-__global__ void __launch_bounds__(kThreads, 2) mfma_visgen_kernel(
+__global__ void __launch_bounds__(kThreads, 1) mfma_visgen_kernel(
     GeomDev g, PhaseKnobs k, int32_t n_maps, const _Float16 *__restrict__ A16,
     const float *__restrict__ inv_scale, float2 *__restrict__ V) {
     __shared__ _Float16 sA[2][2][kMaps][kStage];  // [buffer][hi/lo][map][pixel slot]
@@ -422,7 +424,8 @@ __global__ void __launch_bounds__(kThreads, 2) mfma_visgen_kernel(
 }
 
 // ------------------------------------------------------------------ recon
-// Block: 4 waves x 32 pixels; K runs over visibilities, staged 64 at a time.
+// Block: 8 waves x 32 pixels (one chunk); K runs over visibilities, staged 64 at a
+// time; each staged tile of weighted V serves the chunk's 256 pixels.
 // The weighted-V tile of the next stage moves straight into the other LDS
 // buffer while the current stage computes; wave 0 reads the next stage's
 // visibility geometry into registers at the top of a stage and classifies
@@ -479,7 +482,7 @@ __device__ __forceinline__ void vis_stage_store(const GeomDev &g, const PhaseKno
     }
 }
 
-__global__ void __launch_bounds__(kThreads, 2) mfma_recon_kernel(
+__global__ void __launch_bounds__(kThreads, 1) mfma_recon_kernel(
     GeomDev g, PhaseKnobs k, int32_t n_maps, int64_t n_vis_pad, const _Float16 *__restrict__ A16,
     const float *__restrict__ inv_scale, int32_t n_seg, float *__restrict__ part) {
     __shared__ _Float16 sA[2][4][kMaps][kStage];  // [buffer][re hi, re lo, im hi, im lo][map][vis slot]
@@ -489,7 +492,7 @@ __global__ void __launch_bounds__(kThreads, 2) mfma_recon_kernel(
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
     const int col = lane & 31;
     const int kg = lane >> 5;
-    const int64_t n_ptiles = g.n_pad / (kWaves * kNT);  // n_pad is a multiple of 128
+    const int64_t n_ptiles = g.n_pad / (kWaves * kNT);  // n_pad is a multiple of 256
     const int64_t n_groups = (n_maps + kMaps - 1) / kMaps;
     const int64_t per_seg = (n_vis_pad + n_seg - 1) / n_seg;
     const int64_t seg_len = (per_seg + kStage - 1) / kStage * kStage;
@@ -502,7 +505,7 @@ __global__ void __launch_bounds__(kThreads, 2) mfma_recon_kernel(
         const int64_t grp = rest / n_ptiles;
         const int64_t pi = pt * (kWaves * kNT) + wave * kNT + col;  // chunk-order pixel
         const double4 s = g.pix64[pi];
-        // the block's 128 pixels lie in one chunk (tiles are aligned to kChunk / 2)
+        // the block's pixels are one chunk
         const ChunkRec ch = g.chunk[(pt * (kWaves * kNT)) / kChunk];
         floatx16 acc[2];
         acc[0] = floatx16{};
