@@ -482,7 +482,7 @@ __device__ __forceinline__ void vis_stage_store(const GeomDev &g, const PhaseKno
     }
 }
 
-__global__ void __launch_bounds__(kThreads, 1) mfma_recon_kernel(
+__global__ void __launch_bounds__(kThreads, 4) mfma_recon_kernel(
     GeomDev g, PhaseKnobs k, int32_t n_maps, int64_t n_vis_pad, const _Float16 *__restrict__ A16,
     const float *__restrict__ inv_scale, int32_t n_seg, float *__restrict__ part) {
     __shared__ _Float16 sA[2][4][kMaps][kStage];  // [buffer][re hi, re lo, im hi, im lo][map][vis slot]
