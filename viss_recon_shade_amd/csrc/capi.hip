@@ -848,7 +848,7 @@ static vrs_status recon_common(vrs_ctx *ctx, const vrs_geom *g, const vrs_knobs 
         int64_t n_vis_pad;
         mfma_vis_pad(g->n_vis, &n_vis_pad);
         const int64_t groups = (n_maps + 63) / 64;
-        const int64_t ptiles = g->n_pad / mfma_pix_per_block();
+        const int64_t ptiles = (g->n_pad + mfma_pix_per_block() - 1) / mfma_pix_per_block();
         int64_t n_seg = (2048 + ptiles * groups - 1) / (ptiles * groups);
         n_seg = std::max<int64_t>(1, std::min<int64_t>(n_seg, n_vis_pad / 1024));
         if (n_seg > 8) n_seg = (n_seg + 7) / 8 * 8;
@@ -857,7 +857,7 @@ static vrs_status recon_common(vrs_ctx *ctx, const vrs_geom *g, const vrs_knobs 
         if (vrs_status s = ws_get(ctx, 3, 8 * (size_t)n_maps + 256, &small)) return s;
         if (vrs_status s = ws_get(ctx, 1, sizeof(float) * n_seg * n_maps * g->n_pad, &part)) return s;
         const int grid = persistent_grid(ctx, mfma_recon_kernel_ptr(), ptiles * groups * n_seg,
-                                         mfma_block_threads());
+                                         mfma_recon_block_threads());
         const GeomDev gd = g->dev();
         {
             Timer tm(ctx, "mfma_recon");

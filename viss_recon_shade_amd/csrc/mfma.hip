@@ -31,11 +31,13 @@ constexpr int kMaps = 64;          // maps per pass (two M tiles)
 constexpr int kNT = 32;            // columns per wave (visibilities or pixels)
 constexpr int kKS = 16;            // K per MFMA
 constexpr int kStage = 64;         // K per LDS stage
-constexpr int kWaves = 8;         // one block per CU: two waves per SIMD
+constexpr int kWaves = 8;         // vis-gen block: one per CU, two waves per SIMD
 constexpr int kThreads = kWaves * 64;
+constexpr int kWavesR = 16;       // recon block: two chunks, one block per CU, four waves per SIMD
+constexpr int kThreadsR = kWavesR * 64;
 
 static_assert(kChunk % kStage == 0, "stages must tile chunks");
-static_assert(kWaves * kNT == kChunk, "a recon block covers one chunk");
+static_assert(kWavesR * kNT == 2 * kChunk, "a recon block covers two chunks");
 
 __device__ __forceinline__ floatx16 mfma(half8 a, half8 b, floatx16 c) {
     return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
@@ -247,17 +249,17 @@ typedef __attribute__((address_space(3))) void *lds_ptr_t;
 // every wave moves whole 1 KiB blocks, lane l writing bytes [16 l, 16 l + 16)
 // of its block.  NH halves-planes of the map operand, plus kStage FP64 pixel
 // vectors when pix64 is given.  Completion: s_waitcnt vmcnt(0) + barrier.
-template <int NH>
+template <int NH, int NW>
 __device__ __forceinline__ void stage_load_lds(const _Float16 *__restrict__ Ag, int64_t ld,
                                                const double4 *__restrict__ pix64, int64_t st,
                                                _Float16 *sA, double4 *sP) {
     constexpr int kBlocks = NH * kMaps * (kStage / 8) / 64;   // 1 KiB blocks of the map tile
-    static_assert(kBlocks % kWaves == 0, "map tile splits evenly over the waves");
+    static_assert(kBlocks % NW == 0, "map tile splits evenly over the waves");
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int64_t p0 = st * kStage;
 #pragma unroll
-    for (int r = 0; r < kBlocks / kWaves; ++r) {
-        const int blk = wave * (kBlocks / kWaves) + r;
+    for (int r = 0; r < kBlocks / NW; ++r) {
+        const int blk = wave * (kBlocks / NW) + r;
         const int i = blk * 64 + lane;                       // 16-byte slot index in the tile
         const int hl = i / (kMaps * (kStage / 8));
         const int rem = i % (kMaps * (kStage / 8));
@@ -268,7 +270,7 @@ __device__ __forceinline__ void stage_load_lds(const _Float16 *__restrict__ Ag, 
     }
     if (pix64) {
         // kStage double4 = kStage / 32 blocks of 1 KiB, one per wave
-        static_assert(kStage / 32 <= kWaves, "pixel blocks fit the waves");
+        static_assert(kStage / 32 <= NW, "pixel blocks fit the waves");
         if (wave < kStage / 32) {
             const char *src = reinterpret_cast<const char *>(pix64 + p0) + wave * 1024 + lane * 16;
             __builtin_amdgcn_global_load_lds(src, (lds_ptr_t)(reinterpret_cast<char *>(sP) + wave * 1024),
@@ -331,7 +333,7 @@ __global__ void __launch_bounds__(kThreads, 1) mfma_visgen_kernel(
         const _Float16 *Ag = A16 + grp * 2 * kMaps * g.n_pad;
         // Both buffers are free here: every wave passed the last barrier of
         // the previous tile after its final reads.
-        stage_load_lds<2>(Ag, g.n_pad, g.pix64, 0, &sA[0][0][0][0], &sP[0][0]);
+        stage_load_lds<2, kWaves>(Ag, g.n_pad, g.pix64, 0, &sA[0][0][0][0], &sP[0][0]);
         wait_vm();
         __syncthreads();
         for (int64_t st = 0; st < n_stages; ++st) {
@@ -348,7 +350,7 @@ __global__ void __launch_bounds__(kThreads, 1) mfma_visgen_kernel(
             }
             // the other buffer was last read before the previous barrier
             if (st + 1 < n_stages)
-                stage_load_lds<2>(Ag, g.n_pad, g.pix64, st + 1, &sA[buf ^ 1][0][0][0],
+                stage_load_lds<2, kWaves>(Ag, g.n_pad, g.pix64, st + 1, &sA[buf ^ 1][0][0][0],
                                   &sP[buf ^ 1][0]);
 #pragma unroll 2
             for (int ks = 0; ks < kStage / kKS; ++ks) {
@@ -424,8 +426,9 @@ __global__ void __launch_bounds__(kThreads, 1) mfma_visgen_kernel(
 }
 
 // ------------------------------------------------------------------ recon
-// Block: 8 waves x 32 pixels (one chunk); K runs over visibilities, staged 64 at a
-// time; each staged tile of weighted V serves the chunk's 256 pixels.
+// Block: 16 waves x 32 pixels (two chunks, waves 0-7 and 8-15); K runs over
+// visibilities, staged 64 at a time; each staged tile of weighted V serves the
+// block's 512 pixels.
 // The weighted-V tile of the next stage moves straight into the other LDS
 // buffer while the current stage computes; wave 0 reads the next stage's
 // visibility geometry into registers at the top of a stage and classifies
@@ -437,7 +440,7 @@ struct CapLite {  // the cap fields a pixel test reads (CapRec without its 32-by
 // so their 16-byte reads land on disjoint banks.
 struct VisS {
     double u, v, w, wo;
-    int32_t cls, ia, ib, pad0;
+    int32_t cls, ia, ib, pad0;  // cls: class against the block's chunk h in bits 2h, 2h+1
     CapLite a, b;
 };
 static_assert(sizeof(VisS) == 88, "staged visibility record layout");
@@ -462,7 +465,8 @@ __device__ __forceinline__ void vis_raw_load(const GeomDev &g, int64_t vv, VisRa
 }
 
 __device__ __forceinline__ void vis_stage_store(const GeomDev &g, const PhaseKnobs &k,
-                                                const ChunkRec &ch, const VisRaw &r, VisS &e) {
+                                                const ChunkRec &ch0, const ChunkRec &ch1,
+                                                const VisRaw &r, VisS &e) {
     if (r.ok) {
         e.ia = r.cap.x;
         e.ib = r.cap.y;
@@ -471,28 +475,29 @@ __device__ __forceinline__ void vis_stage_store(const GeomDev &g, const PhaseKno
         e.b = cap_lite(b);
         e.u = r.q.x * k.s; e.v = r.q.y * k.s; e.w = r.q.z * k.s;
         e.wo = k.wfac ? 0.0 : -e.w;
-        e.cls = classify_m(ch, a, b);
+        e.cls = classify_m(ch0, a, b) | (classify_m(ch1, a, b) << 2);
     } else {
         e.ia = e.ib = 0;
         const CapLite none = {0.f, 0.f, 0.f, 2.0f, 0.f};
         e.a = none;
         e.b = none;
         e.u = e.v = e.w = e.wo = 0.0;
-        e.cls = kIn;  // padding: contributes nothing
+        e.cls = kIn | (kIn << 2);  // padding: contributes nothing
     }
 }
 
-__global__ void __launch_bounds__(kThreads, 4) mfma_recon_kernel(
+__global__ void __launch_bounds__(kThreadsR, 4) mfma_recon_kernel(
     GeomDev g, PhaseKnobs k, int32_t n_maps, int64_t n_vis_pad, const _Float16 *__restrict__ A16,
     const float *__restrict__ inv_scale, int32_t n_seg, float *__restrict__ part) {
     __shared__ _Float16 sA[2][4][kMaps][kStage];  // [buffer][re hi, re lo, im hi, im lo][map][vis slot]
     __shared__ VisS sV[2][kStage];
-    __shared__ int sAllIn[2];  // [buffer] every staged visibility shades the block's whole chunk
+    __shared__ int sAllIn[2];  // [buffer] bit h: every staged visibility shades chunk h
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
     const int col = lane & 31;
     const int kg = lane >> 5;
-    const int64_t n_ptiles = g.n_pad / (kWaves * kNT);  // n_pad is a multiple of 256
+    const int half = wave / (kWavesR / 2);  // which of the block's two chunks
+    const int64_t n_ptiles = (g.n_pad + kWavesR * kNT - 1) / (kWavesR * kNT);
     const int64_t n_groups = (n_maps + kMaps - 1) / kMaps;
     const int64_t per_seg = (n_vis_pad + n_seg - 1) / n_seg;
     const int64_t seg_len = (per_seg + kStage - 1) / kStage * kStage;
@@ -503,10 +508,13 @@ __global__ void __launch_bounds__(kThreads, 4) mfma_recon_kernel(
         const int64_t rest = item / n_seg;
         const int64_t pt = rest % n_ptiles;
         const int64_t grp = rest / n_ptiles;
-        const int64_t pi = pt * (kWaves * kNT) + wave * kNT + col;  // chunk-order pixel
-        const double4 s = g.pix64[pi];
-        // the block's pixels are one chunk
-        const ChunkRec ch = g.chunk[(pt * (kWaves * kNT)) / kChunk];
+        const int64_t pi = pt * (kWavesR * kNT) + wave * kNT + col;  // chunk-order pixel
+        // an odd chunk count leaves the last block's second chunk empty
+        const int64_t c0 = (pt * (kWavesR * kNT)) / kChunk;
+        const bool live = c0 + half < g.n_chunk;  // wave-uniform
+        const double4 s = live ? g.pix64[pi] : make_double4(0.0, 0.0, 0.0, 0.0);
+        const ChunkRec ch0 = g.chunk[c0];
+        const ChunkRec ch1 = g.chunk[c0 + 1 < g.n_chunk ? c0 + 1 : c0];
         floatx16 acc[2];
         acc[0] = floatx16{};
         acc[1] = floatx16{};
@@ -517,12 +525,14 @@ __global__ void __launch_bounds__(kThreads, 4) mfma_recon_kernel(
         VisRaw raw;
         if (n_st > 0) {
             // Both buffers are free: every wave passed the previous item's last barrier.
-            stage_load_lds<4>(Ag, n_vis_pad, nullptr, vb / kStage, &sA[0][0][0][0], nullptr);
+            stage_load_lds<4, kWavesR>(Ag, n_vis_pad, nullptr, vb / kStage, &sA[0][0][0][0], nullptr);
             if (threadIdx.x < kStage) {  // wave 0
                 vis_raw_load(g, vb + threadIdx.x, raw);
-                vis_stage_store(g, k, ch, raw, sV[0][threadIdx.x]);
-                const bool all_in = __ballot(sV[0][threadIdx.x].cls == kIn) == ~0ull;
-                if (threadIdx.x == 0) sAllIn[0] = all_in;
+                vis_stage_store(g, k, ch0, ch1, raw, sV[0][threadIdx.x]);
+                const int cl = sV[0][threadIdx.x].cls;
+                const bool in0 = __ballot((cl & 3) == kIn) == ~0ull;
+                const bool in1 = __ballot((cl >> 2) == kIn) == ~0ull;
+                if (threadIdx.x == 0) sAllIn[0] = (int)in0 | ((int)in1 << 1);
             }
             wait_vm();
             __syncthreads();
@@ -533,11 +543,11 @@ __global__ void __launch_bounds__(kThreads, 4) mfma_recon_kernel(
             const int64_t v1 = vb + (st + 1) * kStage;
             if (more) {
                 if (threadIdx.x < kStage) vis_raw_load(g, v1 + threadIdx.x, raw);
-                stage_load_lds<4>(Ag, n_vis_pad, nullptr, v1 / kStage, &sA[buf ^ 1][0][0][0],
+                stage_load_lds<4, kWavesR>(Ag, n_vis_pad, nullptr, v1 / kStage, &sA[buf ^ 1][0][0][0],
                                   nullptr);
             }
-            // a stage whose visibilities all shade the chunk adds nothing: skip it
-            if (!__builtin_amdgcn_readfirstlane(sAllIn[buf])) {
+            // a stage whose visibilities all shade the wave's chunk adds nothing: skip it
+            if (live && !((__builtin_amdgcn_readfirstlane(sAllIn[buf]) >> half) & 1)) {
 #pragma unroll
                 for (int ks = 0; ks < kStage / kKS; ++ks) {
                     // E for this lane's pixel and 8 staged visibilities: straight-line
@@ -552,8 +562,9 @@ __global__ void __launch_bounds__(kThreads, 4) mfma_recon_kernel(
                         const float r = (float)__builtin_amdgcn_fract(ph);
                         cs[t] = __builtin_amdgcn_cosf(r);
                         sn[t] = __builtin_amdgcn_sinf(r);
-                        shm |= (uint32_t)(q.cls == kIn) << t;
-                        stm |= (uint32_t)(q.cls == kStraddle) << t;
+                        const int qc = (q.cls >> (2 * half)) & 3;
+                        shm |= (uint32_t)(qc == kIn) << t;
+                        stm |= (uint32_t)(qc == kStraddle) << t;
                     }
                     H8 bre_h, bre_l, bim_h, bim_l;
                     if (__builtin_amdgcn_readfirstlane(shm | stm)) {  // the classes are wave-uniform
@@ -597,21 +608,25 @@ __global__ void __launch_bounds__(kThreads, 4) mfma_recon_kernel(
             }
             // the other buffer was last read before the previous barrier
             if (more && threadIdx.x < kStage) {  // wave 0
-                vis_stage_store(g, k, ch, raw, sV[buf ^ 1][threadIdx.x]);
-                const bool all_in = __ballot(sV[buf ^ 1][threadIdx.x].cls == kIn) == ~0ull;
-                if (threadIdx.x == 0) sAllIn[buf ^ 1] = all_in;
+                vis_stage_store(g, k, ch0, ch1, raw, sV[buf ^ 1][threadIdx.x]);
+                const int cl = sV[buf ^ 1][threadIdx.x].cls;
+                const bool in0 = __ballot((cl & 3) == kIn) == ~0ull;
+                const bool in1 = __ballot((cl >> 2) == kIn) == ~0ull;
+                if (threadIdx.x == 0) sAllIn[buf ^ 1] = (int)in0 | ((int)in1 << 1);
             }
             wait_vm();
             __syncthreads();
         }
+        if (live) {
 #pragma unroll
-        for (int mt = 0; mt < 2; ++mt)
+            for (int mt = 0; mt < 2; ++mt)
 #pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int64_t m = grp * kMaps + mt * kMT + acc_row(r, lane);
-                if (m < n_maps)
-                    part[(m * n_seg + seg) * g.n_pad + pi] = acc[mt][r] * inv_scale[m];
-            }
+                for (int r = 0; r < 16; ++r) {
+                    const int64_t m = grp * kMaps + mt * kMT + acc_row(r, lane);
+                    if (m < n_maps)
+                        part[(m * n_seg + seg) * g.n_pad + pi] = acc[mt][r] * inv_scale[m];
+                }
+        }
     }
 }
 
@@ -662,7 +677,7 @@ hipError_t launch_mfma_recon(const GeomDev &g, const PhaseKnobs &k, int32_t n_ma
     hipLaunchKernelGGL(split_vis_kernel, dim3(2048), dim3(256), 0, st, g.n_vis, n_vis_pad, n_maps,
                        V, w, kappa, amax, (_Float16 *)a16, inv_scale);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    hipLaunchKernelGGL(mfma_recon_kernel, dim3(grid), dim3(kThreads), 0, st, g, k, n_maps,
+    hipLaunchKernelGGL(mfma_recon_kernel, dim3(grid), dim3(kThreadsR), 0, st, g, k, n_maps,
                        n_vis_pad, (const _Float16 *)a16, inv_scale, n_seg, part);
     return hipGetLastError();
 }
@@ -670,7 +685,8 @@ hipError_t launch_mfma_recon(const GeomDev &g, const PhaseKnobs &k, int32_t n_ma
 const void *mfma_visgen_kernel_ptr() { return (const void *)mfma_visgen_kernel; }
 const void *mfma_recon_kernel_ptr() { return (const void *)mfma_recon_kernel; }
 int mfma_block_threads() { return kThreads; }
+int mfma_recon_block_threads() { return kThreadsR; }
 int mfma_vis_per_block() { return kWaves * kNT; }
-int mfma_pix_per_block() { return kWaves * kNT; }
+int mfma_pix_per_block() { return kWavesR * kNT; }
 
 }  // namespace vrs

@@ -26,7 +26,8 @@ hipError_t launch_mfma_recon(const GeomDev &g, const PhaseKnobs &k, int32_t n_ma
 
 const void *mfma_visgen_kernel_ptr();
 const void *mfma_recon_kernel_ptr();
-int mfma_block_threads();
+int mfma_block_threads();         // vis-gen block
+int mfma_recon_block_threads();   // recon block
 int mfma_vis_per_block();
 int mfma_pix_per_block();
 
