@@ -486,12 +486,34 @@ __device__ __forceinline__ void vis_stage_store(const GeomDev &g, const PhaseKno
     }
 }
 
+// A wave-uniform 64-bit value kept in scalar registers.
+__device__ __forceinline__ uint64_t uniform64(uint64_t x) {
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)x);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(x >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// Wave 0, after staging: bit masks over the 64 staged visibilities of which
+// ones shade all of the block's chunk h, and which straddle it (h = 0, 1).
+__device__ __forceinline__ void stage_masks(int cls, uint64_t (&m)[2][2]) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int c = (cls >> (2 * h)) & 3;
+        const uint64_t in_m = __ballot(c == kIn), st_m = __ballot(c == kStraddle);
+        if ((threadIdx.x & 63) == 0) {
+            m[h][0] = in_m;
+            m[h][1] = st_m;
+        }
+    }
+}
+
 __global__ void __launch_bounds__(kThreadsR, 4) mfma_recon_kernel(
     GeomDev g, PhaseKnobs k, int32_t n_maps, int64_t n_vis_pad, const _Float16 *__restrict__ A16,
     const float *__restrict__ inv_scale, int32_t n_seg, float *__restrict__ part) {
     __shared__ _Float16 sA[2][4][kMaps][kStage];  // [buffer][re hi, re lo, im hi, im lo][map][vis slot]
     __shared__ VisS sV[2][kStage];
-    __shared__ int sAllIn[2];  // [buffer] bit h: every staged visibility shades chunk h
+    // [buffer][chunk h][0: shades all of chunk h, 1: straddles it], one bit per staged visibility
+    __shared__ uint64_t sMask[2][2][2];
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
     const int col = lane & 31;
@@ -529,10 +551,7 @@ __global__ void __launch_bounds__(kThreadsR, 4) mfma_recon_kernel(
             if (threadIdx.x < kStage) {  // wave 0
                 vis_raw_load(g, vb + threadIdx.x, raw);
                 vis_stage_store(g, k, ch0, ch1, raw, sV[0][threadIdx.x]);
-                const int cl = sV[0][threadIdx.x].cls;
-                const bool in0 = __ballot((cl & 3) == kIn) == ~0ull;
-                const bool in1 = __ballot((cl >> 2) == kIn) == ~0ull;
-                if (threadIdx.x == 0) sAllIn[0] = (int)in0 | ((int)in1 << 1);
+                stage_masks(sV[0][threadIdx.x].cls, sMask[0]);
             }
             wait_vm();
             __syncthreads();
@@ -546,15 +565,17 @@ __global__ void __launch_bounds__(kThreadsR, 4) mfma_recon_kernel(
                 stage_load_lds<4, kWavesR>(Ag, n_vis_pad, nullptr, v1 / kStage, &sA[buf ^ 1][0][0][0],
                                   nullptr);
             }
+            const uint64_t in_m = uniform64(sMask[buf][half][0]), st_m = uniform64(sMask[buf][half][1]);
             // a stage whose visibilities all shade the wave's chunk adds nothing: skip it
-            if (live && !((__builtin_amdgcn_readfirstlane(sAllIn[buf]) >> half) & 1)) {
+            if (live && in_m != ~0ull) {
 #pragma unroll
                 for (int ks = 0; ks < kStage / kKS; ++ks) {
                     // E for this lane's pixel and 8 staged visibilities: straight-line
-                    // phase and sin/cos, then the (wave-uniform per visibility) classes.
+                    // phase and sin/cos, then the classes (the same across each half-wave).
                     const int kb = ks * kKS + kg * 8;
                     float cs[8], sn[8];
-                    uint32_t shm = 0, stm = 0;
+                    uint32_t shm = (uint32_t)(in_m >> kb) & 0xffu;
+                    uint32_t stm = (uint32_t)(st_m >> kb) & 0xffu;
 #pragma unroll
                     for (int t = 0; t < 8; ++t) {
                         const VisS &q = sV[buf][kb + t];
@@ -562,12 +583,9 @@ __global__ void __launch_bounds__(kThreadsR, 4) mfma_recon_kernel(
                         const float r = (float)__builtin_amdgcn_fract(ph);
                         cs[t] = __builtin_amdgcn_cosf(r);
                         sn[t] = __builtin_amdgcn_sinf(r);
-                        const int qc = (q.cls >> (2 * half)) & 3;
-                        shm |= (uint32_t)(qc == kIn) << t;
-                        stm |= (uint32_t)(qc == kStraddle) << t;
                     }
                     H8 bre_h, bre_l, bim_h, bim_l;
-                    if (__builtin_amdgcn_readfirstlane(shm | stm)) {  // the classes are wave-uniform
+                    if (__ballot((shm | stm) != 0)) {  // either half-wave has a shaded entry
                         while (stm) {
                             const int t = __ffs(stm) - 1;
                             stm &= stm - 1;
@@ -609,10 +627,7 @@ __global__ void __launch_bounds__(kThreadsR, 4) mfma_recon_kernel(
             // the other buffer was last read before the previous barrier
             if (more && threadIdx.x < kStage) {  // wave 0
                 vis_stage_store(g, k, ch0, ch1, raw, sV[buf ^ 1][threadIdx.x]);
-                const int cl = sV[buf ^ 1][threadIdx.x].cls;
-                const bool in0 = __ballot((cl & 3) == kIn) == ~0ull;
-                const bool in1 = __ballot((cl >> 2) == kIn) == ~0ull;
-                if (threadIdx.x == 0) sAllIn[buf ^ 1] = (int)in0 | ((int)in1 << 1);
+                stage_masks(sV[buf ^ 1][threadIdx.x].cls, sMask[buf ^ 1]);
             }
             wait_vm();
             __syncthreads();
