@@ -291,26 +291,39 @@ __global__ void __launch_bounds__(256) to_chunk_order_kernel(GeomDev g, int32_t 
 // uvw = (r_b - r_a) * inv_lambda[map], formed exactly as stage 1 forms it.
 constexpr int kListPad = 32;
 
+// FP32 cap test that ORs the wave's guard-band pixels into a lane mask (kept
+// in scalar registers) instead of a per-lane flag.
+__device__ __forceinline__ bool occ32w(float sx, float sy, float sz, const CapRec &a,
+                                       uint64_t &unc_w) {
+    const float x = fmaf(sz, a.dz, fmaf(sy, a.dy, sx * a.dx));
+    unc_w |= __ballot(fabsf(x - a.c) <= a.g);
+    return x >= a.c;
+}
+
 // One straddling pair over this lane's pixels: the FP32 test of cap A (and of
-// cap B when TWO) pixel by pixel, T_occ where shaded, I elsewhere; `unc` flags
-// pixels inside a tested cap's guard band.
+// cap B when TWO) pixel by pixel, T_occ where shaded, I elsewhere.  Returns the
+// lanes with a pixel inside a tested cap's guard band (called by the whole wave).
 template <bool TWO, typename T>
-__device__ __forceinline__ void straddle_terms(const T (&l)[kPixPerLane], const T (&m)[kPixPerLane],
-                                               const T (&n)[kPixPerLane],
-                                               const float (&I)[kPixPerLane], const CapRec &A,
-                                               const CapRec &B, T u, T vv, T w, T wo,
-                                               float t_occ, float &are, float &aim, bool &unc) {
+__device__ __forceinline__ uint64_t straddle_terms(const T (&l)[kPixPerLane],
+                                                   const T (&m)[kPixPerLane],
+                                                   const T (&n)[kPixPerLane],
+                                                   const float (&I)[kPixPerLane],
+                                                   const CapRec &A, const CapRec &B, T u, T vv,
+                                                   T w, T wo, float t_occ, float &are,
+                                                   float &aim) {
+    uint64_t unc_w = 0;
 #pragma unroll
     for (int j = 0; j < kPixPerLane; ++j) {
         const float sx = (float)l[j], sy = (float)m[j], sz = (float)n[j];
-        bool sh = occ32(sx, sy, sz, A, unc);
-        if (TWO) sh = occ32(sx, sy, sz, B, unc) | sh;
+        bool sh = occ32w(sx, sy, sz, A, unc_w);
+        if (TWO) sh = occ32w(sx, sy, sz, B, unc_w) | sh;
         const float ie = sh ? t_occ : I[j];
         const T ph = fma(u, l[j], fma(vv, m[j], fma(w, n[j], wo)));
         const float r = fract_rev(ph);
         are = fmaf(ie, __builtin_amdgcn_cosf(r), are);
         aim = fmaf(ie, __builtin_amdgcn_sinf(r), aim);
     }
+    return unc_w;
 }
 
 template <bool F64> struct PhaseT { using T = float; };
@@ -474,15 +487,15 @@ __global__ void __launch_bounds__(kBlock, (!F64 && VT == 32) ? 3 : 4) visgen_ker
                         }
                     } else {                         // straddling: FP32 cap test per pixel
                         const bool ta = (sa_m >> q) & 1u, tb = (sb_m >> q) & 1u;
-                        bool unc = false;
-                        if (ta && tb)
-                            straddle_terms<true>(l, m, n, I, ents[q].a, ents[q].b, u, vv, w, wo,
-                                                 t_occ, are[q], aim[q], unc);
-                        else  // one cap straddles; the other is wholly clear of the chunk
-                            straddle_terms<false>(l, m, n, I, ta ? ents[q].a : ents[q].b,
-                                                  ents[q].b, u, vv, w, wo, t_occ, are[q],
-                                                  aim[q], unc);
-                        if (__ballot(unc)) unc_m |= 1u << q;
+                        const uint64_t unc_w =
+                            (ta && tb)
+                                ? straddle_terms<true>(l, m, n, I, ents[q].a, ents[q].b, u, vv, w,
+                                                       wo, t_occ, are[q], aim[q])
+                                // one cap straddles; the other is wholly clear of the chunk
+                                : straddle_terms<false>(l, m, n, I, ta ? ents[q].a : ents[q].b,
+                                                        ents[q].b, u, vv, w, wo, t_occ, are[q],
+                                                        aim[q]);
+                        if (unc_w) unc_m |= 1u << q;
                     }
                 }
                 // Pixels inside the guard band: replace the FP32 decision by the
