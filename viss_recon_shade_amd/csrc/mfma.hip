@@ -25,13 +25,19 @@ namespace {
 
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 constexpr int kMT = 32;            // rows of one M tile (maps)
 constexpr int kMaps = 64;          // maps per pass (two M tiles)
 constexpr int kNT = 32;            // columns per wave (visibilities or pixels)
 constexpr int kKS = 16;            // K per MFMA
 constexpr int kStage = 64;         // K per LDS stage
-constexpr int kWaves = 8;         // vis-gen block: one per CU, two waves per SIMD
+// vis-gen: v_mfma_f32_16x16x32_f16, 16 maps x 16 visibilities x 32 pixels per
+// instruction; 16 waves x 16 visibilities per block, four waves per SIMD.
+constexpr int kMTV = 16;          // rows of one vis-gen M tile (maps)
+constexpr int kNTV = 16;          // visibilities per vis-gen wave
+constexpr int kKSV = 32;          // K (pixels) per vis-gen MFMA
+constexpr int kWaves = 16;        // vis-gen block: one per CU
 constexpr int kThreads = kWaves * 64;
 constexpr int kWavesR = 16;       // recon block: two chunks, one block per CU, four waves per SIMD
 constexpr int kThreadsR = kWavesR * 64;
@@ -41,6 +47,9 @@ static_assert(kWavesR * kNT == 2 * kChunk, "a recon block covers two chunks");
 
 __device__ __forceinline__ floatx16 mfma(half8 a, half8 b, floatx16 c) {
     return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ floatx4 mfma16(half8 a, half8 b, floatx4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
 }
 
 // x = hi + lo with hi, lo in FP16 (hi rounded toward zero, lo the remainder).
@@ -231,8 +240,9 @@ __device__ __forceinline__ int acc_row(int r, int lane) {
 }
 
 // ------------------------------------------------------------------ vis-gen
-// Block: 8 waves x 32 visibilities; all waves share the staged map tile (each
-// staged tile serves 256 visibilities, halving the map operand's re-reads).
+// Block: 16 waves x 16 visibilities; all waves share the staged map tile (each
+// staged tile serves 256 visibilities).  The 16x16x32 instruction keeps the
+// complex accumulators of 64 maps in 32 registers, so four waves fit a SIMD.
 // Stages of kStage pixels are double-buffered: the next stage's map tile and
 // pixel vectors go straight from global memory into the other LDS buffer
 // while the current stage computes (one barrier per stage).
@@ -281,17 +291,17 @@ __device__ __forceinline__ void stage_load_lds(const _Float16 *__restrict__ Ag, 
 
 __device__ __forceinline__ void wait_vm() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 
-__global__ void __launch_bounds__(kThreads, 1) mfma_visgen_kernel(
+__global__ void __launch_bounds__(kThreads) mfma_visgen_kernel(
     GeomDev g, PhaseKnobs k, int32_t n_maps, const _Float16 *__restrict__ A16,
     const float *__restrict__ inv_scale, float2 *__restrict__ V) {
     __shared__ _Float16 sA[2][2][kMaps][kStage];  // [buffer][hi/lo][map][pixel slot]
     __shared__ double4 sP[2][kStage];             // [buffer] pixel (l, m, n) FP64
-    __shared__ CapRec sCap[kWaves * kNT][2];      // caps of the block's visibilities
+    __shared__ CapRec sCap[kWaves * kNTV][2];     // caps of the block's visibilities
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
-    const int col = lane & 31;                 // this lane's visibility within the wave tile
-    const int kg = lane >> 5;                  // which 8 of the 16 K values this lane supplies
-    const int64_t n_vtiles = (g.n_vis + kWaves * kNT - 1) / (kWaves * kNT);
+    const int col = lane & 15;                 // this lane's visibility within the wave tile
+    const int kg = lane >> 4;                  // which 8 of the 32 K values this lane supplies
+    const int64_t n_vtiles = (g.n_vis + kWaves * kNTV - 1) / (kWaves * kNTV);
     const int64_t n_groups = (n_maps + kMaps - 1) / kMaps;
     const int64_t n_tiles = n_vtiles * n_groups;
     const int64_t n_stages = g.n_pad / kStage;
@@ -299,12 +309,12 @@ __global__ void __launch_bounds__(kThreads, 1) mfma_visgen_kernel(
 
     for (int64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
         const int64_t grp = tile / n_vtiles;
-        const int64_t v = (tile % n_vtiles) * (kWaves * kNT) + wave * kNT + col;
+        const int64_t v = (tile % n_vtiles) * (kWaves * kNTV) + wave * kNTV + col;
         const bool vok = v < g.n_vis;
         // this lane's visibility
         double u = 0.0, vv = 0.0, w = 0.0;
         int ia = 0, ib = 0;
-        const int vc = wave * kNT + col;        // this lane's visibility within the block
+        const int vc = wave * kNTV + col;       // this lane's visibility within the block
         if (vok) {
             const int2 vcap = g.vis_cap[v];
             ia = vcap.x;
@@ -320,11 +330,11 @@ __global__ void __launch_bounds__(kThreads, 1) mfma_visgen_kernel(
             sCap[vc][1] = vok ? g.cap[ib] : none;
         }
         const double wo = k.wfac ? 0.0 : -w;
-        floatx16 acc[2][2];  // [m tile][re, im]
+        floatx4 acc[kMaps / kMTV][2];  // [m tile][re, im]
 #pragma unroll
-        for (int a = 0; a < 2; ++a)
+        for (int a = 0; a < kMaps / kMTV; ++a)
 #pragma unroll
-            for (int c = 0; c < 2; ++c) acc[a][c] = floatx16{};
+            for (int c = 0; c < 2; ++c) acc[a][c] = floatx4{};
         float gre = 0.0f, gim = 0.0f;  // sum of E over shaded pixels
         int cls = kOut;
         bool any_shade = false;
@@ -351,12 +361,11 @@ __global__ void __launch_bounds__(kThreads, 1) mfma_visgen_kernel(
             // the other buffer was last read before the previous barrier
             if (st + 1 < n_stages)
                 stage_load_lds<2, kWaves>(Ag, g.n_pad, g.pix64, st + 1, &sA[buf ^ 1][0][0][0],
-                                  &sP[buf ^ 1][0]);
-#pragma unroll 2
-            for (int ks = 0; ks < kStage / kKS; ++ks) {
+                                          &sP[buf ^ 1][0]);
+            for (int ks = 0; ks < kStage / kKSV; ++ks) {
                 // E for this lane's 8 pixels: straight-line phase and sin/cos
                 // first, then the shade mask, so that the 8 chains interleave.
-                const int kb = ks * kKS + kg * 8;
+                const int kb = ks * kKSV + kg * 8;
                 float cs[8], sn[8];
 #pragma unroll
                 for (int t = 0; t < 8; ++t) {
@@ -390,31 +399,34 @@ __global__ void __launch_bounds__(kThreads, 1) mfma_visgen_kernel(
                         split2(sn[2 * t2], sn[2 * t2 + 1], bim_h.u[t2], bim_l.u[t2]);
                     }
 #pragma unroll
-                    for (int mt = 0; mt < 2; ++mt) {
-                        const int row = mt * kMT + col, so = a_slot(row, ks * 2 + kg) * 8;
+                    for (int mt = 0; mt < kMaps / kMTV; ++mt) {
+                        const int row = mt * kMTV + col, so = a_slot(row, ks * 4 + kg) * 8;
                         const half8 ah = *reinterpret_cast<const half8 *>(&sA[buf][0][row][so]);
                         const half8 al = *reinterpret_cast<const half8 *>(&sA[buf][1][row][so]);
-                        acc[mt][0] = mfma(ah, bre_h.v, acc[mt][0]);
-                        acc[mt][0] = mfma(ah, bre_l.v, acc[mt][0]);
-                        acc[mt][0] = mfma(al, bre_h.v, acc[mt][0]);
-                        acc[mt][1] = mfma(ah, bim_h.v, acc[mt][1]);
-                        acc[mt][1] = mfma(ah, bim_l.v, acc[mt][1]);
-                        acc[mt][1] = mfma(al, bim_h.v, acc[mt][1]);
+                        acc[mt][0] = mfma16(ah, bre_h.v, acc[mt][0]);
+                        acc[mt][0] = mfma16(ah, bre_l.v, acc[mt][0]);
+                        acc[mt][0] = mfma16(al, bre_h.v, acc[mt][0]);
+                        acc[mt][1] = mfma16(ah, bim_h.v, acc[mt][1]);
+                        acc[mt][1] = mfma16(ah, bim_l.v, acc[mt][1]);
+                        acc[mt][1] = mfma16(al, bim_h.v, acc[mt][1]);
                     }
                 }
             }
             wait_vm();
             __syncthreads();
         }
-        // shaded-pixel term: both K halves of this column
+        // shaded-pixel term: the four K groups of this column
+        gre += __shfl_xor(gre, 16, 64);
+        gim += __shfl_xor(gim, 16, 64);
         gre += __shfl_xor(gre, 32, 64);
         gim += __shfl_xor(gim, 32, 64);
         if (vok) {
 #pragma unroll
-            for (int mt = 0; mt < 2; ++mt)
+            for (int mt = 0; mt < kMaps / kMTV; ++mt)
 #pragma unroll
-                for (int r = 0; r < 16; ++r) {
-                    const int64_t m = grp * kMaps + mt * kMT + acc_row(r, lane);
+                for (int r = 0; r < 4; ++r) {
+                    // D of 16x16x32: lane holds rows 4 * (lane / 16) + r of column lane % 16
+                    const int64_t m = grp * kMaps + mt * kMTV + 4 * kg + r;
                     if (m < n_maps) {
                         const float is = inv_scale[m];
                         V[m * g.n_vis + v] = make_float2(acc[mt][0][r] * is + t_occ * gre,
@@ -701,7 +713,7 @@ const void *mfma_visgen_kernel_ptr() { return (const void *)mfma_visgen_kernel; 
 const void *mfma_recon_kernel_ptr() { return (const void *)mfma_recon_kernel; }
 int mfma_block_threads() { return kThreads; }
 int mfma_recon_block_threads() { return kThreadsR; }
-int mfma_vis_per_block() { return kWaves * kNT; }
+int mfma_vis_per_block() { return kWaves * kNTV; }
 int mfma_pix_per_block() { return kWavesR * kNT; }
 
 }  // namespace vrs
