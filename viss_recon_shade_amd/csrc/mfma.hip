@@ -13,7 +13,9 @@
 // products per tile (hi*hi + hi*lo + lo*hi) into FP32 accumulators.  The map
 // operand (I, or the weighted V) is pre-split the same way, scaled per map by
 // a power of two into FP16 range, and staged through LDS for all waves of the
-// block.  Instruction: v_mfma_f32_32x32x16_f16 (M = 32 maps, N = 32, K = 16).
+// block.  Instructions: vis-gen v_mfma_f32_16x16x32_f16 (M = 16 maps, N = 16
+// visibilities, K = 32 pixels), recon v_mfma_f32_32x32x16_f16 (M = 32 maps,
+// N = 32 pixels, K = 16 visibilities).
 #include <hip/hip_runtime.h>
 
 #include "kernels.hpp"
@@ -28,7 +30,7 @@ typedef float floatx16 __attribute__((ext_vector_type(16)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 constexpr int kMT = 32;            // rows of one M tile (maps)
-constexpr int kMaps = 64;          // maps per pass (two M tiles)
+constexpr int kMaps = 64;          // maps per pass
 constexpr int kNT = 32;            // columns per wave (visibilities or pixels)
 constexpr int kKS = 16;            // K per MFMA
 constexpr int kStage = 64;         // K per LDS stage
