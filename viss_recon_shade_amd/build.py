@@ -31,8 +31,9 @@ DEVICE = ["-x", "hip", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "
 # Machine scheduler strategies, chosen by timing on an MI355X (c2 plain, c2
 # FP64-phase and c3 64-map kernels; every default-strategy kernel was 0.4-2.4 %
 # slower): max-ilp for the VALU-bound DFT kernels, iterative-ilp for the
-# matrix-core kernels (two waves per SIMD: instruction-level parallelism inside
-# a wave is what overlaps their VALU work with the MFMA chains).
+# matrix-core kernels (four waves per SIMD with long dependent MFMA chains:
+# instruction-level parallelism inside a wave is what overlaps their VALU work
+# with those chains).
 EXTRA = {"kernels.hip": ["-mllvm", "-disable-promote-alloca-to-vector",
                          "-mllvm", "--amdgpu-sched-strategy=max-ilp"],
          "mfma.hip": ["-mllvm", "--amdgpu-sched-strategy=iterative-ilp"]}
