@@ -198,8 +198,9 @@ __global__ void __launch_bounds__(256) split_maps_kernel(GeomDev g, int32_t n_ma
     }
 }
 
-// V [n_maps][n_vis] complex -> hi/lo FP16 [group][re,im][hi,lo][kMaps][n_vis_pad],
-// weighted by w * kappa and scaled per map.
+// V [n_maps][n_vis] complex -> hi/lo FP16 [group][re,-im][hi,lo][kMaps][n_vis_pad],
+// weighted by w * kappa and scaled per map.  The imaginary rows hold -Im V so
+// that Re(V e^{+i theta}) = Re V cos + (-Im V) sin is a plain sum of products.
 __global__ void __launch_bounds__(256) split_vis_kernel(int64_t n_vis, int64_t n_vis_pad,
                                                         int32_t n_maps,
                                                         const float2 *__restrict__ V,
@@ -222,7 +223,7 @@ __global__ void __launch_bounds__(256) split_vis_kernel(int64_t n_vis, int64_t n
                 const float2 q = V[m * n_vis + v];
                 const float ww = (w ? w[v] : 1.0f) * kappa * sc;
                 xr = q.x * ww;
-                xi = q.y * ww;
+                xi = -(q.y * ww);
             }
             if (v == 0) inv_scale[m] = 1.0f / sc;
         }
@@ -231,8 +232,8 @@ __global__ void __launch_bounds__(256) split_vis_kernel(int64_t n_vis, int64_t n
         const int64_t base = grp * 4 * kMaps;
         out[((base + 0 * kMaps) + row) * n_vis_pad + v] = hr;  // re hi
         out[((base + 1 * kMaps) + row) * n_vis_pad + v] = lr;  // re lo
-        out[((base + 2 * kMaps) + row) * n_vis_pad + v] = hi_; // im hi
-        out[((base + 3 * kMaps) + row) * n_vis_pad + v] = li;  // im lo
+        out[((base + 2 * kMaps) + row) * n_vis_pad + v] = hi_; // -im hi
+        out[((base + 3 * kMaps) + row) * n_vis_pad + v] = li;  // -im lo
     }
 }
 
@@ -337,9 +338,9 @@ __global__ void __launch_bounds__(kThreads) mfma_visgen_kernel(
         for (int a = 0; a < kMaps / kMTV; ++a)
 #pragma unroll
             for (int c = 0; c < 2; ++c) acc[a][c] = floatx4{};
-        float gre = 0.0f, gim = 0.0f;  // sum of E over shaded pixels
+        float gre = 0.0f, gim = 0.0f;  // sum of (cos, sin) over shaded pixels
         int cls = kOut;
-        bool any_shade = false;
+        bool mixed = false;   // some lane's pair shades part of the chunk, or lanes differ
         bool all_in = false;  // every visibility of the wave shades the whole chunk: E' = 0
 
         const _Float16 *Ag = A16 + grp * 2 * kMaps * g.n_pad;
@@ -357,8 +358,8 @@ __global__ void __launch_bounds__(kThreads) mfma_visgen_kernel(
             if (new_chunk) ch = g.chunk[p0 / kChunk];
             if (new_chunk) {
                 cls = classify_m(ch, sCap[vc][0], sCap[vc][1]);
-                any_shade = __ballot(cls != kOut) != 0;
                 all_in = __ballot(cls == kIn) == ~0ull;
+                mixed = !all_in && __ballot(cls != kOut) != 0;
             }
             // the other buffer was last read before the previous barrier
             if (st + 1 < n_stages)
@@ -377,8 +378,10 @@ __global__ void __launch_bounds__(kThreads) mfma_visgen_kernel(
                     cs[t] = __builtin_amdgcn_cosf(r);
                     sn[t] = __builtin_amdgcn_sinf(r);
                 }
+                // The B operand holds (cos, sin) of E = cos - i sin; the sign of the
+                // imaginary part is applied once, at the output.
                 H8 bre_h, bre_l, bim_h, bim_l;
-                if (any_shade) {  // wave-uniform: some lane's pair is not wholly clear
+                if (mixed) {  // wave-uniform: shaded and clear pixels both occur
                     uint32_t shm = (cls == kIn) ? 0xffu : 0u;
                     if (cls == kStraddle)
                         shm = straddle_mask8(g, &sP[buf][kb], sCap[vc][0], sCap[vc][1], ia, ib);
@@ -388,11 +391,18 @@ __global__ void __launch_bounds__(kThreads) mfma_visgen_kernel(
                         gre += sh ? cs[t] : 0.0f;
                         gim += sh ? sn[t] : 0.0f;
                         cs[t] = sh ? 0.0f : cs[t];
-                        sn[t] = sh ? 0.0f : -sn[t];   // E = cos - i sin
+                        sn[t] = sh ? 0.0f : sn[t];
                     }
-                } else {
+                } else if (all_in) {
+                    // every pixel is shaded for every lane: E' = 0, only the shaded sums
+                    float sr = cs[0], si = sn[0];
 #pragma unroll
-                    for (int t = 0; t < 8; ++t) sn[t] = -sn[t];
+                    for (int t = 1; t < 8; ++t) {
+                        sr += cs[t];
+                        si += sn[t];
+                    }
+                    gre += sr;
+                    gim += si;
                 }
                 if (!all_in) {  // wave-uniform
 #pragma unroll
@@ -431,8 +441,9 @@ __global__ void __launch_bounds__(kThreads) mfma_visgen_kernel(
                     const int64_t m = grp * kMaps + mt * kMTV + 4 * kg + r;
                     if (m < n_maps) {
                         const float is = inv_scale[m];
+                        // V = sum I E' + T_occ sum_shaded E, E = cos - i sin
                         V[m * g.n_vis + v] = make_float2(acc[mt][0][r] * is + t_occ * gre,
-                                                         acc[mt][1][r] * is - t_occ * gim);
+                                                         -(acc[mt][1][r] * is) - t_occ * gim);
                     }
                 }
         }
@@ -586,10 +597,12 @@ __global__ void __launch_bounds__(kThreadsR, 4) mfma_recon_kernel(
                 for (int ks = 0; ks < kStage / kKS; ++ks) {
                     // E for this lane's pixel and 8 staged visibilities: straight-line
                     // phase and sin/cos, then the classes (the same across each half-wave).
-                    const int kb = ks * kKS + kg * 8;
+                    // Lanes 0-31 take staged visibilities kb0 .. kb0 + 7, lanes 32-63
+                    // the next 8; their classes are bits of the wave-uniform masks.
+                    const int kb0 = ks * kKS, kb = kb0 + kg * 8;
+                    const uint32_t in16 = (uint32_t)(in_m >> kb0) & 0xffffu;
+                    const uint32_t st16 = (uint32_t)(st_m >> kb0) & 0xffffu;
                     float cs[8], sn[8];
-                    uint32_t shm = (uint32_t)(in_m >> kb) & 0xffu;
-                    uint32_t stm = (uint32_t)(st_m >> kb) & 0xffu;
 #pragma unroll
                     for (int t = 0; t < 8; ++t) {
                         const VisS &q = sV[buf][kb + t];
@@ -599,23 +612,34 @@ __global__ void __launch_bounds__(kThreadsR, 4) mfma_recon_kernel(
                         sn[t] = __builtin_amdgcn_sinf(r);
                     }
                     H8 bre_h, bre_l, bim_h, bim_l;
-                    if (__ballot((shm | stm) != 0)) {  // either half-wave has a shaded entry
-                        while (stm) {
-                            const int t = __ffs(stm) - 1;
-                            stm &= stm - 1;
-                            const VisS &q = sV[buf][kb + t];
-                            if (pair_shaded(g, s, cap_full(q.a), cap_full(q.b), q.ia, q.ib))
-                                shm |= 1u << t;
+                    if ((in16 | st16) != 0) {  // wave-uniform: some entry is shaded
+                        uint64_t lm[8];  // lanes for which entry t is shaded
+                        if (st16 != 0) {  // some entry straddles: test pixel by pixel
+                            uint32_t shm = (in16 >> (8 * kg)) & 0xffu;
+                            uint32_t stm = (st16 >> (8 * kg)) & 0xffu;
+                            while (stm) {
+                                const int t = __ffs(stm) - 1;
+                                stm &= stm - 1;
+                                const VisS &q = sV[buf][kb + t];
+                                if (pair_shaded(g, s, cap_full(q.a), cap_full(q.b), q.ia, q.ib))
+                                    shm |= 1u << t;
+                            }
+#pragma unroll
+                            for (int t = 0; t < 8; ++t) lm[t] = __ballot((shm >> t) & 1u);
+                        } else {
+                            // entry t is shaded for a whole half-wave or not at all: the
+                            // lane mask is formed on the scalar unit
+#pragma unroll
+                            for (int t = 0; t < 8; ++t)
+                                lm[t] = (((in16 >> t) & 1u) ? 0x00000000ffffffffull : 0ull) |
+                                        (((in16 >> (8 + t)) & 1u) ? 0xffffffff00000000ull : 0ull);
                         }
 #pragma unroll
                         for (int t = 0; t < 8; ++t) {
-                            const bool sh = (shm >> t) & 1u;
+                            const bool sh = __builtin_amdgcn_inverse_ballot_w64(lm[t]);
                             cs[t] = sh ? 0.0f : cs[t];
-                            sn[t] = sh ? 0.0f : -sn[t];   // Re(V e^{+i theta}) = Vr cos - Vi sin
+                            sn[t] = sh ? 0.0f : sn[t];
                         }
-                    } else {
-#pragma unroll
-                        for (int t = 0; t < 8; ++t) sn[t] = -sn[t];
                     }
 #pragma unroll
                     for (int t2 = 0; t2 < 4; ++t2) {
