@@ -109,14 +109,15 @@ __device__ __forceinline__ bool pair_shaded(const GeomDev &g, const double4 s64,
     return sh;
 }
 
-// Shade bits of 8 consecutive staged pixels for one pair (bit t: pixel t):
+// Shade bits of 8 staged pixels sp[S t], t = 0..7, for one pair (bit t: pixel S t):
 // FP32 cap tests, the exact FP64 predicate for pixels in the guard band.
+template <int S>
 __device__ __forceinline__ uint32_t straddle_mask8(const GeomDev &g, const double4 *sp,
                                                 const CapRec &A, const CapRec &B, int ia, int ib) {
     uint32_t m = 0, um = 0;
 #pragma unroll
     for (int t = 0; t < 8; ++t) {
-        const double4 s = sp[t];
+        const double4 s = sp[S * t];
         const float sx = (float)s.x, sy = (float)s.y, sz = (float)s.z;
         bool unc = false;
         const bool sh = occ32m(sx, sy, sz, A, unc) | occ32m(sx, sy, sz, B, unc);
@@ -126,7 +127,7 @@ __device__ __forceinline__ uint32_t straddle_mask8(const GeomDev &g, const doubl
     while (um) {
         const int t = __ffs(um) - 1;
         um &= um - 1;
-        const double4 s = sp[t];
+        const double4 s = sp[S * t];
         const bool ex = occ64m(g.pos[ia], s.x, s.y, s.z, g.limb) |
                         occ64m(g.pos[ib], s.x, s.y, s.z, g.limb);
         m = (m & ~(1u << t)) | ((uint32_t)ex << t);
@@ -170,7 +171,17 @@ __device__ __forceinline__ float row_scale(unsigned int amax_bits, float extra) 
     return ldexpf(1.0f, 14 - e);  // m * scale < 2^14
 }
 
-// I [n_maps][n_pix] RING -> hi/lo FP16 [group][2][kMaps][n_pad] chunk order, scaled.
+// K order of the vis-gen map operand inside each K-step of 32 pixels: position
+// 8 g + t holds pixel 4 t + g.  Lane group g of a wave (lanes 16 g .. 16 g + 15)
+// supplies K positions 8 g .. 8 g + 7, so its E entry t is pixel 4 t + g: the
+// four groups read four consecutive 32-byte pixel slots, which lie on distinct
+// LDS banks (8 pixels apart they would share the same banks).
+__device__ __forceinline__ int64_t kstep_pos(int64_t p) {
+    return (p & ~int64_t(31)) | ((p & 3) << 3) | ((p >> 2) & 7);
+}
+
+// I [n_maps][n_pix] RING -> hi/lo FP16 [group][2][kMaps][n_pad] chunk order
+// (permuted by kstep_pos inside each K-step), scaled.
 __global__ void __launch_bounds__(256) split_maps_kernel(GeomDev g, int32_t n_maps, int64_t n_pix,
                                                          const float *__restrict__ I,
                                                          const unsigned int *__restrict__ amax,
@@ -193,8 +204,9 @@ __global__ void __launch_bounds__(256) split_maps_kernel(GeomDev g, int32_t n_ma
         }
         const _Float16 h = (_Float16)x;  // nearest
         const _Float16 l = (_Float16)(x - (float)h);
-        out[((grp * 2 + 0) * kMaps + row) * g.n_pad + pi] = h;
-        out[((grp * 2 + 1) * kMaps + row) * g.n_pad + pi] = l;
+        const int64_t kp = kstep_pos(pi);
+        out[((grp * 2 + 0) * kMaps + row) * g.n_pad + kp] = h;
+        out[((grp * 2 + 1) * kMaps + row) * g.n_pad + kp] = l;
     }
 }
 
@@ -368,11 +380,12 @@ __global__ void __launch_bounds__(kThreads) mfma_visgen_kernel(
             for (int ks = 0; ks < kStage / kKSV; ++ks) {
                 // E for this lane's 8 pixels: straight-line phase and sin/cos
                 // first, then the shade mask, so that the 8 chains interleave.
-                const int kb = ks * kKSV + kg * 8;
+                // entry t of this lane is staged pixel ks * 32 + 4 t + kg (see kstep_pos)
+                const int kb = ks * kKSV + kg;
                 float cs[8], sn[8];
 #pragma unroll
                 for (int t = 0; t < 8; ++t) {
-                    const double4 sp = sP[buf][kb + t];
+                    const double4 sp = sP[buf][kb + 4 * t];
                     const double ph = fma(u, sp.x, fma(vv, sp.y, fma(w, sp.z, wo)));
                     const float r = (float)__builtin_amdgcn_fract(ph);
                     cs[t] = __builtin_amdgcn_cosf(r);
@@ -384,7 +397,7 @@ __global__ void __launch_bounds__(kThreads) mfma_visgen_kernel(
                 if (mixed) {  // wave-uniform: shaded and clear pixels both occur
                     uint32_t shm = (cls == kIn) ? 0xffu : 0u;
                     if (cls == kStraddle)
-                        shm = straddle_mask8(g, &sP[buf][kb], sCap[vc][0], sCap[vc][1], ia, ib);
+                        shm = straddle_mask8<4>(g, &sP[buf][kb], sCap[vc][0], sCap[vc][1], ia, ib);
 #pragma unroll
                     for (int t = 0; t < 8; ++t) {
                         const bool sh = (shm >> t) & 1u;
