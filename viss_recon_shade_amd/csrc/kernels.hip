@@ -68,11 +68,14 @@ __device__ __forceinline__ int classify(const ChunkRec &ch, const CapRec &a, con
     return (ca == kIn || cb == kIn) ? kIn : ((ca == kOut && cb == kOut) ? kOut : kStraddle);
 }
 
-// FP32 cap test; flags pixels inside the guard band.
+// FP32 cap test against the guard band's edges: shaded when dot >= hi (the
+// FP32 dot of two unit vectors is within 4e-7 of the exact one, g >= 4e-6);
+// flags pixels with lo <= dot < hi, which the FP64 predicate decides.
 __device__ __forceinline__ bool occ32(float sx, float sy, float sz, const CapRec &a, bool &unc) {
     const float x = fmaf(sz, a.dz, fmaf(sy, a.dy, sx * a.dx));
-    unc = unc || (fabsf(x - a.c) <= a.g);
-    return x >= a.c;
+    const bool in_hi = x >= a.hi;
+    unc = unc || ((x >= a.lo) && !in_hi);
+    return in_hi;
 }
 
 // Exact predicate, identical operation sequence to the FP64 oracle:
@@ -164,7 +167,8 @@ __global__ void __launch_bounds__(256) orbit_kernel(OrbitArgs a, double4 *__rest
     // near-limb band |delta| < 1e-6 R maps to |dot - c| < 1e-6 R^2 / (|r|^2 c)
     const double band = kNearLimbRel * R * R / (r2 * fmax(c, 1e-12));
     cr.g = (float)fmax((double)kGuardMin, 2.0 * band + 1e-6);
-    cr.pad0 = cr.pad1 = 0.0f;
+    cr.lo = cr.c - cr.g;
+    cr.hi = cr.c + cr.g;
     cap[idx] = cr;
 }
 
@@ -296,8 +300,9 @@ constexpr int kListPad = 32;
 __device__ __forceinline__ bool occ32w(float sx, float sy, float sz, const CapRec &a,
                                        uint64_t &unc_w) {
     const float x = fmaf(sz, a.dz, fmaf(sy, a.dy, sx * a.dx));
-    unc_w |= __ballot(fabsf(x - a.c) <= a.g);
-    return x >= a.c;
+    const uint64_t hi_w = __ballot(x >= a.hi);
+    unc_w |= __ballot(x >= a.lo) & ~hi_w;
+    return x >= a.hi;
 }
 
 // One straddling pair over this lane's pixels: the FP32 test of cap A (and of
@@ -385,7 +390,7 @@ __device__ __forceinline__ void fill_vis_ent(const GeomDev &g, const PhaseKnobs 
     } else {
         // tail / padding: no shading (c = 2 is never reached), zero phase; never written out
         e.ia = e.ib = 0;
-        CapRec none = {0.f, 0.f, 0.f, 2.0f, 0.f, 0.f, 0.f, 0.f};
+        CapRec none = {0.f, 0.f, 0.f, 2.0f, 0.f, 0.f, 2.0f, 2.0f};
         e.a = none;
         e.b = none;
         e.u = e.v = e.w = e.wo = (T)0;
@@ -629,7 +634,7 @@ template <bool F64> struct RecHot {
 // the two caps a pixel test reads (not CapRec itself: its 32-byte alignment
 // would take LDS that the fifth block per CU needs).
 struct CapTest {
-    float dx, dy, dz, c, g;
+    float dx, dy, dz, lo, hi;
 };
 struct RecCold {
     CapTest a, b;
@@ -659,8 +664,8 @@ __device__ __forceinline__ void recon_straddle(const GeomDev &g, int c, int lane
                                                const float (&sz)[kPixPerLane],
                                                float (&acc)[kPixPerLane]) {
     using T = typename PhaseT<F64>::T;
-    const CapRec A2 = {o.a.dx, o.a.dy, o.a.dz, o.a.c, 0.f, o.a.g, 0.f, 0.f};
-    const CapRec B2 = {o.b.dx, o.b.dy, o.b.dz, o.b.c, 0.f, o.b.g, 0.f, 0.f};
+    const CapRec A2 = {o.a.dx, o.a.dy, o.a.dz, 0.f, 0.f, 0.f, o.a.lo, o.a.hi};
+    const CapRec B2 = {o.b.dx, o.b.dy, o.b.dz, 0.f, 0.f, 0.f, o.b.lo, o.b.hi};
 #pragma unroll
     for (int j = 0; j < kPixPerLane; ++j) {
         bool unc = false;
@@ -791,8 +796,8 @@ __device__ __forceinline__ void recon_body(const GeomDev &g, const PhaseKnobs &k
                 RecCold &o = cold[pos];
                 // one straddling cap goes first (cap b is then wholly clear of the chunk)
                 const CapRec &t0 = (ca == kOut) ? B : A;
-                o.a = {t0.dx, t0.dy, t0.dz, t0.c, t0.g};
-                o.b = {B.dx, B.dy, B.dz, B.c, B.g};
+                o.a = {t0.dx, t0.dy, t0.dz, t0.lo, t0.hi};
+                o.b = {B.dx, B.dy, B.dz, B.lo, B.hi};
                 o.ia = ia;
                 o.ib = ib;
             }

@@ -340,7 +340,7 @@ __global__ void __launch_bounds__(kThreads) mfma_visgen_kernel(
         // the caps are read once per chunk and in straddling K-steps: keep them in LDS
         // (sCap is free: every wave passed the previous tile's last barrier)
         if (kg == 0) {
-            const CapRec none = {0.f, 0.f, 0.f, 2.0f, 0.f, 0.f, 0.f, 0.f};
+            const CapRec none = {0.f, 0.f, 0.f, 2.0f, 0.f, 0.f, 2.0f, 2.0f};
             sCap[vc][0] = vok ? g.cap[ia] : none;
             sCap[vc][1] = vok ? g.cap[ib] : none;
         }

@@ -27,11 +27,12 @@ constexpr int kBlock = kWave * kWavesPerBlock;
 
 // Shade cap of one satellite at one step: pixel s is shaded iff
 // dot(s, dir) >= c where dir = -r/|r| and c = sqrt(1 - R^2/|r|^2).
-// sa = sin(alpha) = R/|r|.  g = half-width of the FP32 guard band: pixels
-// with |dot - c| <= g are decided by the FP64 predicate instead.
+// sa = sin(alpha) = R/|r|.  g = half-width of the FP32 guard band; lo = c - g
+// and hi = c + g in FP32: the FP32 pixel test shades when dot >= hi, clears
+// when dot < lo, and leaves pixels with lo <= dot < hi to the FP64 predicate.
 struct alignas(32) CapRec {
     float dx, dy, dz, c;
-    float sa, g, pad0, pad1;
+    float sa, g, lo, hi;
 };
 
 // Bounding cap of a chunk: every pixel centre lies within angle rho of (gx,gy,gz).
