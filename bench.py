@@ -232,7 +232,7 @@ def mask_check(g, wl, pos, lmn, stride=32):
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print(json.dumps({"error": "this build measures one GPU only; the work of several GPUs "
+        print(json.dumps({"error": "this benchmark measures one GPU only; the work of several GPUs "
                                    "runs on one MI355X at full size (use --gpus 1)"}))
         sys.exit(2)
     import torch

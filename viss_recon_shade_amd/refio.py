@@ -1,5 +1,5 @@
 """The reference pipeline's file layout: names and text formats of the
-inter-stage files, so that this build's outputs can replace the reference's
+inter-stage files, so that the library's outputs can replace the reference's
 file for file.
 
 Names (directory per frequency, one file per period p, 1-based):
