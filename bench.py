@@ -53,6 +53,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--cpu-stride", type=int, default=32)
+    ap.add_argument("--mask-stride", type=int, default=32,
+                    help="compare the shade counts and hashes of every n-th visibility "
+                         "with the oracle (1: all of them)")
     ap.add_argument("--no-mfma", action="store_true", help="skip the multi-map matrix-core legs")
     ap.add_argument("--no-batch", action="store_true", help="skip the multi-channel batch leg")
     return ap.parse_args()
@@ -214,17 +217,29 @@ def accuracy(g, wl, pos, uvw, lmn, I, V, F, n_vis_sample=4096, n_pix_sample=256)
                       f"visibilities (F from the GPU's own V)"}
 
 
-def mask_check(g, wl, pos, lmn, stride=32):
-    """Shade counts and XOR hashes of every stride-th visibility against the oracle."""
+def mask_check(g, wl, pos, lmn, stride=32, block=65536):
+    """Shade counts and XOR hashes of every stride-th visibility against the oracle,
+    in blocks of visibilities with a progress line on stderr after each."""
     import oracle as o
     st = g.ShadeMatrix()
     sel = np.arange(0, wl.n_vis, stride, dtype=np.int64)
-    cnt, h, near = o.mask_stats(pos, g.pairs, lmn, g.r_moon_km, vis_sel=sel)
+    n_cnt = n_hash = near = 0
+    t0 = time.perf_counter()
+    for b0 in range(0, len(sel), block):
+        sb = sel[b0:b0 + block]
+        cnt, h, nb = o.mask_stats(pos, g.pairs, lmn, g.r_moon_km, vis_sel=sb)
+        n_cnt += int((st["count"][sb] != cnt).sum())
+        n_hash += int((st["hash"][sb] != h).sum())
+        near += int(nb)
+        if len(sel) > block:
+            print(f"mask check: {b0 + len(sb)} / {len(sel)} visibilities, {n_cnt} count and "
+                  f"{n_hash} hash mismatches, {time.perf_counter() - t0:.0f} s",
+                  file=sys.stderr, flush=True)
     return {"visibilities_compared": int(len(sel)),
             "pixel_pairs_compared": int(len(sel)) * wl.n_pix,
-            "count_mismatches": int((st["count"][sel] != cnt).sum()),
-            "hash_mismatches": int((st["hash"][sel] != h).sum()),
-            "near_limb_pairs_sample": int(near),
+            "count_mismatches": n_cnt,
+            "hash_mismatches": n_hash,
+            "near_limb_pairs_sample": near,
             "near_limb_pairs_all": int(st["near_limb"]),
             "fp64_tests_all": int(st["fp64_tests"])}
 
@@ -353,7 +368,7 @@ def main():
     if not args.no_accuracy:
         out["accuracy"] = {"plain": accuracy(g, wl, pos, uvw, lmn, I, Vp, Fp),
                            "mixed": accuracy(g, wl, pos, uvw, lmn, I, Vm, Fm),
-                           "mask": mask_check(g, wl, pos, lmn)}
+                           "mask": mask_check(g, wl, pos, lmn, args.mask_stride)}
     if not args.no_cpu:
         Vc = Vp[0, :, 0].astype(np.float64) + 1j * Vp[0, :, 1]
         out["cpu_baseline"] = cpu_baseline(g, wl, pos, uvw, lmn, I[0], Vc, args.cpu_stride)
