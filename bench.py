@@ -23,7 +23,9 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK_FP32_TFLOPS = 157.3          # MI355X published FP32 vector/matrix peak
+PEAK_FP32_TFLOPS = 157.3          # MI355X published FP32 vector peak: packed FP32 FMA (2 per lane)
+PEAK_FP32_VALU_TFLOPS = 78.6      # the same VALU issuing one FP32 FMA per lane per clock, which
+                                  # is what the plain kernels issue (built without packed FP32)
 TRANS_PEAK = 256 * 2.4e9 * 16     # v_sin/v_cos issue: 16 lanes/clk/CU (quarter rate)
 FLOPS_PER_TERM = 10               # 3 FMA phase + 2 FMA accumulate
 N_CU, N_XCD = 256, 8              # SQ counters sum over CUs, GRBM_GUI_ACTIVE over XCDs
@@ -86,6 +88,29 @@ def kernel_avg_ms(ctx, step, names, reps):
     return {n: acc[n] / reps for n in names}
 
 
+def cpu_model():
+    """CPU model name of the host (first 'model name' of /proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_error(g, wl, pos, uvw, lmn, I, V, refs):
+    """Error of the FP32 CPU restatement against the FP64 oracle on the same
+    deterministic samples as the GPU accuracy figures (refs from accuracy())."""
+    import oracle as o
+    Vc = o.cpu_visgen_f32(pos, g.pairs, uvw, lmn, I, g.r_moon_km, vis_sel=refs["vis_sel"])
+    Fc = o.cpu_recon_f32(pos, g.pairs, uvw, lmn, V, g.r_moon_km, pix_sel=refs["pix_sel"])
+    return {"V_rel_l2": float(np.linalg.norm(Vc - refs["Vo"]) / np.linalg.norm(refs["Vo"])),
+            "F_rel_l2": float(np.linalg.norm(Fc - refs["Fo"]) / np.linalg.norm(refs["Fo"])),
+            "sample": "the plain-FP32 accuracy sample, F from the GPU's V"}
+
+
 def cpu_baseline(g, wl, pos, uvw, lmn, I, V, stride):
     import oracle as o
     sel = np.arange(0, wl.n_vis, stride, dtype=np.int64)
@@ -99,7 +124,8 @@ def cpu_baseline(g, wl, pos, uvw, lmn, I, V, stride):
     terms = len(sel) * wl.n_pix + len(psel) * wl.n_vis
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {"value": round(terms / (t2 - t0) / 1e9, 4), "unit": "Gterms/s", "cores": cores,
-            "kind": "port",
+            "kind": "port", "cpu_model": cpu_model(),
+            "omp": {"OMP_NUM_THREADS": cores, "OMP_PROC_BIND": os.environ.get("OMP_PROC_BIND")},
             "sample": f"every {stride}th visibility x all {wl.n_pix} pixels (vis-gen) + every "
                       f"{stride}th pixel x all {wl.n_vis} visibilities (recon): {terms:.3e} terms "
                       f"in {t2 - t0:.1f} s; FP32 OpenMP restatement of visscal/imagerecon"}
@@ -144,17 +170,20 @@ def mfma_legs(torch, ctx, vrs, W, wl, g, pos, uvw, lmn, args):
             F = st["F"].cpu().numpy()
             p2, u2 = gg.MIncline()
             l2 = gg.lmn()
-            sel = np.linspace(0, gg.n_vis - 1, 16).astype(np.int64)
-            maps = [0, 31, 63]
-            Vo = o.visgen(p2, gg.pairs, u2, l2, I[maps], gg.r_moon_km, vis_sel=sel)
-            Vc = V[maps][:, sel, 0].astype(np.float64) + 1j * V[maps][:, sel, 1]
-            psel = np.linspace(0, n_pix - 1, 4).astype(np.int64)
-            Vfull = V[maps][..., 0].astype(np.float64) + 1j * V[maps][..., 1]
+            sel = np.linspace(0, gg.n_vis - 1, 256).astype(np.int64)
+            Vo = o.visgen(p2, gg.pairs, u2, l2, I, gg.r_moon_km, vis_sel=sel)
+            Vc = V[:, sel, 0].astype(np.float64) + 1j * V[:, sel, 1]
+            psel = np.linspace(0, n_pix - 1, 64).astype(np.int64)
+            Vfull = V[..., 0].astype(np.float64) + 1j * V[..., 1]
             Fo = o.recon(p2, gg.pairs, u2, l2, Vfull, gg.r_moon_km, pix_sel=psel)
+            ev = np.linalg.norm(Vc - Vo, axis=1) / np.linalg.norm(Vo, axis=1)
+            ef = np.linalg.norm(F[:, psel] - Fo, axis=1) / np.linalg.norm(Fo, axis=1)
             entry["accuracy"] = {
-                "V_rel_l2_max": float((np.linalg.norm(Vc - Vo, axis=1) / np.linalg.norm(Vo, axis=1)).max()),
-                "F_rel_l2_max": float((np.linalg.norm(F[maps][:, psel] - Fo, axis=1) / np.linalg.norm(Fo, axis=1)).max()),
-                "sample": "maps 0, 31, 63; 16 visibilities x all pixels; 4 pixels x all visibilities"}
+                "V_rel_l2_max": float(ev.max()), "V_rel_l2_median": float(np.median(ev)),
+                "F_rel_l2_max": float(ef.max()), "F_rel_l2_median": float(np.median(ef)),
+                "sample": "all 64 maps; 256 visibilities x all pixels (V); 64 pixels x all "
+                          "visibilities (F); per-map relative L2 against the FP64 oracle"}
+            del Vfull, V
         res[name] = entry
         del It, st
         torch.cuda.empty_cache()
@@ -167,11 +196,10 @@ def batch_leg(torch, ctx, vrs, W, args):
     I_c = I0 (nu_c / 1 MHz)^alpha_c with alpha_c ~ N(-2.5, 0.1); one captured
     hipGraph per step, autotuned tiles.  Terms = sum_c kept_c x N_pix per direction."""
     freqs, n_t, alpha = W.c4_channels()
-    nside = 64
-    g = ctx.geometry(16, int(n_t.max()), 1e7, nside)
+    nside = W.C4_NSIDE
+    g = ctx.geometry(W.C4_N_SAT, int(n_t.max()), 1e7, nside)
     b = ctx.batch(g, freqs, n_t, compact=True, precision="mixed", autotune=True)
-    I0 = W.make_sky(W.Workload("c4", 16, int(n_t.max()), 1e7, nside, 1, [(1, 1.0, 0, 1.0, 1e3)]))[0]
-    I = np.stack([I0 * (f / 1e6) ** a for f, a in zip(freqs, alpha)]).astype(np.float32)
+    I = W.c4_skies(freqs, alpha, nside)
     It = torch.from_numpy(I).cuda()
     k = vrs.knobs(precision="mixed")
 
@@ -187,9 +215,45 @@ def batch_leg(torch, ctx, vrs, W, args):
            "samples_before_compaction": int(120 * n_t.sum()),
            "tiles": {"vis_tile": b.info["vis_tile"], "recon_seg": b.info["recon_seg"]},
            "precision": "VRS_MIXED_F64PHASE"}
+    if not args.no_accuracy:
+        V, F = b.run(I, k)
+        out["accuracy"] = batch_accuracy(g, b, freqs, I, V, F)
     b.close()
     g.close()
     return out
+
+
+def batch_accuracy(g, b, freqs, I, V, F, n_vis_sample=64, n_pix_sample=16,
+                   image_channels=(0, 10, 21, 31)):
+    """Per-channel errors of one batch run against the FP64 oracle: V on
+    n_vis_sample kept samples x all pixels of every channel, F (kappa = 1/kept)
+    on n_pix_sample pixels x all kept samples of `image_channels`."""
+    import oracle as o
+    pos, _ = g.MIncline()
+    lmn = g.lmn()
+    ev, ef = [], {}
+    for c in range(len(freqs)):
+        sl = slice(b.offsets[c], b.offsets[c + 1])
+        bt = b.sample_bt[sl]
+        real = bt >= 0
+        bt = bt[real]
+        Vc = V[sl][real][:, 0].astype(np.float64) + 1j * V[sl][real][:, 1]
+        uvw = o.uvw(pos, g.pairs, 299792458.0 / freqs[c])
+        pick = np.linspace(0, len(bt) - 1, n_vis_sample).astype(np.int64)
+        Vo = o.visgen(pos, g.pairs, uvw, lmn, I[c], g.r_moon_km, vis_sel=bt[pick])[0]
+        ev.append(float(np.linalg.norm(Vc[pick] - Vo) / np.linalg.norm(Vo)))
+        if c in image_channels:
+            Vfull = np.zeros(g.n_vis, np.complex128)
+            Vfull[bt] = Vc
+            psel = np.linspace(0, g.n_pix - 1, n_pix_sample).astype(np.int64)
+            Fo = o.recon(pos, g.pairs, uvw, lmn, Vfull, g.r_moon_km, kappa=1.0 / b.kept[c],
+                         pix_sel=psel)[0]
+            ef[str(c)] = float(np.linalg.norm(F[c, psel] - Fo) / np.linalg.norm(Fo))
+    return {"V_rel_l2_max": max(ev), "V_rel_l2_per_channel": [round(x, 9) for x in ev],
+            "F_rel_l2": ef, "F_rel_l2_max": max(ef.values()),
+            "sample": f"V: {n_vis_sample} kept samples x all pixels of every channel; F: "
+                      f"{n_pix_sample} pixels x all kept samples of channels "
+                      f"{list(image_channels)}, kappa = 1/kept; FP64 oracle"}
 
 
 def _errs(x, ref):
@@ -201,9 +265,10 @@ def _errs(x, ref):
             "p99_abs_over_inf": float(np.quantile(d, 0.99) / scale)}
 
 
-def accuracy(g, wl, pos, uvw, lmn, I, V, F, n_vis_sample=4096, n_pix_sample=256):
+def accuracy(g, wl, pos, uvw, lmn, I, V, F, n_vis_sample=4096, n_pix_sample=256, refs=None):
     """Errors against the FP64 oracle on deterministic subsamples: evenly spaced
-    visibilities x all pixels (V), evenly spaced pixels x all visibilities (F)."""
+    visibilities x all pixels (V), evenly spaced pixels x all visibilities (F).
+    `refs` (a dict) receives the samples and the oracle's values."""
     import oracle as o
     sel = np.linspace(0, wl.n_vis - 1, n_vis_sample).astype(np.int64)
     Vo = o.visgen(pos, g.pairs, uvw, lmn, I, g.r_moon_km, vis_sel=sel)[0]
@@ -212,6 +277,8 @@ def accuracy(g, wl, pos, uvw, lmn, I, V, F, n_vis_sample=4096, n_pix_sample=256)
     Vfull = V[0, :, 0].astype(np.float64) + 1j * V[0, :, 1]
     Fo = o.recon(pos, g.pairs, uvw, lmn, Vfull, g.r_moon_km, pix_sel=psel)[0]
     ev, ef = _errs(Vc, Vo), _errs(F[0, psel], Fo)
+    if refs is not None:
+        refs.update(vis_sel=sel, pix_sel=psel, Vo=Vo, Fo=Fo)
     return {"V_rel_l2": ev["rel_l2"], "F_rel_l2": ef["rel_l2"], "V": ev, "F": ef,
             "sample": f"{n_vis_sample} visibilities x all pixels; {n_pix_sample} pixels x all "
                       f"visibilities (F from the GPU's own V)"}
@@ -250,6 +317,8 @@ def main():
         print(json.dumps({"error": "this benchmark measures one GPU only; the work of several GPUs "
                                    "runs on one MI355X at full size (use --gpus 1)"}))
         sys.exit(2)
+    # the CPU legs run the OpenMP restatements with threads bound close
+    os.environ.setdefault("OMP_PROC_BIND", "close")
     import torch
     import viss_recon_shade_amd as vrs
     from viss_recon_shade_amd import workload as W
@@ -340,12 +409,17 @@ def main():
         "config": {"workload": f"{wl.name}: {wl.note}", "n_vis": wl.n_vis, "n_pix": wl.n_pix,
                    "n_maps": wl.n_maps, "freq_hz": wl.freq_hz, "precision": "VRS_PLAIN_F32",
                    "step": "orbit+uvw+caps, vis-gen, recon"},
-        # compute roofline: the published FP32 peak; the engine doing the work is the VALU
-        "roofline": {"bound": "mfma", "engine": "VALU (FP32 FMA + v_sin/v_cos); no matrix-core work "
-                                                 "in a single-map DFT",
+        # compute roofline of a VALU kernel: a single-map DFT has no matrix-core work; each
+        # term is 10 FP32 flops plus two quarter-rate transcendentals
+        "roofline": {"bound": "valu", "engine": "VALU (FP32 FMA + v_sin/v_cos)",
                      "kernel": dom, "achieved": round(achieved, 2),
-                     "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": traffic,
+                     "peak": PEAK_FP32_VALU_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / PEAK_FP32_VALU_TFLOPS, 4),
+                     "peak_note": "MI355X FP32 VALU at one FMA per lane per clock (256 CUs x 64 "
+                                  "lanes x 2 flops x 2.4 GHz); the published 157.3 TFLOP/s counts "
+                                  "packed FP32 FMAs, which this build does not issue",
+                     "frac_of_published_157": round(achieved / PEAK_FP32_TFLOPS, 4),
+                     "traffic": traffic,
                      "traffic_source": "profiles/traffic_c2.json (rocprofv3 FETCH_SIZE/WRITE_SIZE, "
                                        "copy-calibrated)" if traffic else None,
                      "algorithmic_bytes": algorithmic_bytes(wl, dom),
@@ -365,13 +439,16 @@ def main():
         out["mfma_split16"] = mfma_legs(torch, ctx, vrs, W, wl, g, pos, uvw, lmn, args)
     if not args.no_batch:
         out["batch_c4"] = batch_leg(torch, ctx, vrs, W, args)
+    refs = {}
     if not args.no_accuracy:
-        out["accuracy"] = {"plain": accuracy(g, wl, pos, uvw, lmn, I, Vp, Fp),
+        out["accuracy"] = {"plain": accuracy(g, wl, pos, uvw, lmn, I, Vp, Fp, refs=refs),
                            "mixed": accuracy(g, wl, pos, uvw, lmn, I, Vm, Fm),
                            "mask": mask_check(g, wl, pos, lmn, args.mask_stride)}
     if not args.no_cpu:
         Vc = Vp[0, :, 0].astype(np.float64) + 1j * Vp[0, :, 1]
         out["cpu_baseline"] = cpu_baseline(g, wl, pos, uvw, lmn, I[0], Vc, args.cpu_stride)
+        if refs:
+            out["cpu_baseline"]["accuracy"] = cpu_error(g, wl, pos, uvw, lmn, I[0], Vc, refs)
     print(json.dumps(out))
 
 

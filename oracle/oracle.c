@@ -346,22 +346,28 @@ void vo_visgen(int64_t n_t, const double *pos, const int *pairs, const double *u
             wre = cos(2.0 * VO_PI * r);
             wim = -sin(2.0 * VO_PI * r);
         }
-        for (int mp = 0; mp < n_maps; ++mp) {
-            const float *Im = I + (int64_t)mp * n_pix;
-            double are = 0.0, aim = 0.0;
-            for (int64_t p = 0; p < n_pix; ++p) {
-                int na = 0, nb = 0;
-                int m = vo_occ(ra, ra2, lmn + 3 * p, &L, &na) | vo_occ(rb, rb2, lmn + 3 * p, &L, &nb);
-                double ip = m ? t_occ : (double)Im[p];
-                double phi = u * lmn[3 * p] + vv * lmn[3 * p + 1] + w * lmn_nm1[p];
-                double r = phi - rint(phi);
-                are += ip * cos(2.0 * VO_PI * r);
-                aim -= ip * sin(2.0 * VO_PI * r);
+        /* every map sums its pixels in ascending order; the mask and the phase
+         * factor of a (visibility, pixel) pair are formed once for all maps */
+        double *acc = (double *)calloc(2 * (size_t)n_maps, sizeof(double));
+        for (int64_t p = 0; p < n_pix; ++p) {
+            int na = 0, nb = 0;
+            int m = vo_occ(ra, ra2, lmn + 3 * p, &L, &na) | vo_occ(rb, rb2, lmn + 3 * p, &L, &nb);
+            double phi = u * lmn[3 * p] + vv * lmn[3 * p + 1] + w * lmn_nm1[p];
+            double r = phi - rint(phi);
+            double c = cos(2.0 * VO_PI * r), sn = sin(2.0 * VO_PI * r);
+            for (int mp = 0; mp < n_maps; ++mp) {
+                double ip = m ? t_occ : (double)I[(int64_t)mp * n_pix + p];
+                acc[2 * mp] += ip * c;
+                acc[2 * mp + 1] -= ip * sn;
             }
+        }
+        for (int mp = 0; mp < n_maps; ++mp) {
+            double are = acc[2 * mp], aim = acc[2 * mp + 1];
             double *o = V + ((int64_t)mp * n_sel + k) * 2;
             o[0] = are * wre - aim * wim;
             o[1] = are * wim + aim * wre;
         }
+        free(acc);
     }
     free(lmn_nm1);
 }
@@ -449,25 +455,28 @@ void vo_recon(int64_t n_t, const double *pos, const int *pairs, int n_base, cons
         int64_t p = pix_idx ? pix_idx[k] : k;
         const double *sp = lmn + 3 * p;
         double nterm = wfac ? sp[2] : sp[2] - 1.0;
-        for (int mp = 0; mp < n_maps; ++mp) {
-            const double *Vm = V + (int64_t)mp * n_vis * 2;
-            double acc = 0.0;
-            for (int64_t v = 0; v < n_vis; ++v) {
-                int64_t b = v / n_t, t = v % n_t;
-                const double *ra = pos + ((int64_t)pairs[2 * b] * n_t + t) * 3;
-                const double *rb = pos + ((int64_t)pairs[2 * b + 1] * n_t + t) * 3;
-                int na = 0, nb = 0;
-                if (vo_occ(ra, vo_r2(ra), sp, &L, &na) | vo_occ(rb, vo_r2(rb), sp, &L, &nb))
-                    continue;
-                double u = uvw[3 * v] * s, vv = uvw[3 * v + 1] * s, w = uvw[3 * v + 2] * s;
-                double phi = u * sp[0] + vv * sp[1] + w * nterm;
-                double r = phi - rint(phi);
-                double c = cos(2.0 * VO_PI * r), sn = sin(2.0 * VO_PI * r);
-                double wv = wgt ? (double)wgt[v] : 1.0;
-                acc += wv * (Vm[2 * v] * c - Vm[2 * v + 1] * sn);
+        /* every map sums its visibilities in ascending order; the mask and the
+         * phase factor of a (visibility, pixel) pair are formed once for all maps */
+        double *acc = (double *)calloc((size_t)n_maps, sizeof(double));
+        for (int64_t v = 0; v < n_vis; ++v) {
+            int64_t b = v / n_t, t = v % n_t;
+            const double *ra = pos + ((int64_t)pairs[2 * b] * n_t + t) * 3;
+            const double *rb = pos + ((int64_t)pairs[2 * b + 1] * n_t + t) * 3;
+            int na = 0, nb = 0;
+            if (vo_occ(ra, vo_r2(ra), sp, &L, &na) | vo_occ(rb, vo_r2(rb), sp, &L, &nb))
+                continue;
+            double u = uvw[3 * v] * s, vv = uvw[3 * v + 1] * s, w = uvw[3 * v + 2] * s;
+            double phi = u * sp[0] + vv * sp[1] + w * nterm;
+            double r = phi - rint(phi);
+            double c = cos(2.0 * VO_PI * r), sn = sin(2.0 * VO_PI * r);
+            double wv = wgt ? (double)wgt[v] : 1.0;
+            for (int mp = 0; mp < n_maps; ++mp) {
+                const double *Vm = V + (int64_t)mp * n_vis * 2;
+                acc[mp] += wv * (Vm[2 * v] * c - Vm[2 * v + 1] * sn);
             }
-            F[(int64_t)mp * n_sel + k] = kappa * acc;
         }
+        for (int mp = 0; mp < n_maps; ++mp) F[(int64_t)mp * n_sel + k] = kappa * acc[mp];
+        free(acc);
     }
 }
 

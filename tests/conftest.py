@@ -35,3 +35,25 @@ def ctx():
     c = v.Context(0)
     yield c
     c.close()
+
+
+# Accuracy figures that GPU tests report, printed as one section at the end of
+# the run (so they appear in the tail of a quiet `pytest -q` log as well).
+_FIGURES = []
+
+
+@pytest.fixture(scope="session")
+def report():
+    def add(name, **vals):
+        _FIGURES.append((name, vals))
+    return add
+
+
+def pytest_terminal_summary(terminalreporter, exitstatus, config):
+    if not _FIGURES:
+        return
+    terminalreporter.section("parity figures")
+    for name, vals in _FIGURES:
+        body = ", ".join(f"{k}={v:.3g}" if isinstance(v, float) else f"{k}={v}"
+                         for k, v in vals.items())
+        terminalreporter.write_line(f"{name}: {body}")

@@ -189,7 +189,7 @@ def test_argument_errors_on_gpu(ctx):
 
 
 # ------------------------------------------------------------------ 10 MHz at scale
-def test_c2_full_size_sampled_parity(ctx, oracle):
+def test_c2_full_size_sampled_parity(ctx, oracle, report):
     wl, g, pos, uvw, lmn, I = geom(ctx, "c2")
     k = vrs.knobs(precision="mixed")
     V = g.visscal(I, k)
@@ -203,12 +203,18 @@ def test_c2_full_size_sampled_parity(ctx, oracle):
     e_f = rel(F[0, psel], Fo[0])
     print(f"[c2 mixed] V rel {e_v:.2e}  F rel {e_f:.2e}", flush=True)
     assert e_v <= TOL_MIXED_10MHZ and e_f <= TOL_MIXED_10MHZ, (e_v, e_f)
-    # plain FP32 at 10 MHz: the reference-class phase error, bounded
+    # plain FP32 at 10 MHz: the reference's FP32 arithmetic (vissGen10M_shade_optim1.cu:
+    # 183-195, complexExp of vissGen10M_shade.cu:35-39) restated on the host cores has its
+    # own error against FP64 on the same sample; the GPU plain path must sit in that
+    # error class (<= 1.25x of it) and under the stated bound
     kp = vrs.knobs(precision="plain")
     Vp = g.visscal(I, kp)
     e_p = rel(cplx(Vp)[0, sel], Vo[0])
-    print(f"[c2 plain] V rel {e_p:.2e}", flush=True)
-    assert e_p <= 0.1
+    Vcpu = oracle.cpu_visgen_f32(pos, g.pairs, uvw, lmn, I[0], g.r_moon_km, vis_sel=sel)
+    e_cpu = rel(Vcpu, Vo[0])
+    print(f"[c2 plain] V rel {e_p:.2e} (CPU FP32 restatement {e_cpu:.2e})", flush=True)
+    report("c2 plain FP32 V (48 vis x all pix vs FP64)", gpu_rel_l2=e_p, cpu_f32_rel_l2=e_cpu)
+    assert e_p <= TOL_PLAIN and e_p <= 1.25 * e_cpu, (e_p, e_cpu)
     # mask totals: every visibility shades between one and two caps' worth of pixels
     st = g.ShadeMatrix()
     c = math.sqrt(1 - (W.R_MOON_KM / W.A_KM) ** 2)

@@ -49,6 +49,31 @@ def _ptr(x):
     return C.c_void_p(x.ctypes.data)
 
 
+def _f32_input(x, shape, name, device=None):
+    """Validate an input array before its pointer crosses the C-ABI, which reads
+    float32 of exactly `shape` (None entries are free).  numpy inputs are
+    converted to C-contiguous float32 (a float64 array would otherwise be read
+    as float32 bit patterns); torch tensors must already be contiguous float32,
+    on the context's device when they live on a GPU."""
+    if _is_torch(x):
+        if x.dtype != torch.float32:
+            raise TypeError(f"{name} must be float32, got {x.dtype}")
+        if x.is_cuda and device is not None and x.device.index != device:
+            raise ValueError(f"{name} is on {x.device}, the context on cuda:{device}")
+        if not x.is_contiguous():
+            raise ValueError(f"{name} must be contiguous")
+    else:
+        x = np.asarray(x)
+        if x.dtype.kind not in "fiu":
+            raise TypeError(f"{name} must be a real array, got {x.dtype}")
+        x = np.ascontiguousarray(x, np.float32)
+    got = tuple(x.shape)
+    if len(got) != len(shape) or any(s is not None and s != g for s, g in zip(shape, got)):
+        want = "[" + "][".join("*" if s is None else str(s) for s in shape) + "]"
+        raise ValueError(f"{name} must have shape {want}, got {list(got)}")
+    return x
+
+
 def _empty_like_kind(ref, shape, dtype):
     if _is_torch(ref):
         tdt = {np.float32: torch.float32, np.float64: torch.float64, np.int32: torch.int32,
@@ -108,7 +133,7 @@ class Context:
     def shade_box(self, uvw, res, r_moon_ref_km=1737.74):
         """The reference's ShadeMatrix index boxes (1-based [thetaS thetaE phiS phiE])
         for FP32 uvw [n][3] on a res x res grid (shade10M.cu:73-105)."""
-        uvw = uvw if _is_torch(uvw) else np.ascontiguousarray(uvw, np.float32)
+        uvw = _f32_input(uvw, (None, 3), "uvw", self.device)
         n = uvw.shape[0]
         out = _empty_like_kind(uvw, (n, 4), np.int32)
         self._sync_stream(uvw)
@@ -239,9 +264,10 @@ class Geometry:
     def visscal(self, I, k=None):
         """V[map][v] (complex64 as [..., 2] float32) from I[map][pix] float32."""
         k = k or knobs()
+        I = _f32_input(I, (None,) * max(I.ndim, 1), "I", self.ctx.device)
+        if I.ndim not in (1, 2) or I.shape[-1] != self.n_pix:
+            raise ValueError(f"I must be [{self.n_pix}] or [maps][{self.n_pix}], got {list(I.shape)}")
         n_maps = 1 if I.ndim == 1 else I.shape[0]
-        if I.shape[-1] != self.n_pix:
-            raise ValueError(f"I has {I.shape[-1]} pixels, geometry has {self.n_pix}")
         V = _empty_like_kind(I, (n_maps, self.n_vis, 2), np.float32)
         self.ctx._sync_stream(I)
         self.ctx.check(self.ctx._lib.vrs_visgen(self.ctx.handle, self.handle, C.byref(k), n_maps,
@@ -253,6 +279,9 @@ class Geometry:
     def autotune(self, I, k=None):
         """Time the vis-gen tile candidates on input I; returns (tile, ms)."""
         k = k or knobs()
+        I = _f32_input(I, (None,) * max(I.ndim, 1), "I", self.ctx.device)
+        if I.ndim not in (1, 2) or I.shape[-1] != self.n_pix:
+            raise ValueError(f"I must be [{self.n_pix}] or [maps][{self.n_pix}], got {list(I.shape)}")
         n_maps = 1 if I.ndim == 1 else I.shape[0]
         t = C.c_int32()
         ms = C.c_double()
@@ -308,9 +337,12 @@ class Geometry:
     def imagerecon(self, V, w=None, k=None):
         """F[map][pix] float32 from V[map][v] (complex64 as [..., 2] float32)."""
         k = k or knobs()
+        if V.ndim not in (2, 3):
+            raise ValueError(f"V must be [{self.n_vis}][2] or [maps][{self.n_vis}][2]")
+        V = _f32_input(V, (None,) * (V.ndim - 2) + (self.n_vis, 2), "V", self.ctx.device)
+        if w is not None:
+            w = _f32_input(w, (self.n_vis,), "w", self.ctx.device)
         n_maps = 1 if V.ndim == 2 else V.shape[0]
-        if V.shape[-2] != self.n_vis or V.shape[-1] != 2:
-            raise ValueError(f"V must be [maps][{self.n_vis}][2]")
         F = _empty_like_kind(V, (n_maps, self.n_pix), np.float32)
         self.ctx._sync_stream(V, w)
         self.ctx.check(self.ctx._lib.vrs_recon(self.ctx.handle, self.handle, C.byref(k), n_maps,
@@ -323,8 +355,7 @@ class Geometry:
     def adjoint(self, resid, k=None):
         """Gradient of sum |V(I) - V_obs|^2 given resid = V(I) - V_obs ([n_vis][2])."""
         k = k or knobs()
-        if resid.shape[-2:] != (self.n_vis, 2):
-            raise ValueError(f"resid must be [{self.n_vis}][2]")
+        resid = _f32_input(resid, (self.n_vis, 2), "resid", self.ctx.device)
         g = _empty_like_kind(resid, (self.n_pix,), np.float32)
         self.ctx._sync_stream(resid)
         self.ctx.check(self.ctx._lib.vrs_adjoint(self.ctx.handle, self.handle, C.byref(k),
@@ -373,8 +404,7 @@ class Batch:
     def run(self, I, k=None, want_V=True):
         """I [n_ch][n_pix] -> (V [n_vis_padded][2] or None, F [n_ch][n_pix])."""
         k = k or knobs(precision=self.precision)
-        if I.shape != (self.n_ch, self.geom.n_pix):
-            raise ValueError(f"I must be [{self.n_ch}][{self.geom.n_pix}]")
+        I = _f32_input(I, (self.n_ch, self.geom.n_pix), "I", self.ctx.device)
         V = _empty_like_kind(I, (self.info["n_vis_padded"], 2), np.float32) if want_V else None
         F = _empty_like_kind(I, (self.n_ch, self.geom.n_pix), np.float32)
         self.ctx._sync_stream(I)

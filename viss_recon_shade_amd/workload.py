@@ -140,3 +140,16 @@ def fine_sky(nside, factor=4, seed=7, sigma=1.0):
     ff = np.empty(n_cells, np.float32)
     _lib.check(lib.vrs_sky_lognormal(seed, sigma, n_cells, 0, 1.0, 1.0, ff.ctypes.data))
     return lmn, ff, nx
+
+
+C4_NSIDE = 64
+C4_N_SAT = 16
+
+
+def c4_skies(freqs, alpha, nside=C4_NSIDE, seed=1):
+    """Per-channel skies of the batched form: I_c = I0 (nu_c / 1 MHz)^alpha_c,
+    I0 log-normal(0, 1) from `seed` (RING order), [n_ch][n_pix] float32."""
+    n_pix = 12 * nside * nside
+    I0 = make_sky(Workload("c4", C4_N_SAT, 1, 1e6, nside, 1, [(seed, 1.0, 0, 1.0, 1e3)]))[0]
+    return np.stack([I0.astype(np.float64) * (f / 1e6) ** a
+                     for f, a in zip(freqs, alpha)]).astype(np.float32).reshape(len(freqs), n_pix)
