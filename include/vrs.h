@@ -172,6 +172,20 @@ vrs_status vrs_autotune_visgen(vrs_ctx *ctx, const vrs_geom *g, const vrs_knobs 
  * gradient), built on the imagerecon kernel with kappa = 2 and unit weights. */
 vrs_status vrs_adjoint(vrs_ctx *ctx, const vrs_geom *g, const vrs_knobs *k, const float *resid,
                        float *grad);
+/* vrs_recon without any allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and results (identical bits) as vrs_recon, but V, F and w (when
+ * non-NULL) must be device memory and the precision VRS_PLAIN_F32 or
+ * VRS_MIXED_F64PHASE.  Its partial images live in one fixed scratch array per
+ * device of VRS_NOALLOC_SCRATCH_BYTES, held until the process ends; a call
+ * needing more (segments x n_maps x padded pixels x 4 bytes, ~6.3 MB per map at
+ * Nside 128) fails with VRS_E_ARG.  Every call on a device shares that array,
+ * so calls that overlap in time on one device (other streams or contexts) race;
+ * each call writes all of the scratch it reads, nothing carries over. */
+#define VRS_NOALLOC_SCRATCH_BYTES (64u << 20)
+vrs_status vrs_recon_noalloc(vrs_ctx *ctx, const vrs_geom *g, const vrs_knobs *k, int32_t n_maps,
+                             const float *V, const float *w, float *F);
 
 /* ----------------------------------------------------------- ragged batches */
 /* All frequency channels of one array in one go: channel c uses the first

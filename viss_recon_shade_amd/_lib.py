@@ -93,6 +93,7 @@ SIGNATURES = {
     "vrs_visgen": (C.c_int, [_ctx, _vp, C.POINTER(Knobs), C.c_int32, _vp, _vp]),
     "vrs_recon": (C.c_int, [_ctx, _vp, C.POINTER(Knobs), C.c_int32, _vp, _vp, _vp]),
     "vrs_adjoint": (C.c_int, [_ctx, _vp, C.POINTER(Knobs), _vp, _vp]),
+    "vrs_recon_noalloc": (C.c_int, [_ctx, _vp, C.POINTER(Knobs), C.c_int32, _vp, _vp, _vp]),
     "vrs_autotune_visgen": (C.c_int, [_ctx, _vp, C.POINTER(Knobs), C.c_int32, _vp,
                                       C.POINTER(C.c_int32), C.POINTER(C.c_double)]),
     "vrs_batch_create": (C.c_int, [_ctx, _vp, C.POINTER(BatchDesc), C.POINTER(C.c_void_p)]),

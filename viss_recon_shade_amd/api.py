@@ -352,6 +352,28 @@ class Geometry:
 
     recon = imagerecon
 
+    def imagerecon_noalloc(self, V, w=None, k=None, out=None):
+        """imagerecon through vrs_recon_noalloc: device tensors only, no device
+        allocation inside the library (its partial images live in a fixed
+        per-device scratch array).  `out` (float32 [maps][n_pix] on the device)
+        receives F; without it torch allocates F."""
+        k = k or knobs()
+        if not _is_torch(V) or not V.is_cuda:
+            raise ValueError("imagerecon_noalloc takes device tensors")
+        if V.ndim not in (2, 3):
+            raise ValueError(f"V must be [{self.n_vis}][2] or [maps][{self.n_vis}][2]")
+        V = _f32_input(V, (None,) * (V.ndim - 2) + (self.n_vis, 2), "V", self.ctx.device)
+        if w is not None:
+            w = _f32_input(w, (self.n_vis,), "w", self.ctx.device)
+        n_maps = 1 if V.ndim == 2 else V.shape[0]
+        F = out if out is not None else _empty_like_kind(V, (n_maps, self.n_pix), np.float32)
+        _f32_input(F, (n_maps, self.n_pix), "out", self.ctx.device)
+        self.ctx._sync_stream(V, w, F)
+        self.ctx.check(self.ctx._lib.vrs_recon_noalloc(
+            self.ctx.handle, self.handle, C.byref(k), n_maps, _ptr(V),
+            _ptr(w) if w is not None else None, _ptr(F)))
+        return F
+
     def adjoint(self, resid, k=None):
         """Gradient of sum |V(I) - V_obs|^2 given resid = V(I) - V_obs ([n_vis][2])."""
         k = k or knobs()

@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <tuple>
 #include <string>
 #include <vector>
@@ -30,6 +31,35 @@ thread_local std::string g_thread_err;
 constexpr double kC = 299792458.0;
 }  // namespace
 
+// Scratch of vrs_recon_noalloc: the partial images of its (map, visibility
+// segment) slots.  Each device that loads the library holds its own copy at its
+// own address until the process ends.  Host code never uses this name as an
+// address: it takes the device address once per device with
+// hipGetSymbolAddress (noalloc_scratch_addr) and passes that pointer on.
+static __device__ __align__(256) float scratch[VRS_NOALLOC_SCRATCH_BYTES / sizeof(float)];
+
+namespace {
+std::mutex g_scratch_mu;
+std::map<int, float *> g_scratch_addr;  // device -> address of its copy of `scratch`
+
+// The device address of `scratch` on the current device (`device`), looked up
+// once per device.
+hipError_t noalloc_scratch_addr(int device, float **out) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch_addr.find(device);
+    if (it != g_scratch_addr.end()) {
+        *out = it->second;
+        return hipSuccess;
+    }
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_addr[device] = (float *)p;
+    *out = (float *)p;
+    return hipSuccess;
+}
+}  // namespace
+
 struct vrs_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -46,6 +76,8 @@ struct vrs_ctx {
     std::map<std::tuple<int64_t, int64_t, int32_t, int32_t>, std::pair<int, int>> tuned;
     // autotuned single-geometry vis-gen tiles, keyed by (n_vis, n_pix, n_maps, precision)
     std::map<std::tuple<int64_t, int64_t, int32_t, int32_t>, int> tuned_vt;
+    // device address of this device's copy of the allocation-free recon's scratch
+    float *noalloc_scratch = nullptr;
 };
 
 struct vrs_geom {
@@ -317,6 +349,13 @@ vrs_status vrs_create(int device, vrs_ctx **out) {
         (void)hipStreamDestroy(c->stream);
         delete c;
         return fail(nullptr, VRS_E_OOM, "hipMalloc of counters failed");
+    }
+    if (hipError_t e = noalloc_scratch_addr(device, &c->noalloc_scratch)) {
+        (void)hipFree(c->d_tot);
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return fail(nullptr, VRS_E_HIP, "hipGetSymbolAddress of the recon scratch failed: %s",
+                    hipGetErrorString(e));
     }
     *out = c;
     return VRS_OK;
@@ -834,6 +873,16 @@ vrs_status vrs_autotune_visgen(vrs_ctx *ctx, const vrs_geom *g, const vrs_knobs 
     return VRS_OK;
 }
 
+// Segments of the visibility range of the VALU recon: enough work items to fill
+// the chip, a multiple of 8 so that the blocks on each XCD stream their own
+// segments; independent of n_maps, so that each map's sums are grouped the same way.
+static int64_t recon_segments(const vrs_geom *g) {
+    int64_t n_seg = (4096 + g->n_chunk - 1) / g->n_chunk;
+    n_seg = (n_seg + 7) / 8 * 8;
+    while (n_seg > 8 && g->n_vis / n_seg < 1024) n_seg -= 8;
+    return std::min<int64_t>(n_seg, 1024);
+}
+
 static vrs_status recon_common(vrs_ctx *ctx, const vrs_geom *g, const vrs_knobs *k,
                                int32_t n_maps, const float *V, const float *w, float *F,
                                double kappa, const char *timer_name) {
@@ -870,15 +919,8 @@ static vrs_status recon_common(vrs_ctx *ctx, const vrs_geom *g, const vrs_knobs 
                                            (const float *)part, (float *)sF.dev, ctx->stream));
         return sF.finish();
     }
-    // segments of the visibility range: enough work items to fill the chip,
-    // a multiple of 8 so that each XCD keeps to its own segments
-    // (independent of n_maps, so that each map's sums are grouped the same way)
     const int64_t units = (int64_t)n_maps * g->n_chunk;
-    // a multiple of 8 segments, so that the blocks on each XCD stream their own segments
-    int64_t n_seg = (4096 + g->n_chunk - 1) / g->n_chunk;
-    n_seg = (n_seg + 7) / 8 * 8;
-    while (n_seg > 8 && g->n_vis / n_seg < 1024) n_seg -= 8;
-    n_seg = std::min<int64_t>(n_seg, 1024);
+    const int64_t n_seg = recon_segments(g);
     void *part;
     if (vrs_status s = ws_get(ctx, 1, sizeof(float) * n_seg * n_maps * g->n_pad, &part)) return s;
     const GeomDev gd = g->dev();
@@ -916,6 +958,40 @@ vrs_status vrs_adjoint(vrs_ctx *ctx, const vrs_geom *g, const vrs_knobs *k, cons
     vrs_knobs kk = *k;
     if (kk.precision == VRS_MFMA_SPLIT16) kk.precision = VRS_MIXED_F64PHASE;
     return recon_common(ctx, g, &kk, 1, resid, nullptr, grad, 2.0, "adjoint");
+}
+
+vrs_status vrs_recon_noalloc(vrs_ctx *ctx, const vrs_geom *g, const vrs_knobs *k, int32_t n_maps,
+                             const float *V, const float *w, float *F) {
+    if (vrs_status s = check_ctx(ctx)) return s;
+    if (vrs_status s = check_geom(ctx, g)) return s;
+    if (vrs_status s = check_knobs(ctx, k)) return s;
+    if (k->precision == VRS_MFMA_SPLIT16)
+        return fail(ctx, VRS_E_ARG, "vrs_recon_noalloc runs VRS_PLAIN_F32 or VRS_MIXED_F64PHASE");
+    if (n_maps < 1 || n_maps > 65536) return fail(ctx, VRS_E_ARG, "n_maps must be in [1, 65536]");
+    if (!V || !F) return fail(ctx, VRS_E_ARG, "V and F must be non-NULL");
+    if (!ctx->noalloc_scratch) return fail(ctx, VRS_E_STATE, "no recon scratch on this device");
+    DeviceGuard dg(ctx->device);
+    // host arrays would need staging buffers
+    if (!is_device_ptr(V) || !is_device_ptr(F) || (w && !is_device_ptr(w)))
+        return fail(ctx, VRS_E_ARG, "vrs_recon_noalloc takes device arrays only (V, w, F)");
+    const int64_t n_seg = recon_segments(g);
+    const uint64_t need = (uint64_t)n_seg * (uint64_t)n_maps * (uint64_t)g->n_pad * sizeof(float);
+    if (need > (uint64_t)VRS_NOALLOC_SCRATCH_BYTES)
+        return fail(ctx, VRS_E_ARG,
+                    "recon of %d map(s) over %lld padded pixels needs %llu bytes of partial "
+                    "images; vrs_recon_noalloc holds %llu (use vrs_recon or fewer maps)",
+                    n_maps, (long long)g->n_pad, (unsigned long long)need,
+                    (unsigned long long)VRS_NOALLOC_SCRATCH_BYTES);
+    const double kappa = k->kappa > 0 ? k->kappa : 1.0 / (double)g->n_vis;
+    const GeomDev gd = g->dev();
+    const bool f64 = k->precision == VRS_MIXED_F64PHASE;
+    const int grid = persistent_grid(ctx, recon_kernel_ptr(f64), (int64_t)n_maps * g->n_chunk * n_seg);
+    // every (slot, chunk) item writes its whole partial-image block: nothing to clear first
+    VRS_HIP(ctx, launch_recon(gd, phase_knobs(k), f64, n_maps, (const float2 *)V, w, (float)kappa,
+                              (int32_t)n_seg, ctx->noalloc_scratch, grid, ctx->stream));
+    VRS_HIP(ctx, launch_recon_finalize(gd, n_maps, (int32_t)n_seg, g->n_pix,
+                                       ctx->noalloc_scratch, F, ctx->stream));
+    return VRS_OK;
 }
 
 }  // extern "C"
