@@ -58,6 +58,12 @@ typedef struct vrs_orbit_desc {
     double mu_km3_s2;      /* lunar GM; mean motion sqrt(mu / a^3) */
     const double *raan_rad; /* [n_sat], host memory */
     const double *m0_rad;   /* [n_sat], host memory: mean anomaly at t = 0 */
+    /* [n_t] host memory or NULL (then step j is at j * dt_s): step j is taken at
+     * time sample_index[j] * dt_s; a negative entry is a zero-filled sample
+     * (position 0, shades nothing), as the reference writes for orbit windows
+     * past the end of its propagated span.  vrs_uvw_position_index() gives the
+     * reference's one-third-orbit windows. */
+    const int64_t *sample_index;
 } vrs_orbit_desc;
 
 /* Everything that fixes the (visibility, pixel) term set. */
@@ -68,7 +74,31 @@ typedef struct vrs_geom_desc {
     int32_t n_base;        /* 0: all pairs a < b in lexicographic order */
     const int32_t *pairs;  /* [n_base][2] satellite indices (host), or NULL with n_base = 0 */
     int32_t nside;         /* HEALPix RING sky, N_pix = 12 nside^2 */
+    int32_t uvw_flags;     /* VRS_UVW_CEIL | VRS_UVW_LAMBDA_INT, or 0 */
 } vrs_geom_desc;
+
+/* uvw_flags: the reference's gridded uvw (computeUVW,
+ * MIncline_generate_uvw_float.cu:156-158, 190): VRS_UVW_CEIL rounds each
+ * component up to an integer number of wavelengths, VRS_UVW_LAMBDA_INT uses
+ * lambda = (int)(3e8 / f) metres instead of c / f. */
+#define VRS_UVW_CEIL 1
+#define VRS_UVW_LAMBDA_INT 2
+
+/* A geometry from an explicit list of uvw rows instead of orbits: the
+ * reference's stage-3/4 input (the u v w [count] rows of its updated_uvw files,
+ * read by vissGen10M_shade.cu:351-373), one visibility per row (v = row).
+ * There are no satellites, so no occultation: the reference's own masking in
+ * visscal and imagerecon never shades a pixel on these inputs (its fine-grid
+ * shade loop runs only when M3 <= 0, vissGen10M_shade.cu:130, and its recon
+ * skip only at grid pixel (1,1), :165).  Sky: HEALPix RING of `nside` when lmn
+ * is NULL, else n_pix explicit unit directions (pixel p = row p of lmn). */
+typedef struct vrs_uvw_geom_desc {
+    int64_t n_rows;
+    const double *uvw;     /* [n_rows][3] wavelengths, host memory */
+    int32_t nside;         /* used when lmn is NULL */
+    int64_t n_pix;         /* used when lmn is non-NULL */
+    const double *lmn;     /* [n_pix][3] host memory, or NULL */
+} vrs_uvw_geom_desc;
 
 typedef struct vrs_geom_info {
     int64_t n_vis;         /* n_base * n_t; visibility index v = b * n_t + t */
@@ -126,6 +156,12 @@ vrs_status vrs_term_ceiling(vrs_ctx *ctx, double *gterms_per_s);
  * chunks.  Replaces MIncline() (MIncline_generate_uvw_float.cu:184-440) up to the
  * uvw it writes, and the l,m,n inputs of vissGen (vissGen10M_shade.cu:199-248). */
 vrs_status vrs_geom_create(vrs_ctx *ctx, const vrs_geom_desc *desc, vrs_geom **out);
+/* The uvw-list geometry (vrs_uvw_geom_desc).  vrs_visgen, vrs_recon (with the
+ * reference's 1/count weights and kappa = 1/n_rows), vrs_adjoint,
+ * vrs_recon_noalloc, the fine-cell sky and vrs_shade (all counts 0) accept it;
+ * the orbit-based calls (satellite positions, vrs_grid_uvw, batches) return
+ * VRS_E_STATE. */
+vrs_status vrs_geom_create_uvw(vrs_ctx *ctx, const vrs_uvw_geom_desc *desc, vrs_geom **out);
 /* Destroy a geometry; call before destroying the context that created it. */
 void vrs_geom_destroy(vrs_geom *g);
 vrs_status vrs_geom_info_get(const vrs_geom *g, vrs_geom_info *out);
@@ -319,6 +355,17 @@ vrs_status vrs_sky_lognormal(uint64_t seed, double sigma, int64_t n_pix, int32_t
  * from `seed`, as mean-anomaly offsets 2 asin(d_k / (2 a)) (MIncline_generate_uvw_float.cu:51-58). */
 vrs_status vrs_orbit_offsets(int32_t n_sat, uint64_t seed, double a_km, double max_offset_km,
                              double *raan_rad, double *m0_rad);
+/* The reference's orbit windows (uvwPosition, MIncline_generate_uvw_float.cu:113-151,
+ * constants :185-190: span = orbit_res / 3 samples of each orbit, starting
+ * interval = orbit_res / 36 samples later on every successive orbit, wrapped
+ * back by whole orbits once past the propagated span, zero-filled where a window
+ * runs off its end).  Writes orbit_counts * (orbit_res / 3) entries: the source
+ * sample index (orbit k, step po -> k * orbit_res + po) of each kept step, or -1
+ * for a zero-filled one; feed it to vrs_orbit_desc.sample_index.  The reference
+ * tests the end with `>` (:134), reading one sample past the span; here that
+ * sample is zero-filled too. */
+vrs_status vrs_uvw_position_index(int32_t orbit_res, int32_t orbit_counts, int32_t interval,
+                                  int32_t span, int64_t *out);
 /* HEALPix RING pix2vec [12 nside^2][3] (Gorski et al. 2005). */
 vrs_status vrs_healpix_lmn(int32_t nside, double *lmn);
 /* NESTED -> RING index map [12 nside^2]. */

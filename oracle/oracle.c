@@ -15,6 +15,11 @@
  *                                      e = 0 so E = M = nu; FP64; time via mean motion sqrt(mu/a^3)
  *   vo_uvw ........................... computeUVW (MIncline_generate_uvw_float.cu:153-160),
  *                                      without ceil, lambda = c/f
+ *   vo_uvw_position_index ............ uvwPosition's one-third-orbit windows
+ *                                      (MIncline_generate_uvw_float.cu:113-151)
+ *   (pos == NULL in vo_visgen, vo_visgen_fine, vo_recon: the reference's
+ *    stage-3/4 inputs, uvw rows without satellites, on which its own shading
+ *    never masks a pixel, vissGen10M_shade.cu:130, 165)
  *   vo_occ / vo_mask_* ............... the occultation ("shade") predicate: ray from a
  *                                      satellite towards a pixel hits the closed lunar sphere.
  *                                      Replaces ShadeMatrix's box (shade10M.cu:73-105) with the
@@ -219,6 +224,32 @@ void vo_uvw(int64_t n_t, const double *pos, int n_base, const int *pairs, double
     }
 }
 
+/* uvwPosition (MIncline_generate_uvw_float.cu:113-151) as index arithmetic:
+ * for each output step g of orbit k (1-based), the source sample in the
+ * propagated series of OR * OC samples, or -1 where the reference writes 0.
+ * The reference zero-fills only circleIdx > OR*OC (:134) and so reads one
+ * sample past the series at circleIdx == OR*OC; here that sample is -1 too. */
+void vo_uvw_position_index(int OR, int OC, int interval, int span, int64_t *out)
+{
+    const int64_t n_out = (int64_t)OC * (OR / 3);
+    for (int64_t g = 0; g < n_out; ++g) {
+        int64_t k = g / (OR / 3) + 1;
+        int64_t offset = (k - 1) * interval;
+        int64_t endIdx = (k - 1) * OR + offset + span;
+        int64_t startIdx = (k - 1) * OR + offset;
+        if ((k - 1) * ((int64_t)OR + interval) > (int64_t)OR * OC) {
+            int64_t reset_index = ((int64_t)OR * OC) / ((int64_t)OR + interval) + 1;
+            startIdx = (k - 1 - reset_index) * OR + offset;
+            endIdx = (k - 1 - reset_index) * OR + offset + span;
+        }
+        int64_t circleIdx = startIdx + (g - (k - 1) * span);
+        if (endIdx > (int64_t)OR * OC && circleIdx >= (int64_t)OR * OC)
+            out[g] = -1;
+        else
+            out[g] = circleIdx;
+    }
+}
+
 /* ------------------------------------------------------------------ mask */
 /* Ray from r (km) along unit s hits the closed sphere of radius R:
  *   d = r.s < 0  and  q = |r|^2 - d^2 <= R^2.
@@ -336,9 +367,10 @@ void vo_visgen(int64_t n_t, const double *pos, const int *pairs, const double *u
     for (int64_t k = 0; k < n_sel; ++k) {
         int64_t v = vis_idx ? vis_idx[k] : k;
         int64_t b = v / n_t, t = v % n_t;
-        const double *ra = pos + ((int64_t)pairs[2 * b] * n_t + t) * 3;
-        const double *rb = pos + ((int64_t)pairs[2 * b + 1] * n_t + t) * 3;
-        double ra2 = vo_r2(ra), rb2 = vo_r2(rb);
+        /* pos == NULL: uvw rows without satellites, nothing is occulted */
+        const double *ra = pos ? pos + ((int64_t)pairs[2 * b] * n_t + t) * 3 : NULL;
+        const double *rb = pos ? pos + ((int64_t)pairs[2 * b + 1] * n_t + t) * 3 : NULL;
+        double ra2 = pos ? vo_r2(ra) : 0.0, rb2 = pos ? vo_r2(rb) : 0.0;
         double u = uvw[3 * v] * s, vv = uvw[3 * v + 1] * s, w = uvw[3 * v + 2] * s;
         double wre = 1.0, wim = 0.0;
         if (wfac) {
@@ -351,7 +383,8 @@ void vo_visgen(int64_t n_t, const double *pos, const int *pairs, const double *u
         double *acc = (double *)calloc(2 * (size_t)n_maps, sizeof(double));
         for (int64_t p = 0; p < n_pix; ++p) {
             int na = 0, nb = 0;
-            int m = vo_occ(ra, ra2, lmn + 3 * p, &L, &na) | vo_occ(rb, rb2, lmn + 3 * p, &L, &nb);
+            int m = pos && (vo_occ(ra, ra2, lmn + 3 * p, &L, &na) |
+                            vo_occ(rb, rb2, lmn + 3 * p, &L, &nb));
             double phi = u * lmn[3 * p] + vv * lmn[3 * p + 1] + w * lmn_nm1[p];
             double r = phi - rint(phi);
             double c = cos(2.0 * VO_PI * r), sn = sin(2.0 * VO_PI * r);
@@ -407,9 +440,10 @@ void vo_visgen_fine(int64_t n_t, const double *pos, const int *pairs, const doub
     for (int64_t k = 0; k < n_sel; ++k) {
         int64_t v = vis_idx ? vis_idx[k] : k;
         int64_t b = v / n_t, t = v % n_t;
-        const double *ra = pos + ((int64_t)pairs[2 * b] * n_t + t) * 3;
-        const double *rb = pos + ((int64_t)pairs[2 * b + 1] * n_t + t) * 3;
-        double ra2 = vo_r2(ra), rb2 = vo_r2(rb);
+        /* pos == NULL: uvw rows without satellites, nothing is occulted */
+        const double *ra = pos ? pos + ((int64_t)pairs[2 * b] * n_t + t) * 3 : NULL;
+        const double *rb = pos ? pos + ((int64_t)pairs[2 * b + 1] * n_t + t) * 3 : NULL;
+        double ra2 = pos ? vo_r2(ra) : 0.0, rb2 = pos ? vo_r2(rb) : 0.0;
         double u = uvw[3 * v] * s, vv = uvw[3 * v + 1] * s, w = uvw[3 * v + 2] * s;
         double wre = 1.0, wim = 0.0;
         if (wfac) {
@@ -423,8 +457,8 @@ void vo_visgen_fine(int64_t n_t, const double *pos, const int *pairs, const doub
             for (int32_t j = 0; j < count[p]; ++j) {
                 int32_t c = idx[start[p] + j];
                 int na = 0, nb = 0;
-                int m = vo_occ(ra, ra2, cell_lmn + 3 * (int64_t)c, &L, &na) |
-                        vo_occ(rb, rb2, cell_lmn + 3 * (int64_t)c, &L, &nb);
+                int m = pos && (vo_occ(ra, ra2, cell_lmn + 3 * (int64_t)c, &L, &na) |
+                                vo_occ(rb, rb2, cell_lmn + 3 * (int64_t)c, &L, &nb));
                 sum += m ? t_occ : (double)ff[c];
             }
             double ip = count[p] > 0 ? sum / count[p] : 0.0;
@@ -460,11 +494,13 @@ void vo_recon(int64_t n_t, const double *pos, const int *pairs, int n_base, cons
         double *acc = (double *)calloc((size_t)n_maps, sizeof(double));
         for (int64_t v = 0; v < n_vis; ++v) {
             int64_t b = v / n_t, t = v % n_t;
-            const double *ra = pos + ((int64_t)pairs[2 * b] * n_t + t) * 3;
-            const double *rb = pos + ((int64_t)pairs[2 * b + 1] * n_t + t) * 3;
-            int na = 0, nb = 0;
-            if (vo_occ(ra, vo_r2(ra), sp, &L, &na) | vo_occ(rb, vo_r2(rb), sp, &L, &nb))
-                continue;
+            if (pos) { /* pos == NULL: uvw rows without satellites, nothing is occulted */
+                const double *ra = pos + ((int64_t)pairs[2 * b] * n_t + t) * 3;
+                const double *rb = pos + ((int64_t)pairs[2 * b + 1] * n_t + t) * 3;
+                int na = 0, nb = 0;
+                if (vo_occ(ra, vo_r2(ra), sp, &L, &na) | vo_occ(rb, vo_r2(rb), sp, &L, &nb))
+                    continue;
+            }
             double u = uvw[3 * v] * s, vv = uvw[3 * v + 1] * s, w = uvw[3 * v + 2] * s;
             double phi = u * sp[0] + vv * sp[1] + w * nterm;
             double r = phi - rint(phi);

@@ -56,10 +56,10 @@ def _load():
                                 C.c_void_p, _i32, _u32, C.POINTER(C.c_int64)]
     L.vo_mask_bits.argtypes = [C.c_int64, _f64, _i32, C.c_int64, _f64, C.c_double, C.c_int64,
                                C.c_void_p, _u32]
-    L.vo_visgen.argtypes = [C.c_int64, _f64, _i32, _f64, C.c_int64, _f64, C.c_int, _f32,
+    L.vo_visgen.argtypes = [C.c_int64, C.c_void_p, C.c_void_p, _f64, C.c_int64, _f64, C.c_int, _f32,
                             C.c_double, C.c_double, C.c_int, C.c_double, C.c_int64, C.c_void_p,
                             _f64]
-    L.vo_recon.argtypes = [C.c_int64, _f64, _i32, C.c_int, _f64, C.c_int64, _f64, C.c_int,
+    L.vo_recon.argtypes = [C.c_int64, C.c_void_p, C.c_void_p, C.c_int, _f64, C.c_int64, _f64, C.c_int,
                            _f64, C.c_void_p, C.c_double, C.c_double, C.c_int, C.c_double,
                            C.c_int64, C.c_void_p, _f64]
     L.vo_grid_keys.restype = C.c_int64
@@ -68,9 +68,10 @@ def _load():
     L.vo_grid_unique.restype = C.c_int64
     L.vo_grid_unique.argtypes = [C.c_int64, _P(dtype=np.uint64, flags="C_CONTIGUOUS"), _f32, _i32]
     L.vo_fine_index.argtypes = [C.c_int64, _i32, C.c_int32, C.c_int64, _i32, _i64, _i32]
-    L.vo_visgen_fine.argtypes = [C.c_int64, _f64, _i32, _f64, C.c_int64, _f64, C.c_int64, _f64,
+    L.vo_visgen_fine.argtypes = [C.c_int64, C.c_void_p, C.c_void_p, _f64, C.c_int64, _f64, C.c_int64, _f64,
                                  _f32, _i32, _i64, _i32, C.c_double, C.c_double, C.c_int,
                                  C.c_double, C.c_int64, C.c_void_p, _f64]
+    L.vo_uvw_position_index.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _i64]
     L.vo_shade_box.argtypes = [C.c_int64, _f32, _f32, _f32, C.c_int, C.c_float, _i32]
     B = C.CDLL(sb)
     B.vb_visgen_f32.argtypes = [C.c_int64, _f32, _i32, _f32, C.c_int64, _f32, _f32, C.c_float,
@@ -204,7 +205,8 @@ def visgen(pos, pairs, uvw_, lmn, I, R, s=1.0, wfac=0, t_occ=240.0, vis_sel=None
     sel, selp = _idx(vis_sel)
     n_sel = len(sel) if sel is not None else len(pairs) * n_t
     out = np.empty((I.shape[0], n_sel, 2), np.float64)
-    _lib.vo_visgen(n_t, pos, np.ascontiguousarray(pairs, np.int32),
+    pairs = np.ascontiguousarray(pairs, np.int32)
+    _lib.vo_visgen(n_t, pos.ctypes.data, pairs.ctypes.data,
                    np.ascontiguousarray(uvw_, np.float64).reshape(-1), len(lmn),
                    np.ascontiguousarray(lmn, np.float64), I.shape[0], I, R, s, wfac, t_occ,
                    n_sel, selp, out)
@@ -234,7 +236,8 @@ def visgen_fine(pos, pairs, uvw_, lmn, cell_lmn, ff, nx, R, s=1.0, wfac=0, t_occ
     sel, selp = _idx(vis_sel)
     n_sel = len(sel) if sel is not None else len(pairs) * n_t
     out = np.empty((n_sel, 2), np.float64)
-    _lib.vo_visgen_fine(n_t, pos, np.ascontiguousarray(pairs, np.int32),
+    pairs = np.ascontiguousarray(pairs, np.int32)
+    _lib.vo_visgen_fine(n_t, pos.ctypes.data, pairs.ctypes.data,
                         np.ascontiguousarray(uvw_, np.float64).reshape(-1), len(lmn),
                         np.ascontiguousarray(lmn, np.float64), len(cell_lmn),
                         np.ascontiguousarray(cell_lmn, np.float64),
@@ -258,10 +261,75 @@ def recon(pos, pairs, uvw_, lmn, V, R, s=1.0, wfac=0, kappa=None, weights=None, 
     w = None if weights is None else np.ascontiguousarray(weights, np.float32)
     wp = None if w is None else w.ctypes.data_as(C.c_void_p)
     out = np.empty((V.shape[0], n_sel), np.float64)
-    _lib.vo_recon(n_t, pos, np.ascontiguousarray(pairs, np.int32), len(pairs),
+    pairs = np.ascontiguousarray(pairs, np.int32)
+    _lib.vo_recon(n_t, pos.ctypes.data, pairs.ctypes.data, len(pairs),
                   np.ascontiguousarray(uvw_, np.float64).reshape(-1), len(lmn),
                   np.ascontiguousarray(lmn, np.float64), V.shape[0], Vr, wp, R, s, wfac,
                   kappa, n_sel, selp, out)
+    return out
+
+
+# ------------------------------------- the reference's stage-3/4 inputs
+def visgen_rows(uvw_rows, lmn, I, s=1.0, wfac=0, vis_sel=None):
+    """visscal over explicit uvw rows (vissGen10M_shade.cu:351-373 read them from
+    updated_uvw) with no occultation, FP64; complex128 [maps][n_sel]."""
+    _load()
+    I = np.ascontiguousarray(np.atleast_2d(I), np.float32)
+    uvw_rows = np.ascontiguousarray(uvw_rows, np.float64).reshape(-1, 3)
+    n = len(uvw_rows)
+    sel, selp = _idx(vis_sel)
+    n_sel = len(sel) if sel is not None else n
+    out = np.empty((I.shape[0], n_sel, 2), np.float64)
+    _lib.vo_visgen(n, None, None, uvw_rows.reshape(-1), len(lmn),
+                   np.ascontiguousarray(lmn, np.float64), I.shape[0], I, 0.0, s, wfac, 0.0,
+                   n_sel, selp, out)
+    return out[..., 0] + 1j * out[..., 1]
+
+
+def recon_rows(uvw_rows, lmn, V, s=1.0, wfac=0, kappa=None, weights=None, pix_sel=None):
+    """imagerecon over explicit uvw rows with no occultation, FP64; kappa defaults
+    to 1 / n_rows (the reference divides by its row count, vissGen10M_shade.cu:173)."""
+    _load()
+    V = np.atleast_2d(np.asarray(V))
+    Vr = np.ascontiguousarray(np.stack([V.real, V.imag], -1), np.float64)
+    uvw_rows = np.ascontiguousarray(uvw_rows, np.float64).reshape(-1, 3)
+    n = len(uvw_rows)
+    kappa = 1.0 / n if kappa is None else kappa
+    sel, selp = _idx(pix_sel)
+    n_sel = len(sel) if sel is not None else len(lmn)
+    w = None if weights is None else np.ascontiguousarray(weights, np.float32)
+    wp = None if w is None else w.ctypes.data_as(C.c_void_p)
+    out = np.empty((V.shape[0], n_sel), np.float64)
+    _lib.vo_recon(n, None, None, 1, uvw_rows.reshape(-1), len(lmn),
+                  np.ascontiguousarray(lmn, np.float64), V.shape[0], Vr, wp, 0.0, s, wfac,
+                  kappa, n_sel, selp, out)
+    return out
+
+
+def visgen_fine_rows(uvw_rows, lmn, cell_lmn, ff, nx, s=1.0, wfac=0, nx_base=1, vis_sel=None):
+    """visscal's fine-cell sum (vissGen10M_shade.cu:121-138) over explicit uvw rows,
+    no occultation, FP64; complex128 [n_sel]."""
+    _load()
+    uvw_rows = np.ascontiguousarray(uvw_rows, np.float64).reshape(-1, 3)
+    n = len(uvw_rows)
+    count, start, idx = fine_index(nx, len(lmn), nx_base)
+    sel, selp = _idx(vis_sel)
+    n_sel = len(sel) if sel is not None else n
+    out = np.empty((n_sel, 2), np.float64)
+    _lib.vo_visgen_fine(n, None, None, uvw_rows.reshape(-1), len(lmn),
+                        np.ascontiguousarray(lmn, np.float64), len(cell_lmn),
+                        np.ascontiguousarray(cell_lmn, np.float64),
+                        np.ascontiguousarray(ff, np.float32), count, start, idx, 0.0, s, wfac,
+                        0.0, n_sel, selp, out)
+    return out[:, 0] + 1j * out[:, 1]
+
+
+def uvw_position_index(orbit_res, orbit_counts, interval, span):
+    """uvwPosition's windows (MIncline_generate_uvw_float.cu:113-151): source sample
+    of each kept step, -1 where zero-filled."""
+    _load()
+    out = np.empty(orbit_counts * (orbit_res // 3), np.int64)
+    _lib.vo_uvw_position_index(orbit_res, orbit_counts, interval, span, out)
     return out
 
 

@@ -18,6 +18,8 @@ VRS_MIXED_F64PHASE = 1
 VRS_MFMA_SPLIT16 = 2
 VRS_GRID_ORDERED = 1
 VRS_GRID_LAMBDA_INT = 2
+VRS_UVW_CEIL = 1
+VRS_UVW_LAMBDA_INT = 2
 
 
 class VrsError(RuntimeError):
@@ -32,12 +34,19 @@ class VrsError(RuntimeError):
 class OrbitDesc(C.Structure):
     _fields_ = [("n_sat", C.c_int32), ("n_t", C.c_int32), ("dt_s", C.c_double),
                 ("a_km", C.c_double), ("incl_rad", C.c_double), ("mu_km3_s2", C.c_double),
-                ("raan_rad", C.POINTER(C.c_double)), ("m0_rad", C.POINTER(C.c_double))]
+                ("raan_rad", C.POINTER(C.c_double)), ("m0_rad", C.POINTER(C.c_double)),
+                ("sample_index", C.POINTER(C.c_int64))]
 
 
 class GeomDesc(C.Structure):
     _fields_ = [("orbit", OrbitDesc), ("freq_hz", C.c_double), ("r_moon_km", C.c_double),
-                ("n_base", C.c_int32), ("pairs", C.POINTER(C.c_int32)), ("nside", C.c_int32)]
+                ("n_base", C.c_int32), ("pairs", C.POINTER(C.c_int32)), ("nside", C.c_int32),
+                ("uvw_flags", C.c_int32)]
+
+
+class UvwGeomDesc(C.Structure):
+    _fields_ = [("n_rows", C.c_int64), ("uvw", C.POINTER(C.c_double)), ("nside", C.c_int32),
+                ("n_pix", C.c_int64), ("lmn", C.POINTER(C.c_double))]
 
 
 class GeomInfo(C.Structure):
@@ -84,6 +93,7 @@ SIGNATURES = {
     "vrs_timing_last_ms": (C.c_double, [_ctx, C.c_char_p]),
     "vrs_term_ceiling": (C.c_int, [_ctx, C.POINTER(C.c_double)]),
     "vrs_geom_create": (C.c_int, [_ctx, C.POINTER(GeomDesc), C.POINTER(C.c_void_p)]),
+    "vrs_geom_create_uvw": (C.c_int, [_ctx, C.POINTER(UvwGeomDesc), C.POINTER(C.c_void_p)]),
     "vrs_geom_destroy": (None, [_vp]),
     "vrs_geom_info_get": (C.c_int, [_vp, C.POINTER(GeomInfo)]),
     "vrs_orbit_uvw": (C.c_int, [_ctx, _vp, _vp, _vp]),
@@ -123,6 +133,7 @@ SIGNATURES = {
     "vrs_sky_lognormal": (C.c_int, [C.c_uint64, C.c_double, C.c_int64, C.c_int32, C.c_double,
                                     C.c_double, _vp]),
     "vrs_orbit_offsets": (C.c_int, [C.c_int32, C.c_uint64, C.c_double, C.c_double, _vp, _vp]),
+    "vrs_uvw_position_index": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, _vp]),
     "vrs_healpix_lmn": (C.c_int, [C.c_int32, _vp]),
     "vrs_healpix_nest2ring": (C.c_int, [C.c_int32, _vp]),
 }

@@ -123,9 +123,22 @@ class Context:
 
     def geometry(self, n_sat, n_t, freq_hz, nside, raan=None, m0=None, dt_s=W.DT_S,
                  a_km=W.A_KM, incl_rad=W.INCL_RAD, mu=W.MU_MOON, r_moon_km=W.R_MOON_KM,
-                 pairs=None):
+                 pairs=None, sample_index=None, uvw_ceil=False, lambda_int=False):
+        """Orbit geometry.  sample_index [n_t] (optional): step j at time
+        sample_index[j] * dt_s, negative = zero-filled (see uvw_position_index);
+        uvw_ceil / lambda_int: the reference's gridded uvw, ceil(dx * 1e3 / lambda)
+        with lambda = int(3e8 / f) (MIncline_generate_uvw_float.cu:156-158, 190)."""
+        flags = (_lib.VRS_UVW_CEIL if uvw_ceil else 0) | (_lib.VRS_UVW_LAMBDA_INT if lambda_int else 0)
         return Geometry(self, n_sat, n_t, freq_hz, nside, raan, m0, dt_s, a_km, incl_rad, mu,
-                        r_moon_km, pairs)
+                        r_moon_km, pairs, sample_index, flags)
+
+    def uvw_geometry(self, uvw, nside=None, lmn=None):
+        """Geometry from explicit uvw rows [n][3] (wavelengths) -- the reference's
+        updated_uvw input of stages 3-4 (vissGen10M_shade.cu:351-373) -- over a
+        HEALPix RING sky of `nside` or explicit unit directions lmn [n_pix][3].
+        No satellites, hence no occultation (the reference's own masks are no-ops
+        on these inputs, vissGen10M_shade.cu:130, 165)."""
+        return Geometry.from_uvw(self, uvw, nside, lmn)
 
     def workload_geometry(self, wl):
         return self.geometry(wl.n_sat, wl.n_t, wl.freq_hz, wl.nside)
@@ -164,7 +177,7 @@ class Geometry:
     """Orbits, uvw, shade caps and the chunked HEALPix sky on one device (vrs_geom)."""
 
     def __init__(self, ctx, n_sat, n_t, freq_hz, nside, raan, m0, dt_s, a_km, incl_rad, mu,
-                 r_moon_km, pairs):
+                 r_moon_km, pairs, sample_index=None, uvw_flags=0):
         self.ctx = ctx
         lib = ctx._lib
         if raan is None or m0 is None:
@@ -192,17 +205,56 @@ class Geometry:
             self._pairs = W.all_pairs(n_sat)
             d.n_base = 0
         d.nside = nside
+        d.uvw_flags = uvw_flags
+        if sample_index is not None:
+            self.sample_index = np.ascontiguousarray(sample_index, np.int64)
+            if self.sample_index.shape != (n_t,):
+                raise ValueError(f"sample_index must have {n_t} entries")
+            d.orbit.sample_index = self.sample_index.ctypes.data_as(C.POINTER(C.c_int64))
         h = C.c_void_p()
         ctx.check(lib.vrs_geom_create(ctx.handle, C.byref(d), C.byref(h)))
+        self._adopt(h, r_moon_km)
+        self.pairs = self._pairs
+
+    def _adopt(self, h, r_moon_km):
+        ctx = self.ctx
         self.handle = h
         ctx._geoms.add(self)
         info = _lib.GeomInfo()
-        ctx.check(lib.vrs_geom_info_get(h, C.byref(info)))
+        ctx.check(ctx._lib.vrs_geom_info_get(h, C.byref(info)))
         self.n_vis, self.n_pix = info.n_vis, info.n_pix
         self.n_sat, self.n_t, self.n_base = info.n_sat, info.n_t, info.n_base
         self.nside, self.n_chunk, self.lambda_m = info.nside, info.n_chunk, info.lambda_m
         self.r_moon_km = r_moon_km
-        self.pairs = self._pairs
+
+    @classmethod
+    def from_uvw(cls, ctx, uvw, nside=None, lmn=None):
+        self = cls.__new__(cls)
+        self.ctx = ctx
+        self.uvw_rows = np.ascontiguousarray(uvw, np.float64).reshape(-1, 3)
+        d = _lib.UvwGeomDesc()
+        d.n_rows = len(self.uvw_rows)
+        d.uvw = self.uvw_rows.ctypes.data_as(C.POINTER(C.c_double))
+        if lmn is not None:
+            self._lmn = np.ascontiguousarray(lmn, np.float64).reshape(-1, 3)
+            d.n_pix = len(self._lmn)
+            d.lmn = self._lmn.ctypes.data_as(C.POINTER(C.c_double))
+        elif nside is None:
+            raise ValueError("give nside or lmn")
+        else:
+            d.nside = nside
+        h = C.c_void_p()
+        ctx.check(ctx._lib.vrs_geom_create_uvw(ctx.handle, C.byref(d), C.byref(h)))
+        self._adopt(h, 0.0)
+        self.pairs = None
+        self.raan = self.m0 = None
+        return self
+
+    def uvw(self):
+        """uvw of every visibility [n_vis][3] (FP64 wavelengths), as the kernels use them."""
+        out = np.empty((self.n_vis, 3), np.float64)
+        self.ctx.check(self.ctx._lib.vrs_orbit_uvw(self.ctx.handle, self.handle, None, _ptr(out)))
+        return out
 
     # ------------------------------------------------------------- stage 1
     def MIncline(self, export=True):

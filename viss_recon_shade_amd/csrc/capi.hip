@@ -86,11 +86,14 @@ struct vrs_geom {
     int32_t n_sat = 0, n_t = 0, n_base = 0, nside = 0, n_chunk = 0;
     int64_t n_vis = 0, n_pix = 0, n_pad = 0;
     double lambda_m = 0, r_moon = 0, freq_hz = 0;
+    int32_t uvw_flags = 0;
+    bool from_uvw = false;  // built from an explicit uvw row list: no satellites, no orbits
     OrbitArgs orb{};
     std::vector<int32_t> pairs;
     std::vector<double> lmn_ring;  // [n_pix][3] host copy, RING order
     // device
     double *d_orbit = nullptr;  // raan[n_sat], m0[n_sat]
+    int64_t *d_tidx = nullptr;  // [n_t] sample index of each step, or null
     double4 *d_pos = nullptr;
     CapRec *d_cap = nullptr;
     double4 *d_uvw64 = nullptr;
@@ -487,6 +490,20 @@ vrs_status vrs_orbit_offsets(int32_t n_sat, uint64_t seed, double a_km, double m
     return VRS_OK;
 }
 
+vrs_status vrs_uvw_position_index(int32_t orbit_res, int32_t orbit_counts, int32_t interval,
+                                  int32_t span, int64_t *out) {
+    if (orbit_res < 3 || orbit_counts < 1 || interval < 0 || span < 1 || !out)
+        return fail(nullptr, VRS_E_ARG, "need orbit_res >= 3, orbit_counts >= 1, interval >= 0, "
+                                        "span >= 1 and out");
+    if (span != orbit_res / 3)
+        return fail(nullptr, VRS_E_ARG, "span must be orbit_res / 3 (each output orbit block is "
+                                        "orbit_res / 3 steps long)");
+    if ((int64_t)orbit_res * orbit_counts > ((int64_t)1 << 40))
+        return fail(nullptr, VRS_E_ARG, "orbit_res * orbit_counts too large");
+    uvw_position_index(orbit_res, orbit_counts, interval, span, out);
+    return VRS_OK;
+}
+
 vrs_status vrs_healpix_lmn(int32_t nside, double *lmn) {
     if (nside < 1 || nside > 8192 || !lmn) return fail(nullptr, VRS_E_ARG, "bad nside or lmn");
     const int64_t n = 12LL * nside * nside;
@@ -507,45 +524,33 @@ void vrs_geom_destroy(vrs_geom *g) {
     if (!g) return;
     DeviceGuard dg(g->device);
     (void)hipDeviceSynchronize();
-    void *ptrs[] = {g->d_orbit, g->d_pos,  g->d_cap,  g->d_uvw64, g->d_viscap, g->d_pairs,
-                    g->d_pix32, g->d_pix64, g->d_ring, g->d_chunk};
+    void *ptrs[] = {g->d_orbit, g->d_tidx, g->d_pos,  g->d_cap,  g->d_uvw64, g->d_viscap,
+                    g->d_pairs, g->d_pix32, g->d_pix64, g->d_ring, g->d_chunk};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     g->free_fine();
     delete g;
 }
 
-static vrs_status geom_build(vrs_ctx *ctx, const vrs_geom_desc *d, vrs_geom *g) {
-    const vrs_orbit_desc &o = d->orbit;
-    g->n_sat = o.n_sat;
-    g->n_t = o.n_t;
-    g->nside = d->nside;
-    g->r_moon = d->r_moon_km;
-    g->lambda_m = kC / d->freq_hz;
-    g->freq_hz = d->freq_hz;
-    if (d->n_base == 0) {
-        for (int a = 0; a < o.n_sat; ++a)
-            for (int b = a + 1; b < o.n_sat; ++b) {
-                g->pairs.push_back(a);
-                g->pairs.push_back(b);
-            }
-    } else {
-        g->pairs.assign(d->pairs, d->pairs + 2 * d->n_base);
-    }
-    g->n_base = (int32_t)(g->pairs.size() / 2);
-    g->n_vis = (int64_t)g->n_base * g->n_t;
-    g->n_pix = 12LL * d->nside * d->nside;
+// The sky of a geometry: pixel directions in chunk order (HEALPix NESTED for a
+// HEALPix sky, so that a chunk is a compact patch; the given order for explicit
+// directions), their RING / row index, and each chunk's bounding cap.
+static vrs_status sky_build(vrs_ctx *ctx, vrs_geom *g, int32_t nside, int64_t n_pix,
+                            const double *lmn) {
+    g->nside = lmn ? 0 : nside;
+    g->n_pix = lmn ? n_pix : 12LL * nside * nside;
     g->n_chunk = (int32_t)((g->n_pix + kChunk - 1) / kChunk);
     g->n_pad = (int64_t)g->n_chunk * kChunk;
-
-    // pixels in NESTED (chunk) order
     std::vector<float4> pix32(g->n_pad, make_float4(0, 0, 0, 0));
     std::vector<double4> pix64(g->n_pad, make_double4(0, 0, 0, 0));
     std::vector<int32_t> ring(g->n_pad, -1);
     g->lmn_ring.resize(3 * g->n_pix);
-    for (int64_t p = 0; p < g->n_pix; ++p) healpix_ring_vec(d->nside, p, &g->lmn_ring[3 * p]);
+    if (lmn)
+        std::memcpy(g->lmn_ring.data(), lmn, sizeof(double) * 3 * g->n_pix);
+    else
+        for (int64_t p = 0; p < g->n_pix; ++p) healpix_ring_vec(nside, p, &g->lmn_ring[3 * p]);
     for (int64_t q = 0; q < g->n_pix; ++q) {
-        const int64_t r = healpix_nest2ring(d->nside, q);
+        const int64_t r = lmn ? q : healpix_nest2ring(nside, q);
         const double *v = &g->lmn_ring[3 * r];
         pix64[q] = make_double4(v[0], v[1], v[2], 0.0);
         pix32[q] = make_float4((float)v[0], (float)v[1], (float)v[2], 0.0f);
@@ -589,6 +594,46 @@ static vrs_status geom_build(vrs_ctx *ctx, const vrs_geom_desc *d, vrs_geom *g) 
     }
 
     hipStream_t st = ctx->stream;
+    VRS_HIP(ctx, hipMalloc(&g->d_pix32, sizeof(float4) * g->n_pad));
+    VRS_HIP(ctx, hipMalloc(&g->d_pix64, sizeof(double4) * g->n_pad));
+    VRS_HIP(ctx, hipMalloc(&g->d_ring, sizeof(int32_t) * g->n_pad));
+    VRS_HIP(ctx, hipMalloc(&g->d_chunk, sizeof(ChunkRec) * g->n_chunk));
+    VRS_HIP(ctx, hipMemcpyAsync(g->d_pix32, pix32.data(), sizeof(float4) * g->n_pad,
+                                hipMemcpyHostToDevice, st));
+    VRS_HIP(ctx, hipMemcpyAsync(g->d_pix64, pix64.data(), sizeof(double4) * g->n_pad,
+                                hipMemcpyHostToDevice, st));
+    VRS_HIP(ctx, hipMemcpyAsync(g->d_ring, ring.data(), sizeof(int32_t) * g->n_pad,
+                                hipMemcpyHostToDevice, st));
+    VRS_HIP(ctx, hipMemcpyAsync(g->d_chunk, chunks.data(), sizeof(ChunkRec) * g->n_chunk,
+                                hipMemcpyHostToDevice, st));
+    // the host-side copies above are pageable; finish them before the vectors go away
+    VRS_HIP(ctx, hipStreamSynchronize(st));
+    return VRS_OK;
+}
+
+static vrs_status geom_build(vrs_ctx *ctx, const vrs_geom_desc *d, vrs_geom *g) {
+    const vrs_orbit_desc &o = d->orbit;
+    g->n_sat = o.n_sat;
+    g->n_t = o.n_t;
+    g->r_moon = d->r_moon_km;
+    g->uvw_flags = d->uvw_flags;
+    g->lambda_m = (d->uvw_flags & VRS_UVW_LAMBDA_INT) ? (double)(int)(3e8 / d->freq_hz)
+                                                      : kC / d->freq_hz;
+    g->freq_hz = d->freq_hz;
+    if (d->n_base == 0) {
+        for (int a = 0; a < o.n_sat; ++a)
+            for (int b = a + 1; b < o.n_sat; ++b) {
+                g->pairs.push_back(a);
+                g->pairs.push_back(b);
+            }
+    } else {
+        g->pairs.assign(d->pairs, d->pairs + 2 * d->n_base);
+    }
+    g->n_base = (int32_t)(g->pairs.size() / 2);
+    g->n_vis = (int64_t)g->n_base * g->n_t;
+    if (vrs_status s = sky_build(ctx, g, d->nside, 0, nullptr)) return s;
+
+    hipStream_t st = ctx->stream;
     const int64_t nst = (int64_t)g->n_sat * g->n_t;
     VRS_HIP(ctx, hipMalloc(&g->d_orbit, 2 * sizeof(double) * g->n_sat));
     VRS_HIP(ctx, hipMalloc(&g->d_pos, sizeof(double4) * nst));
@@ -596,10 +641,11 @@ static vrs_status geom_build(vrs_ctx *ctx, const vrs_geom_desc *d, vrs_geom *g) 
     VRS_HIP(ctx, hipMalloc(&g->d_uvw64, sizeof(double4) * std::max<int64_t>(g->n_vis, 1)));
     VRS_HIP(ctx, hipMalloc(&g->d_viscap, sizeof(int2) * std::max<int64_t>(g->n_vis, 1)));
     VRS_HIP(ctx, hipMalloc(&g->d_pairs, sizeof(int2) * g->n_base));
-    VRS_HIP(ctx, hipMalloc(&g->d_pix32, sizeof(float4) * g->n_pad));
-    VRS_HIP(ctx, hipMalloc(&g->d_pix64, sizeof(double4) * g->n_pad));
-    VRS_HIP(ctx, hipMalloc(&g->d_ring, sizeof(int32_t) * g->n_pad));
-    VRS_HIP(ctx, hipMalloc(&g->d_chunk, sizeof(ChunkRec) * g->n_chunk));
+    if (o.sample_index) {
+        VRS_HIP(ctx, hipMalloc(&g->d_tidx, sizeof(int64_t) * g->n_t));
+        VRS_HIP(ctx, hipMemcpyAsync(g->d_tidx, o.sample_index, sizeof(int64_t) * g->n_t,
+                                    hipMemcpyHostToDevice, st));
+    }
 
     std::vector<double> orbit(2 * g->n_sat);
     for (int k = 0; k < g->n_sat; ++k) {
@@ -609,14 +655,6 @@ static vrs_status geom_build(vrs_ctx *ctx, const vrs_geom_desc *d, vrs_geom *g) 
     VRS_HIP(ctx, hipMemcpyAsync(g->d_orbit, orbit.data(), sizeof(double) * orbit.size(),
                                 hipMemcpyHostToDevice, st));
     VRS_HIP(ctx, hipMemcpyAsync(g->d_pairs, g->pairs.data(), sizeof(int32_t) * g->pairs.size(),
-                                hipMemcpyHostToDevice, st));
-    VRS_HIP(ctx, hipMemcpyAsync(g->d_pix32, pix32.data(), sizeof(float4) * g->n_pad,
-                                hipMemcpyHostToDevice, st));
-    VRS_HIP(ctx, hipMemcpyAsync(g->d_pix64, pix64.data(), sizeof(double4) * g->n_pad,
-                                hipMemcpyHostToDevice, st));
-    VRS_HIP(ctx, hipMemcpyAsync(g->d_ring, ring.data(), sizeof(int32_t) * g->n_pad,
-                                hipMemcpyHostToDevice, st));
-    VRS_HIP(ctx, hipMemcpyAsync(g->d_chunk, chunks.data(), sizeof(ChunkRec) * g->n_chunk,
                                 hipMemcpyHostToDevice, st));
 
     OrbitArgs &a = g->orb;
@@ -631,6 +669,8 @@ static vrs_status geom_build(vrs_ctx *ctx, const vrs_geom_desc *d, vrs_geom *g) 
     a.inv_lambda_km = 1e3 / g->lambda_m;
     a.raan = g->d_orbit;
     a.m0 = g->d_orbit + g->n_sat;
+    a.t_idx = g->d_tidx;
+    a.ceil_uvw = (d->uvw_flags & VRS_UVW_CEIL) ? 1 : 0;
     // the host-side copies above are pageable; finish them before the vectors go away
     VRS_HIP(ctx, hipStreamSynchronize(st));
     return VRS_OK;
@@ -671,6 +711,15 @@ vrs_status vrs_geom_create(vrs_ctx *ctx, const vrs_geom_desc *d, vrs_geom **out)
         return fail(ctx, VRS_E_ARG, "n_base * n_t = %lld exceeds 2^31", (long long)(nb * o.n_t));
     if ((int64_t)o.n_sat * o.n_t > (int64_t)1 << 31)
         return fail(ctx, VRS_E_ARG, "n_sat * n_t exceeds 2^31");
+    if (d->uvw_flags & ~(VRS_UVW_CEIL | VRS_UVW_LAMBDA_INT))
+        return fail(ctx, VRS_E_ARG, "unknown uvw_flags 0x%x", d->uvw_flags);
+    if ((d->uvw_flags & VRS_UVW_LAMBDA_INT) && !(3e8 / d->freq_hz >= 1.0))
+        return fail(ctx, VRS_E_ARG, "(int)(3e8 / freq_hz) is zero at %g Hz", d->freq_hz);
+    if (o.sample_index)
+        for (int j = 0; j < o.n_t; ++j)
+            if (o.sample_index[j] > ((int64_t)1 << 52))
+                return fail(ctx, VRS_E_ARG, "sample_index[%d] = %lld is too large", j,
+                            (long long)o.sample_index[j]);
     DeviceGuard dg(ctx->device);
     vrs_geom *g = new vrs_geom;
     g->ctx = ctx;
@@ -684,6 +733,70 @@ vrs_status vrs_geom_create(vrs_ctx *ctx, const vrs_geom_desc *d, vrs_geom **out)
         vrs_geom_destroy(g);
     }
     return s;
+}
+
+vrs_status vrs_geom_create_uvw(vrs_ctx *ctx, const vrs_uvw_geom_desc *d, vrs_geom **out) {
+    if (vrs_status s = check_ctx(ctx)) return s;
+    if (!d || !out) return fail(ctx, VRS_E_ARG, "desc and out must be non-NULL");
+    *out = nullptr;
+    if (d->n_rows < 1 || d->n_rows > ((int64_t)1 << 31))
+        return fail(ctx, VRS_E_ARG, "n_rows must be in [1, 2^31]");
+    if (!d->uvw) return fail(ctx, VRS_E_ARG, "uvw is required");
+    for (int64_t i = 0; i < 3 * d->n_rows; ++i)
+        if (!std::isfinite(d->uvw[i])) return fail(ctx, VRS_E_ARG, "uvw row %lld is not finite", (long long)(i / 3));
+    if (d->lmn) {
+        if (d->n_pix < 1 || d->n_pix > ((int64_t)1 << 30))
+            return fail(ctx, VRS_E_ARG, "n_pix must be in [1, 2^30]");
+        for (int64_t p = 0; p < d->n_pix; ++p) {
+            const double *v = d->lmn + 3 * p;
+            const double r2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+            if (!(std::fabs(r2 - 1.0) <= 1e-6))
+                return fail(ctx, VRS_E_ARG, "lmn row %lld is not a unit vector", (long long)p);
+        }
+    } else if (d->nside < 1 || d->nside > 4096 || (d->nside & (d->nside - 1))) {
+        return fail(ctx, VRS_E_ARG, "nside must be a power of two in [1, 4096] (or give lmn)");
+    }
+    DeviceGuard dg(ctx->device);
+    vrs_geom *g = new vrs_geom;
+    g->ctx = ctx;
+    g->device = ctx->device;
+    g->from_uvw = true;
+    // one stand-in satellite step whose cap shades nothing; every row refers to it
+    g->n_sat = 1;
+    g->n_t = (int32_t)d->n_rows;
+    g->n_base = 1;
+    g->pairs = {0, 0};
+    g->n_vis = d->n_rows;
+    g->r_moon = 0.0;
+    auto bail = [&](vrs_status s) { vrs_geom_destroy(g); return s; };
+    if (vrs_status s = sky_build(ctx, g, d->nside, d->n_pix, d->lmn)) return bail(s);
+    std::vector<double4> uvw(g->n_vis);
+    for (int64_t v = 0; v < g->n_vis; ++v)
+        uvw[v] = make_double4(d->uvw[3 * v], d->uvw[3 * v + 1], d->uvw[3 * v + 2], 0.0);
+    const CapRec none = {0.f, 0.f, 0.f, 2.0f, 0.f, kGuardMin, 2.0f - kGuardMin, 2.0f + kGuardMin};
+    const double4 origin = make_double4(0.0, 0.0, 0.0, 0.0);
+    hipStream_t st = ctx->stream;
+    auto hip = [&](hipError_t e, const char *what) -> vrs_status {
+        if (e == hipSuccess) return VRS_OK;
+        return fail(ctx, e == hipErrorOutOfMemory ? VRS_E_OOM : VRS_E_HIP, "%s: %s", what,
+                    hipGetErrorString(e));
+    };
+    vrs_status s = VRS_OK;
+    if (!s) s = hip(hipMalloc(&g->d_pos, sizeof(double4)), "hipMalloc");
+    if (!s) s = hip(hipMalloc(&g->d_cap, sizeof(CapRec)), "hipMalloc");
+    if (!s) s = hip(hipMalloc(&g->d_uvw64, sizeof(double4) * g->n_vis), "hipMalloc");
+    if (!s) s = hip(hipMalloc(&g->d_viscap, sizeof(int2) * g->n_vis), "hipMalloc");
+    if (!s) s = hip(hipMalloc(&g->d_pairs, sizeof(int2)), "hipMalloc");
+    if (!s) s = hip(hipMemcpyAsync(g->d_pos, &origin, sizeof origin, hipMemcpyHostToDevice, st), "copy");
+    if (!s) s = hip(hipMemcpyAsync(g->d_cap, &none, sizeof none, hipMemcpyHostToDevice, st), "copy");
+    if (!s) s = hip(hipMemcpyAsync(g->d_uvw64, uvw.data(), sizeof(double4) * g->n_vis,
+                                   hipMemcpyHostToDevice, st), "copy");
+    if (!s) s = hip(hipMemsetAsync(g->d_viscap, 0, sizeof(int2) * g->n_vis, st), "memset");
+    if (!s) s = hip(hipMemsetAsync(g->d_pairs, 0, sizeof(int2), st), "memset");
+    if (!s) s = hip(hipStreamSynchronize(st), "sync");
+    if (s) return bail(s);
+    *out = g;
+    return VRS_OK;
 }
 
 vrs_status vrs_geom_info_get(const vrs_geom *g, vrs_geom_info *out) {
@@ -703,8 +816,10 @@ vrs_status vrs_geom_info_get(const vrs_geom *g, vrs_geom_info *out) {
 vrs_status vrs_orbit_uvw(vrs_ctx *ctx, vrs_geom *g, double *sat_pos, double *uvw) {
     if (vrs_status s = check_ctx(ctx)) return s;
     if (!g || g->ctx != ctx) return fail(ctx, VRS_E_ARG, "geometry does not belong to this context");
+    if (g->from_uvw && sat_pos)
+        return fail(ctx, VRS_E_STATE, "a geometry built from uvw rows has no satellite positions");
     DeviceGuard dg(ctx->device);
-    {
+    if (!g->from_uvw) {
         Timer tm(ctx, "orbit");
         VRS_HIP(ctx, launch_orbit(g->orb, g->dev(), g->d_pos, g->d_cap, g->d_uvw64, g->d_viscap,
                                   ctx->stream));
@@ -1180,6 +1295,8 @@ vrs_status vrs_batch_create(vrs_ctx *ctx, const vrs_geom *g, const vrs_batch_des
     if (vrs_status s = check_geom(ctx, g)) return s;
     if (!d || !out) return fail(ctx, VRS_E_ARG, "desc and out must be non-NULL");
     *out = nullptr;
+    if (g->from_uvw)
+        return fail(ctx, VRS_E_STATE, "a batch needs satellite positions (uvw-row geometry)");
     if (d->n_ch < 1 || d->n_ch > 4096) return fail(ctx, VRS_E_ARG, "n_ch must be in [1, 4096]");
     if (!d->freq_hz || !d->n_t) return fail(ctx, VRS_E_ARG, "freq_hz and n_t are required");
     if (d->precision != VRS_PLAIN_F32 && d->precision != VRS_MIXED_F64PHASE)
@@ -1490,6 +1607,8 @@ extern "C" vrs_status vrs_grid_uvw(vrs_ctx *ctx, const vrs_geom *g, int32_t flag
     if (vrs_status s = check_ctx(ctx)) return s;
     if (vrs_status s = check_geom(ctx, g)) return s;
     if (!n_unique) return fail(ctx, VRS_E_ARG, "n_unique must be non-NULL");
+    if (g->from_uvw)
+        return fail(ctx, VRS_E_STATE, "gridding needs satellite positions (uvw-row geometry)");
     if ((flags & ~(VRS_GRID_ORDERED | VRS_GRID_LAMBDA_INT)) != 0)
         return fail(ctx, VRS_E_ARG, "unknown grid flags 0x%x", flags);
     if ((uvw_unique || counts) && cap < 1) return fail(ctx, VRS_E_ARG, "cap must be >= 1");

@@ -160,3 +160,28 @@ int64_t healpix_nest2ring(int64_t nside, int64_t p) {
 }
 
 }  // namespace vrs
+
+namespace vrs {
+
+// The reference's one-third-orbit windows (uvwPosition,
+// MIncline_generate_uvw_float.cu:113-151).  Output block k (0-based orbit) holds
+// span = orbit_res / 3 steps starting at k * (orbit_res + interval) in the
+// propagated series of orbit_counts orbits; once such a start lies past the
+// series, whole orbits are taken off again (the first "reset" orbits), keeping
+// the k * interval offset.  A step past the end of the series is zero-filled (-1).
+void uvw_position_index(int32_t orbit_res, int32_t orbit_counts, int32_t interval, int32_t span,
+                        int64_t *out) {
+    const int64_t total = (int64_t)orbit_res * orbit_counts;
+    const int64_t reset = total / ((int64_t)orbit_res + interval) + 1;
+    for (int64_t k = 0; k < orbit_counts; ++k) {
+        int64_t first = k * orbit_res + k * interval;
+        if (k * ((int64_t)orbit_res + interval) > total) first -= reset * orbit_res;
+        const bool runs_off = first + span > total;
+        for (int64_t j = 0; j < span; ++j) {
+            const int64_t src = first + j;
+            out[k * span + j] = (runs_off && src >= total) ? -1 : src;
+        }
+    }
+}
+
+}  // namespace vrs

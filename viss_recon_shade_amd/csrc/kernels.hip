@@ -147,8 +147,16 @@ __global__ void __launch_bounds__(256) orbit_kernel(OrbitArgs a, double4 *__rest
     if (idx >= (int64_t)a.n_sat * a.n_t) return;
     const int k = (int)(idx / a.n_t);
     const int j = (int)(idx % a.n_t);
+    const int64_t ts = a.t_idx ? a.t_idx[j] : (int64_t)j;
+    if (ts < 0) {
+        // a sample past the end of the propagated span: the reference zero-fills
+        // it (uvwPosition, MIncline_generate_uvw_float.cu:134-139); it shades nothing
+        pos[idx] = make_double4(0.0, 0.0, 0.0, 0.0);
+        cap[idx] = {0.f, 0.f, 0.f, 2.0f, 0.f, kGuardMin, 2.0f - kGuardMin, 2.0f + kGuardMin};
+        return;
+    }
     // circular orbit: E = M = nu (calculateOrbit, MIncline_generate_uvw_float.cu:93-104)
-    const double w = a.m0[k] + a.n_mot * ((double)j * a.dt);
+    const double w = a.m0[k] + a.n_mot * ((double)ts * a.dt);
     const double cw = cos(w), sw = sin(w), cO = cos(a.raan[k]), sO = sin(a.raan[k]);
     const double x = a.a * (cw * cO - sw * a.cos_i * sO);
     const double y = a.a * (cw * sO + sw * a.cos_i * cO);
@@ -172,7 +180,7 @@ __global__ void __launch_bounds__(256) orbit_kernel(OrbitArgs a, double4 *__rest
     cap[idx] = cr;
 }
 
-__global__ void __launch_bounds__(256) uvw_kernel(GeomDev g, double inv_lambda_km,
+__global__ void __launch_bounds__(256) uvw_kernel(GeomDev g, double inv_lambda_km, int ceil_uvw,
                                                   double4 *__restrict__ uvw64,
                                                   int2 *__restrict__ vis_cap) {
     const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -180,9 +188,14 @@ __global__ void __launch_bounds__(256) uvw_kernel(GeomDev g, double inv_lambda_k
     const int b = (int)(v / g.n_t), t = (int)(v % g.n_t);
     const int2 pr = g.pairs[b];
     const double4 ra = g.pos[(int64_t)pr.x * g.n_t + t], rb = g.pos[(int64_t)pr.y * g.n_t + t];
-    // computeUVW without ceil (MIncline_generate_uvw_float.cu:156-158)
-    const double u = (rb.x - ra.x) * inv_lambda_km, vv = (rb.y - ra.y) * inv_lambda_km,
-                 w = (rb.z - ra.z) * inv_lambda_km;
+    // computeUVW (MIncline_generate_uvw_float.cu:156-158); its ceil only with ceil_uvw
+    double u = (rb.x - ra.x) * inv_lambda_km, vv = (rb.y - ra.y) * inv_lambda_km,
+           w = (rb.z - ra.z) * inv_lambda_km;
+    if (ceil_uvw) {
+        u = ceil(u);
+        vv = ceil(vv);
+        w = ceil(w);
+    }
     uvw64[v] = make_double4(u, vv, w, 0.0);
     vis_cap[v] = make_int2(pr.x * g.n_t + t, pr.y * g.n_t + t);
 }
@@ -1026,7 +1039,7 @@ hipError_t launch_orbit(const OrbitArgs &a, const GeomDev &g, double4 *pos, CapR
     if (e != hipSuccess) return e;
     if (g.n_vis > 0)
         hipLaunchKernelGGL(uvw_kernel, dim3(blocks_for(g.n_vis, 256)), dim3(256), 0, st, g,
-                           a.inv_lambda_km, uvw64, vis_cap);
+                           a.inv_lambda_km, (int)a.ceil_uvw, uvw64, vis_cap);
     return hipGetLastError();
 }
 

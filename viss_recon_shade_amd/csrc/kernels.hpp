@@ -49,6 +49,10 @@ struct OrbitArgs {
     int32_t n_sat, n_t;
     double dt, a, cos_i, sin_i, n_mot, r_moon, inv_lambda_km;
     const double *raan, *m0;  // device [n_sat]
+    // device [n_t] or null: step j is taken at time t_idx[j] * dt; a negative
+    // entry is a zero-filled sample (position 0, a cap that shades nothing)
+    const int64_t *t_idx;
+    int32_t ceil_uvw;         // 1: uvw = ceil(dx * 1e3 / lambda), the reference's gridding
 };
 
 struct PhaseKnobs {

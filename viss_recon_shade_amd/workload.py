@@ -153,3 +153,17 @@ def c4_skies(freqs, alpha, nside=C4_NSIDE, seed=1):
     I0 = make_sky(Workload("c4", C4_N_SAT, 1, 1e6, nside, 1, [(seed, 1.0, 0, 1.0, 1e3)]))[0]
     return np.stack([I0.astype(np.float64) * (f / 1e6) ** a
                      for f, a in zip(freqs, alpha)]).astype(np.float32).reshape(len(freqs), n_pix)
+
+
+def uvw_position_index(orbit_res, orbit_counts, interval=None, span=None):
+    """The reference's orbit windows (uvwPosition, MIncline_generate_uvw_float.cu:
+    113-151, constants :185-190): source sample index of every kept step, -1 for
+    zero-filled ones.  interval defaults to (orbit_res / 360) * 10, span to
+    orbit_res / 3, both in the reference's integer arithmetic."""
+    lib = _lib.load()
+    interval = (orbit_res // 360) * 10 if interval is None else interval
+    span = orbit_res // 3 if span is None else span
+    out = np.empty(orbit_counts * span, np.int64)
+    _lib.check(lib.vrs_uvw_position_index(orbit_res, orbit_counts, interval, span,
+                                          out.ctypes.data))
+    return out
