@@ -60,6 +60,7 @@ def parse():
                          "with the oracle (1: all of them)")
     ap.add_argument("--no-mfma", action="store_true", help="skip the multi-map matrix-core legs")
     ap.add_argument("--no-batch", action="store_true", help="skip the multi-channel batch leg")
+    ap.add_argument("--no-c5", action="store_true", help="skip the least-squares recovery leg")
     return ap.parse_args()
 
 
@@ -219,6 +220,41 @@ def batch_leg(torch, ctx, vrs, W, args):
         V, F = b.run(I, k)
         out["accuracy"] = batch_accuracy(g, b, freqs, I, V, F)
     b.close()
+    g.close()
+    return out
+
+
+def c5_leg(torch, ctx, vrs, W):
+    """Differentiable form: 16 satellites, 120 x 4320 visibilities at 10 MHz,
+    Nside 64 ground truth, V_obs = V(I_true) + complex Gaussian noise at
+    per-visibility SNR 20 (seed 3); 100 projected-CG iterations of
+    min sum |V(I) - V_obs|^2 over I >= 0, each one forward vis-gen, one shaded
+    back-projection (the hand-written adjoint) and one forward of the search
+    direction (FP64 phase).  Reported with T_occ = 240 and with T_occ = 0."""
+    from viss_recon_shade_amd import lsq
+    wl = W.CONFIGS["c5"]
+    g = ctx.workload_geometry(wl)
+    I = torch.from_numpy(W.make_sky(wl)[0]).cuda()
+    out = {"workload": wl.note, "n_vis": wl.n_vis, "n_pix": wl.n_pix, "method": "projected CG"}
+    for t_occ in (240.0, 0.0):
+        k = vrs.knobs(precision="mixed", t_occ=t_occ)
+        V_true = g.visscal(I[None], k)[0]
+        V_obs = lsq.add_noise(V_true, 20.0, 3)
+        floor = float(((V_true - V_obs).double() ** 2).sum())
+        lsq.recover(g, V_obs, n_iter=2, method="cg", k=k)           # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        Irec, hist = lsq.recover(g, V_obs, n_iter=100, method="cg", k=k)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        n_it = len(hist) - 1
+        passes = 3 * n_it + 1          # forward, adjoint, forward of the direction; final forward
+        terms = passes * wl.n_vis * wl.n_pix
+        out[f"t_occ_{int(t_occ)}"] = {
+            "iterations": n_it, "seconds": round(dt, 3), "ms_per_iteration": round(dt / n_it * 1e3, 3),
+            "value": round(terms / dt / 1e9, 1), "unit": "Gterms/s (vis x pix, all passes)",
+            "loss_initial_over_floor": hist[0] / floor, "loss_final_over_floor": hist[-1] / floor,
+            "sky_rel_err": float(torch.linalg.norm(Irec - I) / torch.linalg.norm(I))}
     g.close()
     return out
 
@@ -439,6 +475,8 @@ def main():
         out["mfma_split16"] = mfma_legs(torch, ctx, vrs, W, wl, g, pos, uvw, lmn, args)
     if not args.no_batch:
         out["batch_c4"] = batch_leg(torch, ctx, vrs, W, args)
+    if not args.no_c5:
+        out["c5_lsq"] = c5_leg(torch, ctx, vrs, W)
     refs = {}
     if not args.no_accuracy:
         out["accuracy"] = {"plain": accuracy(g, wl, pos, uvw, lmn, I, Vp, Fp, refs=refs),

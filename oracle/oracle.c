@@ -518,36 +518,66 @@ void vo_recon(int64_t n_t, const double *pos, const int *pairs, int n_base, cons
 
 /* ------------------------------------------------ reference box predicate */
 /* ShadeMatrix (shade10M.cu:73-105) with Cart2SphKernel (shade10M.cu:62-70)
- * and the r0/dtheta/dphi set-up (shade10M.cu:218-224), evaluated in FP32 as
- * the reference does.  Output per uvw: thetaIndS, thetaIndE, phiIndS, phiIndE. */
+ * and the r0/dtheta/dphi set-up (shade10M.cu:218-224).  The reference works in
+ * FP32 with CUDA's atan2f / asinf; this restatement takes the angles in FP64
+ * and rounds each grid index once (vo_round_shared), the evaluation the GPU
+ * box kernel shares.  Output per uvw: thetaIndS, thetaIndE, phiIndS, phiIndE. */
+/* |uvw| in FP64 (products of FP32 inputs are exact in FP64). */
+static double vo_row_norm(float u, float v, float w)
+{
+    double uu = (double)u * u, vv = (double)v * v, ww = (double)w * w;
+    return sqrt((uu + vv) + ww);
+}
+
+/* Nearest integer of x >= 0; within 1e-9 of a half-integer it goes up (roundf's
+ * direction at an exact tie), so that two FP64 libms that differ in the last
+ * ulp of atan2 / asin / atan still pick the same index. */
+static int vo_round_shared(double x)
+{
+    double f = floor(x), d = x - f;
+    if (fabs(d - 0.5) <= 1e-9) return (int)f + 1;
+    return (int)f + (d > 0.5 ? 1 : 0);
+}
+
 void vo_shade_box(int64_t n, const float *u, const float *v, const float *w, int res,
                   float r_moon_ref, int32_t *out4)
 {
-    float sum_r = 0.0f;
-    for (int64_t i = 0; i < n; ++i)
-        sum_r += sqrtf(u[i] * u[i] + v[i] * v[i] + w[i] * w[i]);
-    float mean_r = sum_r / (float)n;
-    float r0 = atanf(r_moon_ref / (mean_r + r_moon_ref));
-    float dtheta = (float)(2.0 * VO_PI / (double)res); /* float dtheta = 2 * M_PI / RES */
-    float dphi = (float)(VO_PI / (double)res);
+    /* mean |uvw|: FP64 partial sums in a fixed order -- 256 x 256 strided
+     * accumulators, a halving tree over each 256, then the 256 partials in turn */
+    double sum = 0.0;
+    for (int b = 0; b < 256; ++b) {
+        double s[256];
+        for (int t = 0; t < 256; ++t) {
+            double acc = 0.0;
+            for (int64_t i = (int64_t)b * 256 + t; i < n; i += 65536)
+                acc += vo_row_norm(u[i], v[i], w[i]);
+            s[t] = acc;
+        }
+        for (int o = 128; o > 0; o >>= 1)
+            for (int t = 0; t < o; ++t) s[t] += s[t + o];
+        sum += s[0];
+    }
+    double mean_r = sum / (double)n;
+    double R = (double)r_moon_ref;
+    double dtheta = 2.0 * VO_PI / res, dphi = VO_PI / res;
+    int tR = vo_round_shared(atan(R / (mean_r + R)) / dtheta);
+    int pR = tR / 2; /* phiR = roundf(thetaR / 2) with integer division */
     for (int64_t i = 0; i < n; ++i) {
-        float r = sqrtf(u[i] * u[i] + v[i] * v[i] + w[i] * w[i]);
-        float th = atan2f(v[i], u[i]);
-        float ph = asinf(w[i] / r);
-        th += (float)VO_PI;
-        ph += (float)(VO_PI / 2);
-        th = 2.0f * (float)VO_PI - th;
-        ph = (float)VO_PI - ph;
-        int ti = (int)roundf(th / dtheta);
-        int pi_ = (int)roundf(ph / dphi);
-        int tR = (int)roundf(r0 / dtheta);
-        int pR = (int)roundf((float)(tR / 2));
+        double r = vo_row_norm(u[i], v[i], w[i]);
+        double th = atan2((double)v[i], (double)u[i]);
+        double ph = r > 0.0 ? asin((double)w[i] / r) : 0.0;
+        /* antipode of the baseline direction (shade10M.cu:78-83) */
+        int ti = vo_round_shared((2.0 * VO_PI - (th + VO_PI)) / dtheta);
+        int pi_ = vo_round_shared((VO_PI - (ph + VO_PI / 2)) / dphi);
         int tS = ti - tR, tE = ti + tR, pS = pi_ - pR, pE = pi_ + pR;
         if (tS <= 0) tS += res;
         if (tE > res) tE -= res;
         if (pS <= 0) pS += res;
         if (pE > res) pE -= res;
-        out4[4 * i] = tS; out4[4 * i + 1] = tE; out4[4 * i + 2] = pS; out4[4 * i + 3] = pE;
+        out4[4 * i] = tS;
+        out4[4 * i + 1] = tE;
+        out4[4 * i + 2] = pS;
+        out4[4 * i + 3] = pE;
     }
 }
 

@@ -5,16 +5,14 @@ Checks:
   * FP64 reference gradient (oracle, 2 A^H r on sampled pixels): rel L2 <= 1e-5
   * central finite differences, h = 1e-3, of the FP64 oracle loss at a small
     size (exact for this quadratic loss up to FP64 rounding): rel L2 <= 1e-5
-  * central finite differences, h = 1e-3, on 64 random pixels at the c5 size
-    with the loss evaluated in FP64 from GPU visibilities.  V(I +- h e_p)
-    carries FP32 sums of ~5e5 terms, whose rounding differs between the two
-    evaluations by ~1e-4 |V| per visibility while the signal is 2h = 2e-3, so
-    this check runs with t_occ = 0 (|V| of the sky alone) and is bounded at 0.25;
-    the precise finite-difference check is the FP64 one above.
+  * at the c5 size, the gradient of the loss restricted to 512 visibilities
+    (vrs_recon, kappa = 2, 0/1 weights) against FP64 central differences,
+    h = 1e-3, of the oracle's vis-gen on the same subset, 64 pixels: <= 1e-5
   * 100 projected-gradient / projected-CG iterations reach the noise floor:
     the final loss is within 0.1 % of L(I_true) = sum |noise|^2.  (At SNR 20
     relative to |V|, which the 240 K shaded term dominates, the floor is most
-    of the initial loss; the sky-only variant t_occ = 0 is reported.)
+    of the initial loss and the sky is weakly constrained; with t_occ = 0 the
+    recovered sky is within 2 % relative L2 of the truth.)
 """
 import numpy as np
 import pytest
@@ -90,27 +88,43 @@ def test_central_fd_fp64_small(ctx, oracle):
     assert e <= 1e-5
 
 
-def test_central_fd_gpu_c5(ctx):
-    wl, g, I, k, V_obs = setup(ctx, "c5", t_occ=0.0)
-    Ig = I.clone().requires_grad_(True)
-    AG.loss_sq(Ig, g, V_obs, k).backward()
-    grad = Ig.grad.cpu().numpy().astype(np.float64)
+def test_central_fd_c5_visibility_subset(ctx, oracle, report):
+    """c5 size: the gradient of the loss restricted to 512 visibilities S,
+    L_S(I) = sum_{v in S} |V(I)_v - V_obs_v|^2, from the GPU's shaded
+    back-projection (vrs_recon with kappa = 2 and 0/1 weights selecting S)
+    against FP64 central differences (h = 1e-3) of the oracle's vis-gen
+    (vissGen10M_shade_optim1.cu:130-206 restated) on the same subset, on 64
+    random pixels.  L_S is quadratic in I, so the central difference is exact up
+    to FP64 rounding."""
+    wl, g, I, k, V_obs = setup(ctx, "c5")
+    pos, uvw = g.MIncline()
+    lmn = g.lmn()
+    S = np.linspace(0, wl.n_vis - 1, 512).astype(np.int64)
+    I0 = (0.5 * I).cpu().numpy().astype(np.float32)
+    Vobs_S = cplx(V_obs)[S]
+    r_S = oracle.visgen(pos, g.pairs, uvw, lmn, I0, g.r_moon_km, vis_sel=S)[0] - Vobs_S
+    resid = np.zeros((wl.n_vis, 2), np.float32)
+    resid[S, 0], resid[S, 1] = r_S.real, r_S.imag
+    wsel = np.zeros(wl.n_vis, np.float32)
+    wsel[S] = 1.0
+    grad = g.imagerecon(resid, wsel, vrs.knobs(precision="mixed", kappa=2.0))[0].astype(np.float64)
     h = 1e-3
-    psel = np.random.default_rng(2).choice(wl.n_pix, 64, replace=False)
-    Vo = cplx(V_obs)
-    fd = []
-    with torch.no_grad():
-        for p in psel:
-            Ip = I.clone()
-            Im = I.clone()
-            Ip[p] += h
-            Im[p] -= h
-            hh = float(Ip[p] - Im[p]) / 2
-            Vp, Vm = cplx(AG.visgen(Ip, g, k)), cplx(AG.visgen(Im, g, k))
-            fd.append(np.real(np.vdot(Vp - Vm, Vp + Vm - 2 * Vo)) / (2 * hh))
-    e = rel(grad[psel], np.array(fd))
-    print(f"[c5] autograd vs GPU central FD (h=1e-3, 64 px): rel {e:.2e}", flush=True)
-    assert e <= 0.25
+    psel = np.sort(np.random.default_rng(2).choice(wl.n_pix, 64, replace=False))
+    stack = np.repeat(I0[None, :].astype(np.float64), 2 * len(psel), 0)
+    for j, p in enumerate(psel):
+        stack[2 * j, p] += h
+        stack[2 * j + 1, p] -= h
+    stack = stack.astype(np.float32)
+    hh = (stack[0::2, psel].diagonal().astype(np.float64)
+          - stack[1::2, psel].diagonal().astype(np.float64)) / 2   # the perturbation applied
+    V2 = oracle.visgen(pos, g.pairs, uvw, lmn, stack, g.r_moon_km, vis_sel=S)
+    L = (np.abs(V2 - Vobs_S[None, :]) ** 2).sum(1)
+    fd = (L[0::2] - L[1::2]) / (2 * hh)
+    e = rel(grad[psel], fd)
+    print(f"[c5] shaded back-projection vs FP64 central FD (512 vis, 64 px, h=1e-3): rel {e:.2e}",
+          flush=True)
+    report("c5 gradient vs FP64 central FD (512-vis subset, 64 pixels)", rel_l2=e)
+    assert e <= 1e-5
 
 
 @pytest.mark.parametrize("method,t_occ", [("pg", 240.0), ("cg", 240.0), ("cg", 0.0)])
@@ -125,8 +139,11 @@ def test_lsq_recovery_c5(ctx, method, t_occ):
           f"in {len(h) - 1} iterations; sky rel err {err:.3f}", flush=True)
     assert h[-1] <= h[0]
     assert (Irec >= 0).all()
-    if t_occ > 0:
-        assert h[-1] <= floor * 1.001
+    assert h[-1] <= floor * 1.001
+    if t_occ == 0:
+        # the sky alone sets |V|, so SNR 20 per visibility constrains it well:
+        # 100 CG iterations recover it to ~1.2 % (measured on an MI355X)
+        assert err <= 0.02, err
     # the line search minimises exactly along the free-set direction; the
     # projection onto I >= 0 afterwards may cost a little, rarely
     assert np.count_nonzero(np.diff(h) > 1e-6 * h[:-1]) <= 0.1 * (len(h) - 1)

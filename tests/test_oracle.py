@@ -307,6 +307,30 @@ def test_shade_box_reference_formula(oracle):
     assert list(out) == [ti - tR, ti + tR, pi_ - pR, pi_ + pR]
 
 
+def test_shade_box_matches_numpy_formula(oracle):
+    """The box of many random integer rows against the same geometry written
+    independently in numpy: antipode (2 pi - (theta + pi), pi - (phi + pi/2)),
+    nearest grid index, half-widths round(r0 / dtheta) and its integer half,
+    wrap onto (0, res].  Ties within 1e-9 of a half do not occur in this sample."""
+    rng = np.random.default_rng(17)
+    uvw = rng.integers(-40000, 40001, (5000, 3)).astype(np.float32)
+    for res in (2094, 20940):
+        out = oracle.shade_box(uvw[:, 0], uvw[:, 1], uvw[:, 2], res)
+        d = uvw.astype(np.float64)
+        r = np.sqrt((d ** 2).sum(1))
+        dth, dph = 2 * math.pi / res, math.pi / res
+        tR = int(np.floor(math.atan(1737.74 / (r.mean() + 1737.74)) / dth + 0.5))
+        pR = tR // 2
+        ti = np.floor((math.pi - np.arctan2(d[:, 1], d[:, 0])) / dth + 0.5).astype(np.int64)
+        pi_ = np.floor((math.pi / 2 - np.arcsin(d[:, 2] / r)) / dph + 0.5).astype(np.int64)
+        want = np.stack([ti - tR, ti + tR, pi_ - pR, pi_ + pR], 1)
+        want[:, 0] += np.where(want[:, 0] <= 0, res, 0)
+        want[:, 1] -= np.where(want[:, 1] > res, res, 0)
+        want[:, 2] += np.where(want[:, 2] <= 0, res, 0)
+        want[:, 3] -= np.where(want[:, 3] > res, res, 0)
+        assert np.array_equal(out, want), res
+
+
 def test_grid_uvw_matches_numpy_unique(oracle):
     # the oracle's sort/unique/count against numpy's lexsort-based unique
     raan = np.array([0.0, 2.1, 4.2, 1.0])

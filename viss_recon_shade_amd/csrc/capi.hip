@@ -1526,10 +1526,10 @@ vrs_status vrs_shade_box(vrs_ctx *ctx, const float *uvw, int64_t n, int32_t res,
     if (vrs_status s = su.prepare()) return s;
     if (vrs_status s = sb.prepare()) return s;
     void *partial;
-    if (vrs_status s = ws_get(ctx, 3, sizeof(float) * shade_box_partials(), &partial)) return s;
+    if (vrs_status s = ws_get(ctx, 3, sizeof(double) * shade_box_partials(), &partial)) return s;
     {
         Timer tm(ctx, "shade_box");
-        VRS_HIP(ctx, launch_shade_box((const float *)su.dev, n, res, r_moon_ref_km, (float *)partial,
+        VRS_HIP(ctx, launch_shade_box((const float *)su.dev, n, res, r_moon_ref_km, (double *)partial,
                                       (int32_t *)sb.dev, ctx->stream));
     }
     return sb.finish();
