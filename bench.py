@@ -61,6 +61,10 @@ def parse():
     ap.add_argument("--no-mfma", action="store_true", help="skip the multi-map matrix-core legs")
     ap.add_argument("--no-batch", action="store_true", help="skip the multi-channel batch leg")
     ap.add_argument("--no-c5", action="store_true", help="skip the least-squares recovery leg")
+    ap.add_argument("--no-grid", action="store_true", help="skip the uvw gridding leg")
+    ap.add_argument("--grid-ref", action="store_true",
+                    help="also grid at the reference's own stage-1 scale (8 satellites x 40.32 M "
+                         "windowed samples, 56 ordered pairs: 2.26e9 keys, ~110 GB of HBM)")
     return ap.parse_args()
 
 
@@ -256,6 +260,56 @@ def c5_leg(torch, ctx, vrs, W):
             "loss_initial_over_floor": hist[0] / floor, "loss_final_over_floor": hist[-1] / floor,
             "sky_rel_err": float(torch.linalg.norm(Irec - I) / torch.linalg.norm(I))}
     g.close()
+    return out
+
+
+def _radix_passes(uvw):
+    """Digit passes the radix sort runs: 8-bit digits of the 63-bit cell keys that
+    are not the same for every cell."""
+    k = ((((uvw[:, 0].astype(np.int64) + (1 << 20)) << 42) | ((uvw[:, 1].astype(np.int64) + (1 << 20)) << 21)
+          | (uvw[:, 2].astype(np.int64) + (1 << 20)))).astype(np.uint64)
+    return int(sum(len(np.unique((k >> np.uint64(8 * d)) & np.uint64(255))) > 1 for d in range(8)))
+
+
+def grid_leg(ctx, vrs, W, g_c2, wl, ref_scale):
+    """uvw gridding (ceil, lambda = int(3e8/f)), sort, unique and counts of every
+    ordered satellite pair: c2's 16 satellites x 8640 steps, and optionally the
+    reference's own stage-1 scale (uvwPosition windows, OrbitRes 864000 x 140
+    orbits -> 40.32 M samples per satellite, 8 satellites, 56 ordered pairs).
+    Algorithmic bytes per key: 8 (key written) + 8 (AND/OR digit check) +
+    24 per radix pass (histogram read, scatter read + write) + 8 (run count)."""
+    import ctypes as C
+    import math
+    lib = ctx._lib
+    out = {}
+
+    def run(g, n_keys, label):
+        ctx.timing(True)
+        n_u = C.c_int64()
+        for _ in range(2):
+            ctx.check(lib.vrs_grid_uvw(ctx.handle, g.handle, 3, 0, None, None, C.byref(n_u), None, None))
+            ms = ctx.last_ms("grid")
+        ctx.timing(False)
+        uu = np.empty((n_u.value, 3), np.float32)
+        ctx.check(lib.vrs_grid_uvw(ctx.handle, g.handle, 3, n_u.value, uu.ctypes.data, None,
+                                   C.byref(n_u), None, None))
+        passes = _radix_passes(uu)
+        alg = n_keys * (8 + 8 + 24 * passes + 8)
+        out[label] = {"keys": n_keys, "unique_cells": int(n_u.value), "ms": round(ms, 3),
+                      "gkeys_per_s": round(n_keys / (ms * 1e-3) / 1e9, 2), "radix_passes": passes,
+                      "algorithmic_bytes": alg,
+                      "hbm_frac": round(alg / (ms * 1e-3) / 8e12, 3)}
+
+    run(g_c2, wl.n_sat * (wl.n_sat - 1) * wl.n_t, "c2_ordered_pairs")
+    if ref_scale:
+        OR, OC = 864000, 140
+        idx = W.uvw_position_index(OR, OC)
+        period = 2 * math.pi * math.sqrt(W.A_KM ** 3 / W.MU_MOON)
+        g = ctx.geometry(8, len(idx), 1e7, 1, dt_s=period / OR, sample_index=idx)
+        run(g, 8 * 7 * len(idx), "reference_stage1_scale")
+        g.close()
+    out["note"] = ("ms = keys + digit check + radix sort + unique count (HIP events, 'grid'); "
+                   "hbm_frac = algorithmic bytes / ms / 8 TB/s")
     return out
 
 
@@ -477,6 +531,8 @@ def main():
         out["batch_c4"] = batch_leg(torch, ctx, vrs, W, args)
     if not args.no_c5:
         out["c5_lsq"] = c5_leg(torch, ctx, vrs, W)
+    if not args.no_grid:
+        out["gridding"] = grid_leg(ctx, vrs, W, g, wl, args.grid_ref)
     refs = {}
     if not args.no_accuracy:
         out["accuracy"] = {"plain": accuracy(g, wl, pos, uvw, lmn, I, Vp, Fp, refs=refs),

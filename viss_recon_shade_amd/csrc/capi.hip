@@ -1623,7 +1623,7 @@ extern "C" vrs_status vrs_grid_uvw(vrs_ctx *ctx, const vrs_geom *g, int32_t flag
     }
     const int n_pairs = (int)(pr.size() / 2);
     const int64_t n = (int64_t)n_pairs * g->n_t;
-    if (n > ((int64_t)1 << 31) - 1) return fail(ctx, VRS_E_ARG, "too many gridded samples");
+    if (n > ((int64_t)1 << 36)) return fail(ctx, VRS_E_ARG, "too many gridded samples");
     const double lam = (flags & VRS_GRID_LAMBDA_INT) ? (double)(int)(3e8 / g->freq_hz) : g->lambda_m;
     if (!(lam > 0)) return fail(ctx, VRS_E_ARG, "wavelength rounds to zero");
     const double sc = 1e3 / lam;
@@ -1631,50 +1631,51 @@ extern "C" vrs_status vrs_grid_uvw(vrs_ctx *ctx, const vrs_geom *g, int32_t flag
     struct Buf {
         void *p = nullptr;
         ~Buf() { if (p) (void)hipFree(p); }
-    } b_pairs, b_keys, b_tmp, b_isc, b_small, b_ukeys, b_cnt, b_vkeys, b_bad;
-    const int64_t nb = (n + grid_tile() - 1) / grid_tile();
+    } b_pairs, b_keys, b_tmp, b_rad, b_uniq, b_ukeys, b_cnt, b_vkeys, b_bad;
     VRS_HIP(ctx, hipMalloc(&b_pairs.p, sizeof(int32_t) * pr.size()));
     VRS_HIP(ctx, hipMalloc(&b_keys.p, sizeof(uint64_t) * n));
     VRS_HIP(ctx, hipMalloc(&b_tmp.p, sizeof(uint64_t) * n));
-    VRS_HIP(ctx, hipMalloc(&b_isc.p, sizeof(int) * 2 * n));
-    VRS_HIP(ctx, hipMalloc(&b_small.p, sizeof(int) * (nb + 4)));
-    VRS_HIP(ctx, hipMalloc(&b_ukeys.p, sizeof(uint64_t) * n));
-    VRS_HIP(ctx, hipMalloc(&b_cnt.p, sizeof(int32_t) * n));
-    VRS_HIP(ctx, hipMalloc(&b_bad.p, sizeof(int) * 2));
+    VRS_HIP(ctx, hipMalloc(&b_rad.p, radix_scratch_bytes(n)));
+    VRS_HIP(ctx, hipMalloc(&b_uniq.p, unique_scratch_bytes(n)));
+    VRS_HIP(ctx, hipMalloc(&b_bad.p, sizeof(int)));
     VRS_HIP(ctx, hipMemcpyAsync(b_pairs.p, pr.data(), sizeof(int32_t) * pr.size(),
                                 hipMemcpyHostToDevice, ctx->stream));
-    VRS_HIP(ctx, hipMemsetAsync(b_bad.p, 0, sizeof(int) * 2, ctx->stream));
-    int *n_u_dev = (int *)b_bad.p + 1;
+    VRS_HIP(ctx, hipMemsetAsync(b_bad.p, 0, sizeof(int), ctx->stream));
+    int64_t nu = 0;
     {
         Timer tm(ctx, "grid");
         VRS_HIP(ctx, grid_build_keys(g->d_pos, g->n_t, (const int2 *)b_pairs.p, n_pairs, sc,
                                      (uint64_t *)b_keys.p, (int *)b_bad.p, ctx->stream));
-        VRS_HIP(ctx, grid_sort_unique((uint64_t *)b_keys.p, (uint64_t *)b_tmp.p, n, (int *)b_isc.p,
-                                      (int *)b_small.p, (uint64_t *)b_ukeys.p, (int32_t *)b_cnt.p,
-                                      n_u_dev, ctx->stream));
+        VRS_HIP(ctx, radix_sort((uint64_t *)b_keys.p, (uint64_t *)b_tmp.p, n, 0, 8, b_rad.p,
+                                ctx->stream, nullptr));
+        VRS_HIP(ctx, sorted_unique_count((const uint64_t *)b_keys.p, n, b_uniq.p, &nu, ctx->stream));
     }
-    int hb[2];
-    VRS_HIP(ctx, hipMemcpyAsync(hb, b_bad.p, sizeof hb, hipMemcpyDeviceToHost, ctx->stream));
+    int bad = 0;
+    VRS_HIP(ctx, hipMemcpyAsync(&bad, b_bad.p, sizeof bad, hipMemcpyDeviceToHost, ctx->stream));
     VRS_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    if (hb[0]) return fail(ctx, VRS_E_ARG, "gridded |uvw| exceeds 2^20 wavelengths");
-    const int nu = hb[1];
+    if (bad) return fail(ctx, VRS_E_ARG, "gridded |uvw| exceeds 2^20 wavelengths");
     *n_unique = nu;
-    VRS_HIP(ctx, grid_counts((const int *)b_isc.p + n, nu, n, (int32_t *)b_cnt.p, ctx->stream));
-    if (uvw_unique || counts) {
-        if (nu > cap) return fail(ctx, VRS_E_ARG, "%d unique cells exceed cap %lld", nu, (long long)cap);
-        if (uvw_unique) {
-            Staged su(ctx, uvw_unique, sizeof(float) * 3 * nu, true);
-            if (vrs_status s = su.prepare()) return s;
-            VRS_HIP(ctx, grid_decode((const uint64_t *)b_ukeys.p, nu, (float *)su.dev, ctx->stream));
-            if (vrs_status s = su.finish()) return s;
-        }
-        if (counts) {
-            Staged sc2(ctx, counts, sizeof(int32_t) * nu, true);
-            if (vrs_status s = sc2.prepare()) return s;
-            VRS_HIP(ctx, hipMemcpyAsync(sc2.dev, b_cnt.p, sizeof(int32_t) * nu,
-                                        hipMemcpyDeviceToDevice, ctx->stream));
-            if (vrs_status s = sc2.finish()) return s;
-        }
+    if (!(uvw_unique || counts || weight || vis_count)) return VRS_OK;
+    if ((uvw_unique || counts) && nu > cap)
+        return fail(ctx, VRS_E_ARG, "%lld unique cells exceed cap %lld", (long long)nu, (long long)cap);
+    VRS_HIP(ctx, hipMalloc(&b_ukeys.p, sizeof(uint64_t) * std::max<int64_t>(nu, 1)));
+    VRS_HIP(ctx, hipMalloc(&b_cnt.p, sizeof(int32_t) * std::max<int64_t>(nu, 1)));
+    // the sort's second key buffer is free now: it holds the runs' first positions
+    VRS_HIP(ctx, sorted_unique_runs((const uint64_t *)b_keys.p, n, b_uniq.p, nu,
+                                    (uint64_t *)b_ukeys.p, (int64_t *)b_tmp.p, (int32_t *)b_cnt.p,
+                                    ctx->stream));
+    if (uvw_unique) {
+        Staged su(ctx, uvw_unique, sizeof(float) * 3 * nu, true);
+        if (vrs_status s = su.prepare()) return s;
+        VRS_HIP(ctx, grid_decode((const uint64_t *)b_ukeys.p, nu, (float *)su.dev, ctx->stream));
+        if (vrs_status s = su.finish()) return s;
+    }
+    if (counts) {
+        Staged sc2(ctx, counts, sizeof(int32_t) * nu, true);
+        if (vrs_status s = sc2.prepare()) return s;
+        VRS_HIP(ctx, hipMemcpyAsync(sc2.dev, b_cnt.p, sizeof(int32_t) * nu,
+                                    hipMemcpyDeviceToDevice, ctx->stream));
+        if (vrs_status s = sc2.finish()) return s;
     }
     if (weight || vis_count) {
         VRS_HIP(ctx, hipMalloc(&b_vkeys.p, sizeof(uint64_t) * std::max<int64_t>(g->n_vis, 1)));
@@ -1720,7 +1721,7 @@ vrs_status vrs_geom_fine_sky(vrs_ctx *ctx, vrs_geom *g, int64_t n_cells, const d
         void *p = nullptr;
         ~Buf() { if (p) (void)hipFree(p); }
     } b_keys, b_tmp, b_small;
-    const int64_t n = n_cells, nb = (n + grid_tile() - 1) / grid_tile();
+    const int64_t n = n_cells;
     vrs_status st = VRS_OK;
     auto alloc = [&](void **p, size_t bytes) {
         if (st == VRS_OK && hipMalloc(p, std::max(bytes, (size_t)16)) != hipSuccess) {
@@ -1738,17 +1739,17 @@ vrs_status vrs_geom_fine_sky(vrs_ctx *ctx, vrs_geom *g, int64_t n_cells, const d
     alloc((void **)&g->d_fchunk, sizeof(ChunkRec) * g->n_chunk);
     alloc(&b_keys.p, sizeof(uint64_t) * n);
     alloc(&b_tmp.p, sizeof(uint64_t) * n);
-    alloc(&b_small.p, sizeof(int) * (nb + 8));
+    alloc(&b_small.p, radix_scratch_bytes(n) + 256);
     if (st != VRS_OK) {
         g->free_fine();
         return st;
     }
-    int *bad = (int *)b_small.p + nb + 4;
+    int *bad = (int *)((char *)b_small.p + radix_scratch_bytes(n));
     hipError_t he = hipMemsetAsync(bad, 0, sizeof(int), ctx->stream);
     if (he == hipSuccess)
         he = fine_build_index((const double *)sl.dev, (const float *)sf.dev, (const int32_t *)sn.dev,
                               nx_base, n, g->n_pix, g->d_ring, g->n_pad, g->d_cell32, g->d_cell64,
-                              (uint64_t *)b_keys.p, (uint64_t *)b_tmp.p, (int *)b_small.p, bad,
+                              (uint64_t *)b_keys.p, (uint64_t *)b_tmp.p, b_small.p, bad,
                               g->d_fidx, g->d_fstart, g->d_fcount, ctx->stream);
     int hb = 0;
     if (he == hipSuccess) he = hipMemcpyAsync(&hb, bad, sizeof hb, hipMemcpyDeviceToHost, ctx->stream);
