@@ -15,6 +15,10 @@
     # markdown table of a --kernel-trace --stats run
     python -m viss_recon_shade_amd.pmc stats OUT/trace
 
+    # per-sort breakdown of the uvw gridding in a --kernel-trace run (one table
+    # per vrs_grid_uvw call, keyed by its step count)
+    python -m viss_recon_shade_amd.pmc sorts OUT/trace
+
 Counter values are summed over the dimensions rocprofv3 reports (shader
 engines, XCDs) per dispatch and averaged over the dispatches of each kernel.
 """
@@ -168,6 +172,38 @@ def stats(args):
         print(f"| `{n}` | {c} | {a:.3f} | {mn:.3f} | {mx:.3f} | {pc:.2f} |")
 
 
+def sorts(args):
+    """One markdown table per gridding call in a kernel trace: the dispatches from
+    one grid_keys_kernel to the next, summed per kernel."""
+    files = glob.glob(os.path.join(args.dir, "**", "*kernel_trace.csv"), recursive=True)
+    if not files:
+        raise SystemExit(f"no kernel_trace.csv under {args.dir}")
+    with open(files[0]) as fh:
+        rows = sorted(csv.DictReader(fh), key=lambda r: int(r["Start_Timestamp"]))
+    calls, cur = [], None
+    for r in rows:
+        name = short_name(r["Kernel_Name"])
+        if name.startswith("grid_keys_kernel"):
+            cur = {"steps": int(r["Grid_Size_X"]), "k": collections.OrderedDict()}
+            calls.append(cur)
+        elif cur is None or not any(name.startswith(p) for p in (
+                "key_and_or", "tile_hist", "seg_sum", "scan64", "seg_offsets", "radix_scatter",
+                "run_count", "run_starts", "run_counts", "decode")):
+            continue
+        ms = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-6
+        c = cur["k"].setdefault(name, [0, 0.0])
+        c[0] += 1
+        c[1] += ms
+    for c in calls:
+        tot = sum(v[1] for v in c["k"].values())
+        print(f"\nvrs_grid_uvw call, grid_keys_kernel launched over {c['steps']} threads "
+              f"(steps): {tot:.3f} ms of kernels\n")
+        print("| kernel | dispatches | total ms | average ms |")
+        print("|---|---|---|---|")
+        for n, (k, ms) in sorted(c["k"].items(), key=lambda kv: -kv[1][1]):
+            print(f"| `{n}` | {k} | {ms:.3f} | {ms / k:.3f} |")
+
+
 def copy(_args):
     import torch
     x = torch.ones(COPY_BYTES // 4, device="cuda")
@@ -192,8 +228,10 @@ def main():
     st = sub.add_parser("stats")
     st.add_argument("dir")
     st.add_argument("--all", action="store_true")
+    so = sub.add_parser("sorts")
+    so.add_argument("dir")
     args = ap.parse_args()
-    {"traffic": traffic, "mix": mix, "copy": copy, "stats": stats}[args.cmd](args)
+    {"traffic": traffic, "mix": mix, "copy": copy, "stats": stats, "sorts": sorts}[args.cmd](args)
 
 
 if __name__ == "__main__":
