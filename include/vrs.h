@@ -329,6 +329,23 @@ vrs_status vrs_write_shade_txt(const char *path, const int32_t *box4, int64_t n)
 vrs_status vrs_read_shade_txt(const char *path, int32_t *box4, int64_t cap, int64_t *n_read);
 vrs_status vrs_write_image_txt(const char *path, const float *F, int64_t n);
 vrs_status vrs_read_image_txt(const char *path, float *F, int64_t cap, int64_t *n_read);
+/* The 1 MHz pipeline's weights (vissGen1M_shade_accurate.cu:248-289, 352-368):
+ * map_path is a uvwMap file (first line: the number of cells; then one
+ * "u v w count" line of integers per cell; lines not starting with four
+ * integers are skipped, a later line for a cell replaces an earlier one).
+ * w[i] = 1 / count of row i's cell, 1 where the table has no such cell; the
+ * cell of a row is its u v w rounded to FP32 and truncated toward zero.
+ * n_cells, n_skipped, n_found (nullable): cells in the table, lines skipped,
+ * rows whose cell was found.  Unlike the reference, a file that does not open
+ * (VRS_E_IO; the reference silently weights every row 1), a first line
+ * without a count (VRS_E_IO), a count <= 0 (VRS_E_ARG; the reference divides
+ * by it) and a row outside the int32 range (VRS_E_ARG) are errors.
+ * vrs_write_uvw_map_txt writes such a table from integer-valued rows and
+ * their counts (e.g. vrs_grid_uvw's). */
+vrs_status vrs_uvw_map_weights(const char *map_path, const double *uvw, int64_t n, float *w,
+                               int64_t *n_cells, int64_t *n_skipped, int64_t *n_found);
+vrs_status vrs_write_uvw_map_txt(const char *path, const double *uvw, const int32_t *counts,
+                                 int64_t n);
 /* Image expanded to fine cells, one line per cell c: F[nx[c] - nx_base]
  * (the Ex files, vissGen1M_shade_accurate.cu:567-588). */
 vrs_status vrs_write_ex_txt(const char *path, const float *F, int64_t n_pix, const int32_t *nx,

@@ -218,3 +218,45 @@ def test_orbit_windows_sample_index(ctx, oracle):
     dead = np.concatenate([b * len(idx) + np.nonzero(~live)[0] for b in range(g.n_base)])
     assert np.all(st["count"][dead] == 0)
     g.close()
+
+
+def test_uvw_map_weighted_rows_1mhz(ctx, oracle, tmp_path, report):
+    """The 1 MHz pipeline's stage 3-4 inputs (vissGen1M_shade_accurate.cu:248-289,
+    345-381): one period's gridded samples as the rows of its uvw file, weighted
+    1/count through a uvwMap table of cell counts over three periods (the
+    reference's table counts over 130).  The orbits repeat each period, so most
+    cells count 3.  The looked-up weights equal the gridding's own per-sample
+    weights bit for bit; V and F against the FP64 oracle."""
+    wl, n_per = W.CONFIGS["small"], 3
+    period = 2 * math.pi * math.sqrt(W.A_KM ** 3 / W.MU_MOON)
+    raan, m0 = W.orbit_params(wl.n_sat)
+    g = ctx.geometry(wl.n_sat, n_per * wl.n_t, 1e6, wl.nside, raan=raan, m0=m0,
+                     dt_s=period / wl.n_t, uvw_ceil=True, lambda_int=True)
+    _, uvw = g.MIncline()
+    gr = g.grid_uvw(ordered=False, lambda_int=True, weights=True)
+    assert np.median(gr["counts"]) == n_per
+    rows = uvw[:, :wl.n_t].reshape(-1, 3).astype(np.float64)       # period 1
+    want_w = gr["weight"].reshape(len(g.pairs), -1)[:, :wl.n_t].reshape(-1)
+    p = refio.paths(str(tmp_path), 1e6, 1)
+    refio.write_uvw_map(p["uvwMap"], gr["uvw"], gr["counts"])
+    refio.write_uvw(p["uvw"], rows)
+    rows_in = refio.read_uvw(p["uvw"], n_cols=3)
+    assert np.array_equal(rows_in, rows)
+    w, info = refio.uvw_map_weights(p["uvwMap"], rows_in)
+    assert info["found"] == len(rows) and info["cells"] == len(gr["counts"])
+    assert np.array_equal(w, want_w)
+    gu = ctx.uvw_geometry(rows_in, nside=wl.nside)
+    I = W.make_sky(wl)
+    k = vrs.knobs("reference", precision="mixed")
+    V = gu.visscal(I, k)
+    F = gu.imagerecon(V, w, k)
+    lmn = gu.lmn()
+    Vo = oracle.visgen_rows(rows_in, lmn, I, s=k.phase_scale, wfac=k.wfac)
+    Fo = oracle.recon_rows(rows_in, lmn, cplx(V), s=k.phase_scale, wfac=k.wfac, weights=w)
+    ev, ef = rel(cplx(V), Vo), rel(F, Fo)
+    print(f"[1 MHz uvwMap rows] {len(rows)} rows, {info['cells']} cells: V rel {ev:.2e} "
+          f"F rel {ef:.2e}", flush=True)
+    report("1 MHz uvw rows weighted through uvwMap vs FP64", V=ev, F=ef)
+    assert ev <= TOL and ef <= TOL
+    gu.close()
+    g.close()

@@ -105,3 +105,76 @@ def test_binary_visibilities_round_trip(tmp_path):
     with pytest.raises(vrs.VrsError) as e:
         refio.read_vis(p)
     assert e.value.code == "VRS_E_IO"
+
+
+# ------------------------------------------------------------------ uvwMap
+UVW_MAP = """5
+-23 -288 -166 4
+0 0 0 2
+7 -1 3 8
+this line is not a cell
+7 -1 3 5
+12 40 -9
+"""
+
+
+def test_uvw_map_weights_by_hand(tmp_path):
+    """vissGen1M_shade_accurate.cu:248-289, 352-368 worked by hand: the header
+    line is the cell count; "7 -1 3" appears twice and the later count (5) wins;
+    the prose line and the three-field line are skipped; rows are truncated
+    toward zero after rounding to FP32; a row without a cell weighs 1."""
+    p = tmp_path / "uvwMap130.txt"
+    p.write_text(UVW_MAP)
+    rows = np.array([[-23, -288, -166],     # 1/4
+                     [0.9, -0.7, 0.2],      # truncates to 0 0 0: 1/2
+                     [7.99, -1.5, 3.0],     # 7 -1 3: 1/5 (the later line)
+                     [12, 40, -9],          # only in the three-field line: 1
+                     [-23, -288, -167],     # absent: 1
+                     [-0.5, 0, 0]],         # -0.5 truncates to 0: 1/2
+                    np.float64)
+    w, info = refio.uvw_map_weights(str(p), rows)
+    assert w.dtype == np.float32
+    assert w.tolist() == [np.float32(1 / 4), np.float32(1 / 2), np.float32(1 / 5), 1.0, 1.0,
+                          np.float32(1 / 2)]
+    assert info == {"cells": 3, "skipped": 2, "found": 4}
+
+
+def test_uvw_map_round_trip_with_gridded_counts(tmp_path):
+    rng = np.random.default_rng(5)
+    cells = np.unique(rng.integers(-500, 501, (300, 3)), axis=0).astype(np.float64)
+    counts = rng.integers(1, 40, len(cells)).astype(np.int32)
+    p = os.path.join(str(tmp_path), "frequency_1M", "uvwMap130.txt")
+    refio.write_uvw_map(p, cells, counts)
+    lines = open(p).read().splitlines()
+    assert lines[0] == str(len(cells)) and len(lines) == len(cells) + 1
+    assert lines[1] == " ".join(str(int(x)) for x in cells[0]) + f" {counts[0]}"
+    rows = np.concatenate([cells, cells[::7] + 1000])          # the shifted rows are absent
+    w, info = refio.uvw_map_weights(p, rows)
+    assert np.array_equal(w[:len(cells)], (1.0 / counts.astype(np.float32)).astype(np.float32))
+    assert np.all(w[len(cells):] == 1.0)
+    assert info == {"cells": len(cells), "skipped": 0, "found": len(cells)}
+    assert refio.paths(str(tmp_path), 1e6, 1)["uvwMap"] == p
+
+
+def test_uvw_map_errors(tmp_path):
+    rows = np.zeros((2, 3))
+    with pytest.raises(vrs.VrsError, match="VRS_E_IO"):
+        refio.uvw_map_weights(str(tmp_path / "missing.txt"), rows)
+    bad = tmp_path / "nohead.txt"
+    bad.write_text("cells\n1 2 3 4\n")
+    with pytest.raises(vrs.VrsError, match="VRS_E_IO"):
+        refio.uvw_map_weights(str(bad), rows)
+    zero = tmp_path / "zero.txt"
+    zero.write_text("1\n0 0 0 0\n")
+    with pytest.raises(vrs.VrsError, match="VRS_E_ARG"):
+        refio.uvw_map_weights(str(zero), rows)
+    ok = tmp_path / "ok.txt"
+    ok.write_text("1\n0 0 0 3\n")
+    with pytest.raises(vrs.VrsError, match="VRS_E_ARG"):
+        refio.uvw_map_weights(str(ok), np.array([[3e9, 0, 0]]))
+    with pytest.raises(vrs.VrsError, match="VRS_E_ARG"):
+        refio.uvw_map_weights(str(ok), np.array([[np.nan, 0, 0]]))
+    with pytest.raises(vrs.VrsError, match="VRS_E_ARG"):
+        refio.write_uvw_map(str(tmp_path / "w.txt"), rows, [1, 0])
+    w, info = refio.uvw_map_weights(str(ok), np.zeros((0, 3)))
+    assert w.shape == (0,) and info["cells"] == 1

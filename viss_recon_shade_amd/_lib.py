@@ -118,6 +118,8 @@ SIGNATURES = {
     "vrs_grid_uvw": (C.c_int, [_ctx, _vp, C.c_int32, C.c_int64, _vp, _vp, C.POINTER(C.c_int64),
                                _vp, _vp]),
     "vrs_write_uvw_txt": (C.c_int, [C.c_char_p, _vp, C.c_int64, _vp]),
+    "vrs_uvw_map_weights": (C.c_int, [C.c_char_p, _vp, C.c_int64, _vp, _vp, _vp, _vp]),
+    "vrs_write_uvw_map_txt": (C.c_int, [C.c_char_p, _vp, _vp, C.c_int64]),
     "vrs_read_uvw_txt": (C.c_int, [C.c_char_p, C.c_int32, _vp, _vp, C.c_int64,
                                    C.POINTER(C.c_int64)]),
     "vrs_write_shade_txt": (C.c_int, [C.c_char_p, _vp, C.c_int64]),

@@ -1548,6 +1548,29 @@ vrs_status vrs_read_uvw_txt(const char *path, int32_t n_cols, double *uvw, float
     return io_status(read_uvw_txt(path, n_cols, uvw, col4, cap, n_read), "vrs_read_uvw_txt", path);
 }
 
+vrs_status vrs_uvw_map_weights(const char *map_path, const double *uvw, int64_t n, float *w,
+                               int64_t *n_cells, int64_t *n_skipped, int64_t *n_found) {
+    if (!map_path || n < 0 || (n > 0 && (!uvw || !w)))
+        return fail(nullptr, VRS_E_ARG, "bad map_path, uvw, w or n");
+    switch (uvw_map_weights(map_path, uvw, n, w, n_cells, n_skipped, n_found)) {
+        case 0: return VRS_OK;
+        case -3: return fail(nullptr, VRS_E_IO, "vrs_uvw_map_weights: no cell count on the first line of %s", map_path);
+        case -4: return fail(nullptr, VRS_E_ARG, "vrs_uvw_map_weights: a uvw row is not finite or outside the int32 range");
+        case -5: return fail(nullptr, VRS_E_ARG, "vrs_uvw_map_weights: a count <= 0 in %s", map_path);
+        default: return fail(nullptr, VRS_E_IO, "vrs_uvw_map_weights: cannot open or read %s", map_path);
+    }
+}
+
+vrs_status vrs_write_uvw_map_txt(const char *path, const double *uvw, const int32_t *counts,
+                                 int64_t n) {
+    if (!path || n < 0 || (n > 0 && (!uvw || !counts)))
+        return fail(nullptr, VRS_E_ARG, "bad path, uvw, counts or n");
+    const int rc = write_uvw_map_txt(path, uvw, counts, n);
+    if (rc == -4)
+        return fail(nullptr, VRS_E_ARG, "vrs_write_uvw_map_txt: a row outside the int32 range or a count <= 0");
+    return io_status(rc, "vrs_write_uvw_map_txt", path);
+}
+
 vrs_status vrs_write_shade_txt(const char *path, const int32_t *box4, int64_t n) {
     if (!path || (n > 0 && !box4) || n < 0) return fail(nullptr, VRS_E_ARG, "bad path, box4 or n");
     return io_status(write_shade_txt(path, box4, n), "vrs_write_shade_txt", path);

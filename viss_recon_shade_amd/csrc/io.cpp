@@ -8,13 +8,18 @@
 //                                                 vissGen10M_shade.cu:376-411
 //   F (image)          one real value per line    vissGen10M_shade.cu:563-583
 //   Ex (fine cells)    F of each cell's parent    vissGen1M_shade_accurate.cu:567-588
+//   uvwMap             "n" then "u v w count"     read at vissGen1M_shade_accurate.cu:248-289
 // plus a binary complex64 visibility file (the reference never writes V).
 // The readers never write past the caller's capacity (the reference's fixed
 // uvw_presize buffers have no bound check, vissGen10M_shade.cu:56,359).
+#include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <sstream>
 #include <string>
+#include <unordered_map>
 
 #include "io.hpp"
 
@@ -146,6 +151,93 @@ int read_vis_bin(const char *path, float *V, int64_t cap, int64_t *n_maps, int64
     if (nm * nv > cap) return -2;
     f.read(reinterpret_cast<char *>(V), (std::streamsize)(sizeof(float) * 2 * nm * nv));
     return f.good() ? 0 : -3;
+}
+
+// ------------------------------------------------------------------ uvwMap
+// The 1 MHz pipeline's cell-count table (vissGen1M_shade_accurate.cu:248-289):
+// a first line holding the number of cells, then one "u v w count" line of
+// integers per cell.  A line that does not start with four integers is
+// skipped; a later line for a cell replaces an earlier one.  Row i's weight is
+// 1 / count of its cell, or 1 where the table has no such cell (:352-368).  A
+// row's cell is its u v w rounded to FP32 (the reference reads the uvw file
+// as float) and truncated toward zero (static_cast<int>).
+namespace {
+struct Cell {
+    int32_t u, v, w;
+    bool operator==(const Cell &o) const { return u == o.u && v == o.v && w == o.w; }
+};
+struct CellHash {
+    size_t operator()(const Cell &c) const {
+        uint64_t h = (uint64_t)(uint32_t)c.u * 0x9E3779B97F4A7C15ull;
+        h ^= (uint64_t)(uint32_t)c.v + 0x7F4A7C159E3779B9ull + (h << 6) + (h >> 2);
+        h ^= (uint64_t)(uint32_t)c.w * 0xC2B2AE3D27D4EB4Full + (h << 6) + (h >> 2);
+        return (size_t)(h ^ (h >> 31));
+    }
+};
+
+// static_cast<int>(float) is defined only inside the int range
+bool row_cell(const double *r, Cell *c) {
+    int32_t out[3];
+    for (int k = 0; k < 3; ++k) {
+        const float x = (float)r[k];
+        if (!std::isfinite(x) || x <= -2147483904.0f || x >= 2147483648.0f) return false;
+        out[k] = (int32_t)x;
+    }
+    *c = Cell{out[0], out[1], out[2]};
+    return true;
+}
+}  // namespace
+
+int uvw_map_weights(const char *path, const double *uvw, int64_t n, float *w, int64_t *n_cells,
+                    int64_t *n_skipped, int64_t *n_found) {
+    std::ifstream f(path);
+    if (!f.is_open()) return -1;
+    std::string line;
+    if (!std::getline(f, line)) return -3;
+    char *end = nullptr;
+    const long long declared = std::strtoll(line.c_str(), &end, 10);
+    if (end == line.c_str() || declared < 0) return -3;      // no leading count
+    std::unordered_map<Cell, float, CellHash> table;
+    table.reserve((size_t)std::min<long long>(declared, 1ll << 28));
+    int64_t skipped = 0;
+    while (std::getline(f, line)) {
+        std::istringstream in(line);
+        int u, v, ww, count;
+        if (in >> u >> v >> ww >> count) {
+            if (count <= 0) return -5;
+            table[Cell{u, v, ww}] = (float)count;
+        } else {
+            ++skipped;
+        }
+    }
+    int64_t found = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        Cell c;
+        if (!row_cell(uvw + 3 * i, &c)) return -4;
+        const auto it = table.find(c);
+        if (it != table.end()) {
+            w[i] = 1.0f / it->second;
+            ++found;
+        } else {
+            w[i] = 1.0f;
+        }
+    }
+    if (n_cells) *n_cells = (int64_t)table.size();
+    if (n_skipped) *n_skipped = skipped;
+    if (n_found) *n_found = found;
+    return 0;
+}
+
+int write_uvw_map_txt(const char *path, const double *uvw, const int32_t *counts, int64_t n) {
+    std::ofstream f(path);
+    if (!f.is_open()) return -1;
+    f << n << "\n";
+    for (int64_t i = 0; i < n; ++i) {
+        Cell c;
+        if (!row_cell(uvw + 3 * i, &c) || counts[i] <= 0) return -4;
+        f << c.u << " " << c.v << " " << c.w << " " << counts[i] << "\n";
+    }
+    return f.good() ? 0 : -1;
 }
 
 }  // namespace vrs

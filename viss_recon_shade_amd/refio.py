@@ -9,6 +9,8 @@ Names (directory per frequency, one file per period p, 1-based):
   F_recon_{F}M/F{p}period{F}M.txt                   vissGen10M_shade.cu:573
   F_recon_{F}M_accu/Ex{p}frequency{F}M.txt          vissGen1M_shade_accurate.cu:573-582
   frequency_{F}M/V{p}frequency{F}M.bin              binary complex64 (no reference file)
+  frequency_1M/uvwMap{P}.txt                        cell counts over P periods, read at
+                                                    vissGen1M_shade_accurate.cu:252
 The readers and writers are the C-ABI's (vrs_read_* / vrs_write_*).
 """
 import ctypes as C
@@ -34,6 +36,7 @@ def paths(root, freq_hz, period):
         "F": os.path.join(root, f"F_recon_{f}M", f"F{period}period{f}M.txt"),
         "Ex": os.path.join(root, f"F_recon_{f}M_accu", f"Ex{period}frequency{f}M.txt"),
         "V": os.path.join(d, f"V{period}frequency{f}M.bin"),
+        "uvwMap": os.path.join(d, "uvwMap130.txt"),
     }
 
 
@@ -60,6 +63,29 @@ def read_uvw(path, n_cols=3, cap=1 << 26):
     _lib.check(_lib.load().vrs_read_uvw_txt(path.encode(), n_cols, uvw.ctypes.data,
                                             col4.ctypes.data, len(uvw), C.byref(n)))
     return (uvw[:n.value], col4[:n.value]) if n_cols == 4 else uvw[:n.value]
+
+
+def uvw_map_weights(map_path, uvw):
+    """Weights 1/count of each uvw row's cell in a uvwMap table (1 for a cell
+    the table lacks), as vissGen1M_shade_accurate.cu:248-289, 352-368.
+    Returns (w float32 [n], info) with info = cells, skipped lines, rows found."""
+    uvw = np.ascontiguousarray(uvw, np.float64).reshape(-1, 3)
+    w = np.empty(len(uvw), np.float32)
+    cells, skipped, found = C.c_int64(), C.c_int64(), C.c_int64()
+    _lib.check(_lib.load().vrs_uvw_map_weights(
+        map_path.encode(), uvw.ctypes.data, len(uvw), w.ctypes.data, C.byref(cells),
+        C.byref(skipped), C.byref(found)))
+    return w, {"cells": cells.value, "skipped": skipped.value, "found": found.value}
+
+
+def write_uvw_map(path, uvw, counts):
+    """A uvwMap table: the number of cells, then "u v w count" per cell."""
+    uvw = np.ascontiguousarray(uvw, np.float64).reshape(-1, 3)
+    counts = np.ascontiguousarray(counts, np.int32).reshape(-1)
+    if len(counts) != len(uvw):
+        raise ValueError(f"counts has {len(counts)} entries for {len(uvw)} rows")
+    _lib.check(_lib.load().vrs_write_uvw_map_txt(_mk(path), uvw.ctypes.data, counts.ctypes.data,
+                                                 len(uvw)))
 
 
 def write_shade(path, box4):
