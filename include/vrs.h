@@ -118,6 +118,11 @@ typedef struct vrs_knobs {
     float t_occ;           /* intensity of shaded pixels in vis-gen (vissGen10M_shade.cu:132: 240) */
     double kappa;          /* recon normalisation; 0 selects 1/N_vis (vissGen10M_shade.cu:156,173) */
     int32_t precision;     /* vrs_precision */
+    int32_t antipodal;     /* 1 (default): on a HEALPix sky of Nside >= 16 one phase and one
+                            * sin/cos serve a pixel and its antipode, whose phase is
+                            * 2 wo - phi (E(-p) = conj(E(p)) exp(-4 pi i wo)); 0: every
+                            * (visibility, pixel) term forms its own phase.  Shading is
+                            * decided per pixel either way. */
 } vrs_knobs;
 
 typedef struct vrs_shade_stats {
@@ -387,6 +392,11 @@ vrs_status vrs_uvw_position_index(int32_t orbit_res, int32_t orbit_counts, int32
 vrs_status vrs_healpix_lmn(int32_t nside, double *lmn);
 /* NESTED -> RING index map [12 nside^2]. */
 vrs_status vrs_healpix_nest2ring(int32_t nside, int64_t *ring_of_nest);
+/* RING index of each RING pixel's antipode [12 nside^2] (pixel centres are
+ * antipodally symmetric: ring R and ring 4 nside - R, azimuth phi + pi); the
+ * vis-gen and recon kernels pair a pixel with its antipode under
+ * vrs_knobs.antipodal. */
+vrs_status vrs_healpix_antipode(int32_t nside, int64_t *antipode_of_ring);
 
 #ifdef __cplusplus
 }

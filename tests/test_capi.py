@@ -133,6 +133,29 @@ def test_healpix_matches_oracle(oracle, nside):
     assert np.array_equal(n2r, oracle.nest2ring(nside))
 
 
+@pytest.mark.parametrize("nside", [1, 4, 16, 128])
+def test_healpix_antipode(oracle, nside):
+    """Pixel centres are antipodally symmetric: the map is an involution onto
+    the pixel set and each pair of directions sums to zero (to FP64 rounding of
+    pix2vec).  Checked against the oracle's HEALPix vectors."""
+    lib = vrs.load()
+    n = 12 * nside * nside
+    a = np.empty(n, np.int64)
+    _lib.check(lib.vrs_healpix_antipode(nside, a.ctypes.data))
+    assert np.array_equal(np.sort(a), np.arange(n)) and np.array_equal(a[a], np.arange(n))
+    assert not np.any(a == np.arange(n))
+    v = oracle.healpix_ring(nside)
+    assert np.abs(v + v[a]).max() <= 2e-15
+    # the antipode of a NESTED face 0-5 pixel lies in faces 6-11 (the chunk layout
+    # pairs the two halves chunk by chunk)
+    n2r = np.empty(n, np.int64)
+    _lib.check(lib.vrs_healpix_nest2ring(nside, n2r.ctypes.data))
+    r2n = np.empty(n, np.int64)
+    r2n[n2r] = np.arange(n)
+    assert np.all(r2n[a[n2r[: n // 2]]] >= n // 2)
+    assert lib.vrs_healpix_antipode(3, a.ctypes.data) == -1
+
+
 def test_pix_hash_matches_oracle(oracle):
     lib = vrs.load()
     for p in [0, 1, 7, 196607, 2 ** 31 + 5]:

@@ -57,7 +57,7 @@ class GeomInfo(C.Structure):
 
 class Knobs(C.Structure):
     _fields_ = [("phase_scale", C.c_double), ("wfac", C.c_int32), ("t_occ", C.c_float),
-                ("kappa", C.c_double), ("precision", C.c_int32)]
+                ("kappa", C.c_double), ("precision", C.c_int32), ("antipodal", C.c_int32)]
 
 
 class BatchDesc(C.Structure):
@@ -138,6 +138,7 @@ SIGNATURES = {
     "vrs_uvw_position_index": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, _vp]),
     "vrs_healpix_lmn": (C.c_int, [C.c_int32, _vp]),
     "vrs_healpix_nest2ring": (C.c_int, [C.c_int32, _vp]),
+    "vrs_healpix_antipode": (C.c_int, [C.c_int32, _vp]),
 }
 
 _lib = None

@@ -84,6 +84,7 @@ struct vrs_geom {
     vrs_ctx *ctx = nullptr;  // owning context; must outlive the geometry
     int device = 0;
     int32_t n_sat = 0, n_t = 0, n_base = 0, nside = 0, n_chunk = 0;
+    int32_t n_half = 0;  // > 0: chunk k + n_half holds the antipodes of chunk k, slot by slot
     int64_t n_vis = 0, n_pix = 0, n_pad = 0;
     double lambda_m = 0, r_moon = 0, freq_hz = 0;
     int32_t uvw_flags = 0;
@@ -121,7 +122,7 @@ struct vrs_geom {
     }
     GeomDev dev() const {
         GeomDev g = {};
-        g.n_sat = n_sat; g.n_t = n_t; g.n_base = n_base; g.n_chunk = n_chunk;
+        g.n_sat = n_sat; g.n_t = n_t; g.n_base = n_base; g.n_chunk = n_chunk; g.n_half = n_half;
         g.n_vis = n_vis; g.n_pad = n_pad;
         g.pos = d_pos; g.cap = d_cap; g.uvw64 = d_uvw64; g.vis_cap = d_viscap; g.pairs = d_pairs;
         g.pix32 = d_pix32; g.pix64 = d_pix64; g.ring = d_ring; g.chunk = d_chunk;
@@ -278,14 +279,19 @@ vrs_status check_knobs(vrs_ctx *ctx, const vrs_knobs *k) {
         return fail(ctx, VRS_E_ARG, "kappa must be finite and >= 0 (0 selects 1/N_vis)");
     if (k->precision < VRS_PLAIN_F32 || k->precision > VRS_MFMA_SPLIT16)
         return fail(ctx, VRS_E_ARG, "unknown precision %d", k->precision);
+    if (k->antipodal != 0 && k->antipodal != 1) return fail(ctx, VRS_E_ARG, "antipodal must be 0 or 1");
     return VRS_OK;
 }
+
+// Autotuner cache key part: precision and antipodal pairing.
+int32_t tune_mode(const vrs_knobs *k) { return k->precision + 16 * k->antipodal; }
 
 PhaseKnobs phase_knobs(const vrs_knobs *k) {
     PhaseKnobs p;
     p.s = k->phase_scale;
     p.wfac = k->wfac;
     p.t_occ = k->t_occ;
+    p.pair = k->antipodal;
     return p;
 }
 
@@ -302,6 +308,7 @@ void vrs_knobs_default(vrs_knobs *k) {
     k->t_occ = 240.0f;
     k->kappa = 0.0;
     k->precision = VRS_PLAIN_F32;
+    k->antipodal = 1;
 }
 
 void vrs_knobs_reference(vrs_knobs *k) {
@@ -311,6 +318,7 @@ void vrs_knobs_reference(vrs_knobs *k) {
     k->t_occ = 240.0f;
     k->kappa = 0.0;
     k->precision = VRS_PLAIN_F32;
+    k->antipodal = 1;
 }
 
 int vrs_device_count(void) {
@@ -519,6 +527,14 @@ vrs_status vrs_healpix_nest2ring(int32_t nside, int64_t *out) {
     return VRS_OK;
 }
 
+vrs_status vrs_healpix_antipode(int32_t nside, int64_t *out) {
+    if (nside < 1 || nside > 8192 || (nside & (nside - 1)) || !out)
+        return fail(nullptr, VRS_E_ARG, "nside must be a power of two in [1, 8192]");
+    const int64_t n = 12LL * nside * nside;
+    for (int64_t p = 0; p < n; ++p) out[p] = healpix_ring_antipode(nside, p);
+    return VRS_OK;
+}
+
 // ---------------------------------------------------------------- geometry
 void vrs_geom_destroy(vrs_geom *g) {
     if (!g) return;
@@ -549,8 +565,34 @@ static vrs_status sky_build(vrs_ctx *ctx, vrs_geom *g, int32_t nside, int64_t n_
         std::memcpy(g->lmn_ring.data(), lmn, sizeof(double) * 3 * g->n_pix);
     else
         for (int64_t p = 0; p < g->n_pix; ++p) healpix_ring_vec(nside, p, &g->lmn_ring[3 * p]);
+    // Chunk order: NESTED (16 x 16 blocks of a base face are chunks).  From
+    // Nside 16 on, the second half is re-ordered so that position n/2 + q holds
+    // the antipode of position q: base faces 0-5 face faces 6-11, so chunk
+    // k + n_chunk/2 is the antipodal image of chunk k, slot by slot.
+    g->n_half = 0;
+    const bool paired = !lmn && nside >= 16;
+    std::vector<int64_t> order(g->n_pix);
+    for (int64_t q = 0; q < g->n_pix; ++q) order[q] = lmn ? q : healpix_nest2ring(nside, q);
+    if (paired) {
+        const int64_t h = g->n_pix / 2;
+        std::vector<uint8_t> used(g->n_pix, 0);
+        bool ok = (g->n_pix % (2 * kChunk)) == 0;
+        for (int64_t q = 0; q < h && ok; ++q) {
+            const int64_t a = healpix_ring_antipode(nside, order[q]);
+            order[h + q] = a;
+        }
+        for (int64_t q = 0; q < g->n_pix && ok; ++q) {
+            if (used[order[q]]) ok = false;
+            used[order[q]] = 1;
+        }
+        if (ok) {
+            g->n_half = g->n_chunk / 2;
+        } else {
+            for (int64_t q = 0; q < g->n_pix; ++q) order[q] = healpix_nest2ring(nside, q);
+        }
+    }
     for (int64_t q = 0; q < g->n_pix; ++q) {
-        const int64_t r = lmn ? q : healpix_nest2ring(nside, q);
+        const int64_t r = order[q];
         const double *v = &g->lmn_ring[3 * r];
         pix64[q] = make_double4(v[0], v[1], v[2], 0.0);
         pix32[q] = make_float4((float)v[0], (float)v[1], (float)v[2], 0.0f);
@@ -930,16 +972,17 @@ vrs_status vrs_visgen(vrs_ctx *ctx, const vrs_geom *g, const vrs_knobs *k, int32
     VRS_HIP(ctx, launch_to_chunk_order(gd, n_maps, g->n_pix, (const float *)sI.dev, (float *)ich,
                                        ctx->stream));
     const bool f64 = k->precision == VRS_MIXED_F64PHASE;
+    const bool pair = k->antipodal && g->n_half > 0;
     int vt = visgen_tile_vis();
     {
-        auto it = ctx->tuned_vt.find(std::make_tuple(g->n_vis, g->n_pix, n_maps, k->precision));
+        auto it = ctx->tuned_vt.find(std::make_tuple(g->n_vis, g->n_pix, n_maps, tune_mode(k)));
         if (it != ctx->tuned_vt.end()) vt = it->second;
     }
     const int64_t n_tiles = ((g->n_vis + vt - 1) / vt) * n_maps;
-    const int grid = persistent_grid(ctx, visgen_kernel_ptr(f64, vt), n_tiles);
+    const int grid = persistent_grid(ctx, visgen_kernel_ptr(f64, vt, pair), n_tiles);
     {
         Timer tm(ctx, "visgen");
-        VRS_HIP(ctx, launch_visgen(gd, phase_knobs(k), f64, vt, n_maps, (const float *)ich,
+        VRS_HIP(ctx, launch_visgen(gd, phase_knobs(k), f64, vt, pair, n_maps, (const float *)ich,
                                    (float2 *)sV.dev, grid, ctx->stream));
     }
     if (vrs_status s = sV.finish()) return s;
@@ -957,7 +1000,7 @@ vrs_status vrs_autotune_visgen(vrs_ctx *ctx, const vrs_geom *g, const vrs_knobs 
     DeviceGuard dg(ctx->device);
     void *V;
     if (vrs_status s = ws_get(ctx, 2, sizeof(float) * 2 * n_maps * g->n_vis, &V)) return s;
-    const auto key = std::make_tuple(g->n_vis, g->n_pix, n_maps, k->precision);
+    const auto key = std::make_tuple(g->n_vis, g->n_pix, n_maps, tune_mode(k));
     hipEvent_t e0, e1;
     VRS_HIP(ctx, hipEventCreate(&e0));
     VRS_HIP(ctx, hipEventCreate(&e1));
@@ -1138,7 +1181,7 @@ struct vrs_batch {
     hipGraph_t graph = nullptr;
     // knobs the graph was captured with
     double cap_s = 0, cap_kappa = -1;
-    int32_t cap_wfac = -1;
+    int32_t cap_wfac = -1, cap_pair = -1;
     float cap_tocc = 0;
     GeomDev dev() const {
         GeomDev d = g->dev();
@@ -1217,8 +1260,10 @@ vrs_status batch_launch(vrs_batch *b, const PhaseKnobs &pk, bool skip_recon = fa
     const bool f64 = b->precision == VRS_MIXED_F64PHASE;
     if (!skip_visgen) {
         VRS_HIP(ctx, launch_to_chunk_order(gd, b->n_ch, b->g->n_pix, b->d_I, b->d_Ich, b->stream));
-        const int grid = persistent_grid(ctx, visgen_kernel_ptr(f64, b->vt), b->n_pad_vis / b->vt);
-        VRS_HIP(ctx, launch_visgen(gd, pk, f64, b->vt, b->n_ch, b->d_Ich, b->d_V, grid, b->stream));
+        const bool pair = pk.pair && gd.n_half > 0;
+        const int grid = persistent_grid(ctx, visgen_kernel_ptr(f64, b->vt, pair), b->n_pad_vis / b->vt);
+        VRS_HIP(ctx, launch_visgen(gd, pk, f64, b->vt, pair, b->n_ch, b->d_Ich, b->d_V, grid,
+                                   b->stream));
     }
     if (!skip_recon) {
         const int grid = persistent_grid(ctx, recon_kernel_ptr(f64), b->n_slots * b->g->n_chunk);
@@ -1265,6 +1310,7 @@ vrs_status batch_capture(vrs_batch *b, const PhaseKnobs &pk) {
     b->cap_s = pk.s;
     b->cap_wfac = pk.wfac;
     b->cap_tocc = pk.t_occ;
+    b->cap_pair = pk.pair;
     return VRS_OK;
 }
 
@@ -1400,6 +1446,7 @@ vrs_status vrs_batch_create(vrs_ctx *ctx, const vrs_geom *g, const vrs_batch_des
         pk.s = 1.0;
         pk.wfac = 0;
         pk.t_occ = 240.0f;
+        pk.pair = 1;
         double best = 1e300, ms = 0;
         int best_vt = 16;
         if (vrs_status s = batch_time(b, pk, true, &ms)) return cleanup(s);  // warm-up
@@ -1462,7 +1509,8 @@ vrs_status vrs_batch_run(vrs_ctx *ctx, vrs_batch *b, const vrs_knobs *k, const f
     for (int c = 0; c < b->n_ch; ++c)
         kap[c] = (float)(k->kappa > 0 ? k->kappa : (b->kept[c] > 0 ? 1.0 / (double)b->kept[c] : 0.0));
     const PhaseKnobs pk = phase_knobs(k);
-    if (!b->exec || pk.s != b->cap_s || pk.wfac != b->cap_wfac || pk.t_occ != b->cap_tocc) {
+    if (!b->exec || pk.s != b->cap_s || pk.wfac != b->cap_wfac || pk.t_occ != b->cap_tocc ||
+        pk.pair != b->cap_pair) {
         if (vrs_status s = batch_capture(b, pk)) return s;
     }
     // order after the caller's stream, run on the batch stream, order the caller after it

@@ -131,6 +131,42 @@ void healpix_ring_vec(int64_t nside, int64_t p, double v[3]) {
     v[2] = z;
 }
 
+// RING index of the antipode of RING pixel p.  HEALPix centres are antipodally
+// symmetric: ring R (1 .. 4 nside - 1, from the north pole) and ring
+// 4 nside - R hold the same number of pixels at the same azimuths, and pixel j
+// of ring R faces pixel j + (ring size) / 2 of ring 4 nside - R (phi + pi).
+int64_t healpix_ring_antipode(int64_t nside, int64_t p) {
+    const int64_t npix = 12 * nside * nside, ncap = 2 * nside * (nside - 1);
+    auto start = [&](int64_t R) {
+        if (R < nside) return 2 * R * (R - 1);
+        if (R <= 3 * nside) return ncap + (R - nside) * 4 * nside;
+        const int64_t i = 4 * nside - R;
+        return npix - 2 * i * (i + 1);
+    };
+    auto size = [&](int64_t R) {
+        if (R < nside) return 4 * R;
+        if (R <= 3 * nside) return 4 * nside;
+        return 4 * (4 * nside - R);
+    };
+    int64_t R;
+    if (p < ncap) {
+        R = (int64_t)(0.5 * (1.0 + std::sqrt(1.0 + 2.0 * (double)p)));
+        while (2 * R * (R - 1) > p) --R;
+        while (2 * R * (R + 1) <= p) ++R;
+    } else if (p < npix - ncap) {
+        R = (p - ncap) / (4 * nside) + nside;
+    } else {
+        const int64_t q = npix - p;
+        int64_t i = (int64_t)(0.5 * (1.0 + std::sqrt(2.0 * (double)q - 1.0)));
+        while (2 * i * (i - 1) >= q) --i;
+        while (2 * i * (i + 1) < q) ++i;
+        R = 4 * nside - i;
+    }
+    const int64_t j = p - start(R), n = size(R);
+    const int64_t R2 = 4 * nside - R;
+    return start(R2) + (j + n / 2) % n;
+}
+
 int64_t healpix_nest2ring(int64_t nside, int64_t p) {
     static const int jrll[12] = {2, 2, 2, 2, 3, 3, 3, 3, 4, 4, 4, 4};
     static const int jpll[12] = {1, 3, 5, 7, 0, 2, 4, 6, 1, 3, 5, 7};

@@ -11,6 +11,7 @@ void orbit_offsets(int32_t n_sat, uint64_t seed, double a_km, double max_offset_
                    double *raan, double *m0);
 void healpix_ring_vec(int64_t nside, int64_t p, double v[3]);
 int64_t healpix_nest2ring(int64_t nside, int64_t p);
+int64_t healpix_ring_antipode(int64_t nside, int64_t p);
 void uvw_position_index(int32_t orbit_res, int32_t orbit_counts, int32_t interval, int32_t span,
                         int64_t *out);
 }  // namespace vrs

@@ -634,6 +634,323 @@ __global__ void __launch_bounds__(kBlock, (!F64 && VT == 32) ? 3 : 4) visgen_ker
     }
 }
 
+// ------------------------------------------------------- antipodal pairs
+// On the antipodal layout (g.n_half > 0) pixel p of chunk k and its antipode
+// p' = -p, held in the same slot of chunk k + n_half, share one phase:
+// phi(p') = u.p' + wo = 2 wo - phi(p), so E(p') = exp(-2 pi i phi(p')) =
+// conj(E(p)) exp(-4 pi i wo).  One fract and one v_sin / v_cos serve both
+// terms.  With E(p) = cos - i sin and C the pixel's intensity (t_occ where
+// shaded),
+//   V = (A1 - i B1) + exp(-4 pi i wo) (A2 + i B2),
+//   A1 = sum C_p cos, B1 = sum C_p sin, A2 = sum C_p' cos, B2 = sum C_p' sin.
+// Every shade decision is still made per pixel: the FP32 test of p' uses -p
+// (its dot with a cap axis is exactly minus p's), the FP64 predicate of a
+// guard-band pixel reads the antipode's own FP64 direction.
+
+// Classification of the tile's VT pairs against one chunk (lane q + VT s tests
+// cap s of pair q): wholly shaded, wholly clear, cap A / cap B straddling.
+template <bool F64, int VT>
+__device__ __forceinline__ void tile_masks(const ChunkRec &ch, const VisEnt<F64> *ents, int lane,
+                                           uint32_t &in_m, uint32_t &out_m, uint32_t &sa_m,
+                                           uint32_t &sb_m) {
+    constexpr uint32_t kTileMask = (VT == 32) ? 0xffffffffu : ((1u << VT) - 1u);
+    const int q = lane & (VT - 1), s = (lane / VT) & 1;
+    bool cin, cout;
+    cap_in_out(ch, (ch.flags & kChunkAlwaysStraddle) != 0, (&ents[q].a)[s], cin, cout);
+    const uint64_t bin = __ballot(cin), bout = __ballot(cout);
+    const uint32_t ain = (uint32_t)bin & kTileMask;
+    const uint32_t bin_ = (uint32_t)(bin >> VT) & kTileMask;
+    const uint32_t aout = (uint32_t)bout & kTileMask;
+    const uint32_t bout_ = (uint32_t)(bout >> VT) & kTileMask;
+    in_m = ain | bin_;
+    out_m = aout & bout_;
+    sa_m = ~(in_m | aout) & kTileMask;
+    sb_m = ~(in_m | bout_) & kTileMask;
+}
+
+// acc += a * b in the accumulator's own register (v_fmac_f32): the paired
+// vis-gen updates its accumulators from three code paths per step, and the
+// tied operand keeps the register allocator from giving each path its own
+// copy (which costs register moves at every join).
+__device__ __forceinline__ void acc_fmac(float &acc, float a, float b) {
+    asm("v_fmac_f32 %0, %1, %2" : "+v"(acc) : "v"(a), "v"(b));
+}
+
+// One pair's terms over this lane's pixel pairs: one phase, fract, v_sin and
+// v_cos per pixel, accumulated with the pixel's and the antipode's intensity.
+template <bool F64>
+__device__ __forceinline__ void pair_terms(typename PhaseT<F64>::T u, typename PhaseT<F64>::T vv,
+                                           typename PhaseT<F64>::T w, typename PhaseT<F64>::T wo,
+                                           const typename PhaseT<F64>::T (&l)[kPixPerLane],
+                                           const typename PhaseT<F64>::T (&m)[kPixPerLane],
+                                           const typename PhaseT<F64>::T (&n)[kPixPerLane],
+                                           const float (&e1)[kPixPerLane],
+                                           const float (&e2)[kPixPerLane], float &a1, float &b1,
+                                           float &a2, float &b2) {
+    using T = typename PhaseT<F64>::T;
+#pragma unroll
+    for (int j = 0; j < kPixPerLane; ++j) {
+        const T ph = fma(u, l[j], fma(vv, m[j], fma(w, n[j], wo)));
+        const float r = fract_rev(ph);
+        const float cs = __builtin_amdgcn_cosf(r), sn = __builtin_amdgcn_sinf(r);
+        acc_fmac(a1, e1[j], cs);
+        acc_fmac(b1, e1[j], sn);
+        acc_fmac(a2, e2[j], cs);
+        acc_fmac(b2, e2[j], sn);
+    }
+}
+
+// FP32 tests of one cap on this lane's pixel pairs: x = dot(p, dir) for each
+// pixel p and -x for its antipode (negation is exact, and -p rounded to FP32
+// is the antipode's FP32 direction up to one FP32 ulp, far inside the guard
+// band).  t1 / t2: test the pixels / the antipodes (wave-uniform).  Sets the
+// shade flags and ORs in the wave's lanes with a pixel inside the guard band.
+template <typename T>
+__device__ __forceinline__ void pair_cap_tests(const T (&l)[kPixPerLane], const T (&m)[kPixPerLane],
+                                               const T (&n)[kPixPerLane], const CapRec &a, bool t1,
+                                               bool t2, bool (&sh1)[kPixPerLane],
+                                               bool (&sh2)[kPixPerLane], uint64_t &unc1,
+                                               uint64_t &unc2) {
+    float x[kPixPerLane];
+#pragma unroll
+    for (int j = 0; j < kPixPerLane; ++j)
+        x[j] = fmaf((float)n[j], a.dz, fmaf((float)m[j], a.dy, (float)l[j] * a.dx));
+    if (t1) {
+#pragma unroll
+        for (int j = 0; j < kPixPerLane; ++j) {
+            const bool hi = x[j] >= a.hi;
+            unc1 |= __ballot(x[j] >= a.lo) & ~__ballot(hi);
+            sh1[j] = sh1[j] || hi;
+        }
+    }
+    if (t2) {
+#pragma unroll
+        for (int j = 0; j < kPixPerLane; ++j) {
+            const float y = -x[j];
+            const bool hi = y >= a.hi;
+            unc2 |= __ballot(y >= a.lo) & ~__ballot(hi);
+            sh2[j] = sh2[j] || hi;
+        }
+    }
+}
+
+// Guard-band pixels of one straddling pair in chunk `base` (neg: the antipodal
+// chunk, whose FP32 test used the negated FP32 direction of the pixel at
+// base - off2): the exact FP64 predicate on the pixel's own direction replaces
+// the FP32 decision; returns the intensity difference times the pixel's own E.
+template <bool F64>
+__device__ __forceinline__ float2 band_fix(const GeomDev &g, int64_t base, int64_t off2, bool neg,
+                                           int lane, bool ta, bool tb, const VisEnt<F64> &e,
+                                           const float (&I)[kPixPerLane], float t_occ) {
+    using T = typename PhaseT<F64>::T;
+    float dre = 0.0f, dim = 0.0f;
+#pragma unroll
+    for (int j = 0; j < kPixPerLane; ++j) {
+        const int64_t pi = base + j * kWave + lane;
+        float4 p32 = g.pix32[neg ? pi - off2 : pi];
+        if (neg) {
+            p32.x = -p32.x; p32.y = -p32.y; p32.z = -p32.z;
+        }
+        bool unc = false, sh = false;
+        if (ta) sh = occ32(p32.x, p32.y, p32.z, e.a, unc);
+        if (tb) sh = occ32(p32.x, p32.y, p32.z, e.b, unc) || sh;
+        if (unc) {
+            int nl;
+            const bool ex = occ_pair64(g.pos, g.pix64, pi, e.ia, e.ib, g.limb, &nl);
+            if (ex != sh) {
+                const float d = ex ? (t_occ - I[j]) : (I[j] - t_occ);
+                T l, m, n;
+                if (F64) {
+                    const double4 p = g.pix64[pi];
+                    l = (T)p.x; m = (T)p.y; n = (T)p.z;
+                } else {
+                    const float4 p = g.pix32[pi];
+                    l = (T)p.x; m = (T)p.y; n = (T)p.z;
+                }
+                const T ph = fma(e.u, l, fma(e.v, m, fma(e.w, n, e.wo)));
+                const float r = fract_rev(ph);
+                dre = fmaf(d, __builtin_amdgcn_cosf(r), dre);
+                dim = fmaf(d, __builtin_amdgcn_sinf(r), dim);
+            }
+        }
+    }
+    return make_float2(dre, dim);
+}
+
+template <bool F64, int VT>
+__global__ void __launch_bounds__(kBlock, VT == 8 ? 4 : 3) visgen_pair_kernel(GeomDev g, PhaseKnobs k, int32_t n_maps,
+                                                             const float *__restrict__ Ich,
+                                                             float2 *__restrict__ V) {
+    using T = typename PhaseT<F64>::T;
+    static_assert(VT == 8 || VT == 16 || VT == 32, "tile of 8, 16 or 32 visibilities");
+    static_assert(kListPad % VT == 0, "list padding must be whole tiles");
+    __shared__ VisEnt<F64> ents[VT];
+    __shared__ float red[kWavesPerBlock][VT][4];
+    __shared__ float corr[kWavesPerBlock][VT][2];  // guard-band fixes, per wave
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
+    const bool list = g.vis_sat != nullptr;
+    const int64_t n_vtiles = (g.n_vis + VT - 1) / VT;
+    const int64_t n_tiles = list ? n_vtiles : n_vtiles * n_maps;
+    const int nh = g.n_half;
+    const int64_t off2 = (int64_t)nh * kChunk;  // position of a pixel's antipode, relative
+    const float t_occ = k.t_occ;
+    const float tv[kPixPerLane] = {t_occ, t_occ, t_occ, t_occ};
+
+    for (int64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+        const int64_t map = list ? g.tile_map[(tile * VT) / kListPad] : tile / n_vtiles;
+        const int64_t v0 = (list ? tile : tile % n_vtiles) * VT;
+        const float *__restrict__ Im = Ich + map * g.n_pad;
+        if (threadIdx.x < VT) fill_vis_ent<F64>(g, k, v0 + threadIdx.x, map, ents[threadIdx.x]);
+        if (threadIdx.x < kWavesPerBlock * VT * 2) (&corr[0][0][0])[threadIdx.x] = 0.0f;
+        __syncthreads();
+
+        float a1[VT], b1[VT], a2[VT], b2[VT];
+#pragma unroll
+        for (int q = 0; q < VT; ++q) a1[q] = b1[q] = a2[q] = b2[q] = 0.0f;
+
+        for (int c = wave; c < nh; c += kWavesPerBlock) {
+            asm volatile("" ::: "memory");
+            const ChunkRec ch1 = g.chunk[c], ch2 = g.chunk[c + nh];
+            T l[kPixPerLane], m[kPixPerLane], n[kPixPerLane];
+            float I1[kPixPerLane], I2[kPixPerLane];
+#pragma unroll
+            for (int j = 0; j < kPixPerLane; ++j) {
+                const int64_t pi = (int64_t)c * kChunk + j * kWave + lane;
+                if (F64) {
+                    const double4 p = g.pix64[pi];
+                    l[j] = (T)p.x; m[j] = (T)p.y; n[j] = (T)p.z;
+                } else {
+                    const float4 p = g.pix32[pi];
+                    l[j] = (T)p.x; m[j] = (T)p.y; n[j] = (T)p.z;
+                }
+                I1[j] = Im[pi];
+                I2[j] = Im[pi + off2];
+            }
+            uint32_t in1, out1, sa1, sb1, in2, out2, sa2, sb2;
+            tile_masks<F64, VT>(ch1, ents, lane, in1, out1, sa1, sb1);
+            tile_masks<F64, VT>(ch2, ents, lane, in2, out2, sa2, sb2);
+            // Materialise the masks here, in scalar registers: left to itself the
+            // compiler re-derives them from the ballots inside every pair's step.
+            uint32_t both_out = __builtin_amdgcn_readfirstlane(out1 & out2);
+            uint32_t straddle = __builtin_amdgcn_readfirstlane(sa1 | sb1 | sa2 | sb2);
+            in1 = __builtin_amdgcn_readfirstlane(in1);
+            in2 = __builtin_amdgcn_readfirstlane(in2);
+            sa1 = __builtin_amdgcn_readfirstlane(sa1);
+            sb1 = __builtin_amdgcn_readfirstlane(sb1);
+            sa2 = __builtin_amdgcn_readfirstlane(sa2);
+            sb2 = __builtin_amdgcn_readfirstlane(sb2);
+            asm volatile("" : "+s"(both_out), "+s"(straddle), "+s"(in1), "+s"(in2));
+            asm volatile("" : "+s"(sa1), "+s"(sb1), "+s"(sa2), "+s"(sb2));
+            uint32_t unc1 = 0, unc2 = 0;  // pairs with a guard-band pixel in chunk c / c + nh
+#pragma unroll
+            for (int q = 0; q < VT; ++q) {
+                const T u = ents[q].u, vv = ents[q].v, w = ents[q].w, wo = ents[q].wo;
+                // Three paths per pair; each updates the accumulators in place
+                // (acc_fmac), so that they keep their registers across the joins.
+                if ((both_out >> q) & 1u) {
+                    pair_terms<F64>(u, vv, w, wo, l, m, n, I1, I2, a1[q], b1[q], a2[q], b2[q]);
+                } else if (!((straddle >> q) & 1u)) {
+                    // each chunk wholly shaded or wholly clear for this pair, one
+                    // of them shaded: t_occ is a wave-uniform operand
+                    const bool in1q = (in1 >> q) & 1u, in2q = (in2 >> q) & 1u;
+                    if (in1q && in2q)
+                        pair_terms<F64>(u, vv, w, wo, l, m, n, tv, tv, a1[q], b1[q], a2[q], b2[q]);
+                    else if (in1q)
+                        pair_terms<F64>(u, vv, w, wo, l, m, n, tv, I2, a1[q], b1[q], a2[q], b2[q]);
+                    else
+                        pair_terms<F64>(u, vv, w, wo, l, m, n, I1, tv, a1[q], b1[q], a2[q], b2[q]);
+                } else {
+                    // a cap's FP32 dot serves the pixel and (negated) its antipode;
+                    // the wave-uniform tests branch once per cap, not per pixel
+                    const bool ta1 = (sa1 >> q) & 1u, tb1 = (sb1 >> q) & 1u;
+                    const bool ta2 = (sa2 >> q) & 1u, tb2 = (sb2 >> q) & 1u;
+                    bool sh1[kPixPerLane], sh2[kPixPerLane];
+#pragma unroll
+                    for (int j = 0; j < kPixPerLane; ++j) {
+                        sh1[j] = (in1 >> q) & 1u;
+                        sh2[j] = (in2 >> q) & 1u;
+                    }
+                    uint64_t uw1 = 0, uw2 = 0;
+                    if (ta1 || ta2) pair_cap_tests(l, m, n, ents[q].a, ta1, ta2, sh1, sh2, uw1, uw2);
+                    if (tb1 || tb2) pair_cap_tests(l, m, n, ents[q].b, tb1, tb2, sh1, sh2, uw1, uw2);
+                    if (uw1) unc1 |= 1u << q;
+                    if (uw2) unc2 |= 1u << q;
+                    float e1[kPixPerLane], e2[kPixPerLane];
+#pragma unroll
+                    for (int j = 0; j < kPixPerLane; ++j) {
+                        e1[j] = sh1[j] ? t_occ : I1[j];
+                        e2[j] = sh2[j] ? t_occ : I2[j];
+                    }
+                    pair_terms<F64>(u, vv, w, wo, l, m, n, e1, e2, a1[q], b1[q], a2[q], b2[q]);
+                }
+            }
+            // Guard-band pixels: the exact predicate; the difference goes, with
+            // the pixel's own E, to the wave's LDS accumulator (this wave only:
+            // deterministic).
+            while (unc1) {
+                const int q = __ffs(unc1) - 1;
+                unc1 &= unc1 - 1;
+                float2 d = band_fix<F64>(g, (int64_t)c * kChunk, off2, false, lane,
+                                         (sa1 >> q) & 1u, (sb1 >> q) & 1u, ents[q], I1, t_occ);
+                d.x = wave_sum(d.x);
+                d.y = wave_sum(d.y);
+                if (lane == 0) {
+                    corr[wave][q][0] += d.x;
+                    corr[wave][q][1] += d.y;
+                }
+            }
+            while (unc2) {
+                const int q = __ffs(unc2) - 1;
+                unc2 &= unc2 - 1;
+                float2 d = band_fix<F64>(g, (int64_t)c * kChunk + off2, off2, true, lane,
+                                         (sa2 >> q) & 1u, (sb2 >> q) & 1u, ents[q], I2, t_occ);
+                d.x = wave_sum(d.x);
+                d.y = wave_sum(d.y);
+                if (lane == 0) {
+                    corr[wave][q][0] += d.x;
+                    corr[wave][q][1] += d.y;
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < VT; ++q) {
+            const float s0 = wave_sum(a1[q]), s1 = wave_sum(b1[q]);
+            const float s2 = wave_sum(a2[q]), s3 = wave_sum(b2[q]);
+            if (lane == 0) {
+                red[wave][q][0] = s0;
+                red[wave][q][1] = s1;
+                red[wave][q][2] = s2;
+                red[wave][q][3] = s3;
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x < VT) {
+            const int q = threadIdx.x;
+            const int64_t v = v0 + q;
+            const bool real = list ? (v < g.n_vis && g.vis_sat[v].x >= 0) : (v < g.n_vis);
+            if (real) {
+                float A1 = 0.0f, B1 = 0.0f, A2 = 0.0f, B2 = 0.0f, cr = 0.0f, ci = 0.0f;
+#pragma unroll
+                for (int w = 0; w < kWavesPerBlock; ++w) {
+                    A1 += red[w][q][0];
+                    B1 += red[w][q][1];
+                    A2 += red[w][q][2];
+                    B2 += red[w][q][3];
+                    cr += corr[w][q][0];
+                    ci += corr[w][q][1];
+                }
+                // exp(-4 pi i wo) = Cr - i Sr
+                const float r2 = fract_rev(ents[q].wo + ents[q].wo);
+                const float Cr = __builtin_amdgcn_cosf(r2), Sr = __builtin_amdgcn_sinf(r2);
+                V[(list ? 0 : map * g.n_vis) + v] =
+                    make_float2(A1 + fmaf(Cr, A2, Sr * B2) + cr, fmaf(Cr, B2, -Sr * A2) - B1 - ci);
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // ------------------------------------------------------------------ stage 4
 constexpr int kReconStage = kBlock;  // visibilities staged in LDS per step (one per thread)
 
@@ -1058,25 +1375,30 @@ hipError_t launch_to_chunk_order(const GeomDev &g, int32_t n_maps, int64_t n_pix
 }
 
 template <bool F64, int VT>
-static hipError_t launch_visgen_t(const GeomDev &g, const PhaseKnobs &k, int32_t n_maps,
+static hipError_t launch_visgen_t(const GeomDev &g, const PhaseKnobs &k, bool pair, int32_t n_maps,
                                   const float *Ich, float2 *V, int grid, hipStream_t st) {
-    hipLaunchKernelGGL((visgen_kernel<F64, VT>), dim3(grid), dim3(kBlock), 0, st, g, k, n_maps,
-                       Ich, V);
+    if (pair)
+        hipLaunchKernelGGL((visgen_pair_kernel<F64, VT>), dim3(grid), dim3(kBlock), 0, st, g, k,
+                           n_maps, Ich, V);
+    else
+        hipLaunchKernelGGL((visgen_kernel<F64, VT>), dim3(grid), dim3(kBlock), 0, st, g, k,
+                           n_maps, Ich, V);
     return hipGetLastError();
 }
 
-hipError_t launch_visgen(const GeomDev &g, const PhaseKnobs &k, bool f64, int vt, int32_t n_maps,
-                         const float *Ich, float2 *V, int grid, hipStream_t st) {
+hipError_t launch_visgen(const GeomDev &g, const PhaseKnobs &k, bool f64, int vt, bool pair,
+                         int32_t n_maps, const float *Ich, float2 *V, int grid, hipStream_t st) {
+    pair = pair && g.n_half > 0;
     switch (vt) {
         case 8:
-            return f64 ? launch_visgen_t<true, 8>(g, k, n_maps, Ich, V, grid, st)
-                       : launch_visgen_t<false, 8>(g, k, n_maps, Ich, V, grid, st);
+            return f64 ? launch_visgen_t<true, 8>(g, k, pair, n_maps, Ich, V, grid, st)
+                       : launch_visgen_t<false, 8>(g, k, pair, n_maps, Ich, V, grid, st);
         case 32:
-            return f64 ? launch_visgen_t<true, 32>(g, k, n_maps, Ich, V, grid, st)
-                       : launch_visgen_t<false, 32>(g, k, n_maps, Ich, V, grid, st);
+            return f64 ? launch_visgen_t<true, 32>(g, k, pair, n_maps, Ich, V, grid, st)
+                       : launch_visgen_t<false, 32>(g, k, pair, n_maps, Ich, V, grid, st);
         default:
-            return f64 ? launch_visgen_t<true, 16>(g, k, n_maps, Ich, V, grid, st)
-                       : launch_visgen_t<false, 16>(g, k, n_maps, Ich, V, grid, st);
+            return f64 ? launch_visgen_t<true, 16>(g, k, pair, n_maps, Ich, V, grid, st)
+                       : launch_visgen_t<false, 16>(g, k, pair, n_maps, Ich, V, grid, st);
     }
 }
 
@@ -1160,7 +1482,14 @@ int64_t term_ceiling_terms(int32_t iters, int32_t blocks) {
 int visgen_tile_vis() { return 16; }
 int list_pad() { return kListPad; }
 int recon_stage_vis() { return kReconStage; }
-const void *visgen_kernel_ptr(bool f64, int vt) {
+const void *visgen_kernel_ptr(bool f64, int vt, bool pair) {
+    if (pair) {
+        switch (vt) {
+            case 8: return f64 ? (const void *)visgen_pair_kernel<true, 8> : (const void *)visgen_pair_kernel<false, 8>;
+            case 32: return f64 ? (const void *)visgen_pair_kernel<true, 32> : (const void *)visgen_pair_kernel<false, 32>;
+            default: return f64 ? (const void *)visgen_pair_kernel<true, 16> : (const void *)visgen_pair_kernel<false, 16>;
+        }
+    }
     switch (vt) {
         case 8: return f64 ? (const void *)visgen_kernel<true, 8> : (const void *)visgen_kernel<false, 8>;
         case 32: return f64 ? (const void *)visgen_kernel<true, 32> : (const void *)visgen_kernel<false, 32>;

@@ -23,6 +23,9 @@ struct FineDev {
 // Device view of one geometry; passed to kernels by value.
 struct GeomDev {
     int32_t n_sat, n_t, n_base, n_chunk;
+    // > 0: antipodal layout, chunk k + n_half holds the antipodes of chunk k
+    // slot by slot (n_chunk = 2 n_half); 0: no such pairing
+    int32_t n_half;
     int64_t n_vis, n_pad;
     const double4 *pos;    // [n_sat * n_t]  (x, y, z, |r|^2)
     const CapRec *cap;     // [n_sat * n_t]
@@ -59,6 +62,7 @@ struct PhaseKnobs {
     double s;       // phase scale
     int32_t wfac;   // 1: use n (trailing exp(-2 pi i s w) folded in), 0: n - 1
     float t_occ;
+    int32_t pair;   // 1: antipodal pixel pairs share a phase (host-side kernel choice)
 };
 
 // Stage 1: positions, caps, uvw.
@@ -74,8 +78,9 @@ hipError_t launch_to_chunk_order(const GeomDev &g, int32_t n_maps, int64_t n_pix
                                  const float *I, float *Ich, hipStream_t st);
 
 // Stage 3.  Ich: [n_maps][n_pad]; V: [n_maps][n_vis] complex64.
-// vt = visibilities per tile (8, 16 or 32).
-hipError_t launch_visgen(const GeomDev &g, const PhaseKnobs &k, bool f64_phase, int vt,
+// vt = visibilities per tile (8, 16 or 32); pair: one phase serves a pixel and
+// its antipode (used when g.n_half > 0).
+hipError_t launch_visgen(const GeomDev &g, const PhaseKnobs &k, bool f64_phase, int vt, bool pair,
                          int32_t n_maps, const float *Ich, float2 *V, int grid, hipStream_t st);
 
 // Stage 4.  V: [n_maps][n_vis] complex64; w: [n_vis] or null; part: [n_maps][n_seg][n_pad].
@@ -109,7 +114,7 @@ const void *visgen_fine_kernel_ptr(bool f64);
 int visgen_tile_vis();           // default visibilities per vis-gen tile
 int list_pad();                  // list-mode padding granule (entries)
 int recon_stage_vis();           // visibilities staged per recon LDS tile
-const void *visgen_kernel_ptr(bool f64_phase, int vt);
+const void *visgen_kernel_ptr(bool f64_phase, int vt, bool pair);
 const void *recon_kernel_ptr(bool f64_phase);
 const void *shade_kernel_ptr();
 

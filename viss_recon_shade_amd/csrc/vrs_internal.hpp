@@ -11,7 +11,10 @@
 //   ring   int32   [n_pad]           RING index of each chunk-ordered pixel, -1 for padding
 //   chunk  ChunkRec[n_chunk]         bounding cap of each chunk of kChunk pixels
 // Chunk order is HEALPix NESTED order, so a chunk of 256 pixels is a 16 x 16
-// patch of one base face whenever nside >= 16.
+// patch of one base face whenever nside >= 16.  From nside 16 on the second
+// half is the antipodal image of the first (position n/2 + q holds the
+// antipode of position q; base faces 0-5 face faces 6-11), so chunk
+// k + n_chunk/2 is chunk k mirrored through the centre, slot by slot.
 #pragma once
 
 #include <hip/hip_runtime.h>
