@@ -118,11 +118,13 @@ typedef struct vrs_knobs {
     float t_occ;           /* intensity of shaded pixels in vis-gen (vissGen10M_shade.cu:132: 240) */
     double kappa;          /* recon normalisation; 0 selects 1/N_vis (vissGen10M_shade.cu:156,173) */
     int32_t precision;     /* vrs_precision */
-    int32_t antipodal;     /* 1 (default): on a HEALPix sky of Nside >= 16 one phase and one
-                            * sin/cos serve a pixel and its antipode, whose phase is
-                            * 2 wo - phi (E(-p) = conj(E(p)) exp(-4 pi i wo)); 0: every
-                            * (visibility, pixel) term forms its own phase.  Shading is
-                            * decided per pixel either way. */
+    int32_t antipodal;     /* 1 (default): on a HEALPix sky of Nside >= 16 the vis-gen
+                            * kernels let one phase and one sin/cos serve a pixel and its
+                            * antipode, whose phase is 2 wo - phi (E(-p) = conj(E(p))
+                            * exp(-4 pi i wo)); 0: every (visibility, pixel) term forms its
+                            * own phase.  Shading is decided per pixel either way; recon
+                            * (most pixel pairs have one pixel wholly shaded) always runs
+                            * per pixel. */
 } vrs_knobs;
 
 typedef struct vrs_shade_stats {
@@ -218,10 +220,11 @@ vrs_status vrs_adjoint(vrs_ctx *ctx, const vrs_geom *g, const vrs_knobs *k, cons
  * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
  * arguments and results (identical bits) as vrs_recon, but V, F and w (when
  * non-NULL) must be device memory and the precision VRS_PLAIN_F32 or
- * VRS_MIXED_F64PHASE.  Its partial images live in one fixed scratch array per
- * device of VRS_NOALLOC_SCRATCH_BYTES, held until the process ends; a call
- * needing more (segments x n_maps x padded pixels x 4 bytes, ~6.3 MB per map at
- * Nside 128) fails with VRS_E_ARG.  Every call on a device shares that array,
+ * VRS_MIXED_F64PHASE.  Its partial images and its kernel's work counter live
+ * in one fixed scratch array per device of VRS_NOALLOC_SCRATCH_BYTES, held
+ * until the process ends; partial images need segments x n_maps x padded
+ * pixels x 4 bytes (~6.3 MB per map at Nside 128) and may use all but the last
+ * 256 bytes, a call needing more fails with VRS_E_ARG.  Every call on a device shares that array,
  * so calls that overlap in time on one device (other streams or contexts) race;
  * each call writes all of the scratch it reads, nothing carries over. */
 #define VRS_NOALLOC_SCRATCH_BYTES (64u << 20)

@@ -32,13 +32,15 @@ constexpr double kC = 299792458.0;
 }  // namespace
 
 // Scratch of vrs_recon_noalloc: the partial images of its (map, visibility
-// segment) slots.  Each device that loads the library holds its own copy at its
+// segment) slots, and in its last kNoallocQueueBytes the recon kernel's work
+// counter.  Each device that loads the library holds its own copy at its
 // own address until the process ends.  Host code never uses this name as an
 // address: it takes the device address once per device with
 // hipGetSymbolAddress (noalloc_scratch_addr) and passes that pointer on.
 static __device__ __align__(256) float scratch[VRS_NOALLOC_SCRATCH_BYTES / sizeof(float)];
 
 namespace {
+constexpr size_t kNoallocQueueBytes = 256;
 std::mutex g_scratch_mu;
 std::map<int, float *> g_scratch_addr;  // device -> address of its copy of `scratch`
 
@@ -356,7 +358,8 @@ vrs_status vrs_create(int device, vrs_ctx **out) {
         return fail(nullptr, VRS_E_HIP, "hipStreamCreate failed");
     }
     c->own_stream = true;
-    if (hipMalloc(&c->d_tot, 4 * sizeof(unsigned long long)) != hipSuccess) {
+    // [0, 4): shade statistics; [4]: the recon work counter (context stream)
+    if (hipMalloc(&c->d_tot, 8 * sizeof(unsigned long long)) != hipSuccess) {
         (void)hipStreamDestroy(c->stream);
         delete c;
         return fail(nullptr, VRS_E_OOM, "hipMalloc of counters failed");
@@ -1088,7 +1091,7 @@ static vrs_status recon_common(vrs_ctx *ctx, const vrs_geom *g, const vrs_knobs 
         Timer tm(ctx, timer_name);
         VRS_HIP(ctx, launch_recon(gd, phase_knobs(k), f64, n_maps, (const float2 *)sV.dev,
                                   (const float *)sw.dev, (float)kappa, (int32_t)n_seg,
-                                  (float *)part, grid, ctx->stream));
+                                  (float *)part, ctx->d_tot + 4, grid, ctx->stream));
     }
     VRS_HIP(ctx, launch_recon_finalize(gd, n_maps, (int32_t)n_seg, g->n_pix, (const float *)part,
                                        (float *)sF.dev, ctx->stream));
@@ -1134,19 +1137,21 @@ vrs_status vrs_recon_noalloc(vrs_ctx *ctx, const vrs_geom *g, const vrs_knobs *k
         return fail(ctx, VRS_E_ARG, "vrs_recon_noalloc takes device arrays only (V, w, F)");
     const int64_t n_seg = recon_segments(g);
     const uint64_t need = (uint64_t)n_seg * (uint64_t)n_maps * (uint64_t)g->n_pad * sizeof(float);
-    if (need > (uint64_t)VRS_NOALLOC_SCRATCH_BYTES)
+    if (need > (uint64_t)(VRS_NOALLOC_SCRATCH_BYTES - kNoallocQueueBytes))
         return fail(ctx, VRS_E_ARG,
                     "recon of %d map(s) over %lld padded pixels needs %llu bytes of partial "
                     "images; vrs_recon_noalloc holds %llu (use vrs_recon or fewer maps)",
                     n_maps, (long long)g->n_pad, (unsigned long long)need,
-                    (unsigned long long)VRS_NOALLOC_SCRATCH_BYTES);
+                    (unsigned long long)(VRS_NOALLOC_SCRATCH_BYTES - kNoallocQueueBytes));
     const double kappa = k->kappa > 0 ? k->kappa : 1.0 / (double)g->n_vis;
     const GeomDev gd = g->dev();
     const bool f64 = k->precision == VRS_MIXED_F64PHASE;
     const int grid = persistent_grid(ctx, recon_kernel_ptr(f64), (int64_t)n_maps * g->n_chunk * n_seg);
     // every (slot, chunk) item writes its whole partial-image block: nothing to clear first
+    unsigned long long *queue = (unsigned long long *)((char *)ctx->noalloc_scratch +
+                                                       VRS_NOALLOC_SCRATCH_BYTES - kNoallocQueueBytes);
     VRS_HIP(ctx, launch_recon(gd, phase_knobs(k), f64, n_maps, (const float2 *)V, w, (float)kappa,
-                              (int32_t)n_seg, ctx->noalloc_scratch, grid, ctx->stream));
+                              (int32_t)n_seg, ctx->noalloc_scratch, queue, grid, ctx->stream));
     VRS_HIP(ctx, launch_recon_finalize(gd, n_maps, (int32_t)n_seg, g->n_pix,
                                        ctx->noalloc_scratch, F, ctx->stream));
     return VRS_OK;
@@ -1173,6 +1178,7 @@ struct vrs_batch {
     int4 *d_slots = nullptr;
     int32_t *d_map_slot0 = nullptr;
     float *d_kappa = nullptr;
+    unsigned long long *d_queue = nullptr;  // recon work counter (batch stream)
     float *d_I = nullptr, *d_Ich = nullptr, *d_F = nullptr, *d_part = nullptr;
     float2 *d_V = nullptr;
     hipStream_t stream = nullptr;
@@ -1267,7 +1273,7 @@ vrs_status batch_launch(vrs_batch *b, const PhaseKnobs &pk, bool skip_recon = fa
     }
     if (!skip_recon) {
         const int grid = persistent_grid(ctx, recon_kernel_ptr(f64), b->n_slots * b->g->n_chunk);
-        VRS_HIP(ctx, launch_recon(gd, pk, f64, b->n_ch, b->d_V, nullptr, 0.0f, 1, b->d_part, grid,
+        VRS_HIP(ctx, launch_recon(gd, pk, f64, b->n_ch, b->d_V, nullptr, 0.0f, 1, b->d_part, b->d_queue, grid,
                                   b->stream));
         VRS_HIP(ctx, launch_recon_finalize(gd, b->n_ch, 1, b->g->n_pix, b->d_part, b->d_F, b->stream));
     }
@@ -1325,7 +1331,7 @@ void vrs_batch_destroy(vrs_batch *b) {
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     void *ptrs[] = {b->d_vis_sat, b->d_tile_map, b->d_inv_lambda, b->d_slots, b->d_map_slot0,
-                    b->d_kappa, b->d_I, b->d_Ich, b->d_F, b->d_part, b->d_V};
+                    b->d_kappa, b->d_I, b->d_Ich, b->d_F, b->d_part, b->d_V, b->d_queue};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     hipEvent_t evs[] = {b->ev_in, b->ev_out, b->ev_t0, b->ev_t1};
@@ -1418,6 +1424,7 @@ vrs_status vrs_batch_create(vrs_ctx *ctx, const vrs_geom *g, const vrs_batch_des
     VRS_BHIP(hipMalloc(&b->d_inv_lambda, sizeof(double) * b->n_ch));
     VRS_BHIP(hipMalloc(&b->d_map_slot0, sizeof(int32_t) * (b->n_ch + 1)));
     VRS_BHIP(hipMalloc(&b->d_kappa, sizeof(float) * b->n_ch));
+    VRS_BHIP(hipMalloc(&b->d_queue, sizeof(unsigned long long)));
     VRS_BHIP(hipMalloc(&b->d_I, sizeof(float) * b->n_ch * g->n_pix));
     VRS_BHIP(hipMalloc(&b->d_Ich, sizeof(float) * b->n_ch * g->n_pad));
     VRS_BHIP(hipMalloc(&b->d_F, sizeof(float) * b->n_ch * g->n_pix));

@@ -1087,11 +1087,27 @@ __device__ __forceinline__ void recon_straddle(const GeomDev &g, int c, int lane
         recon_straddle_branch<F64, TWO>(g, c, lane, e, o, l, m, n, sx, sy, sz, acc);
 }
 
+// Work queue of the recon kernels: blocks take (slot, chunk) items in index
+// order from a device counter (zeroed by the launcher) instead of a fixed
+// stride, so that blocks which drew cheap items (chunks most visibilities
+// shade wholly) take more of them and the tail of the launch stays short
+// whatever the chunk order.  Every block leaves once the counter passes the
+// item count.
+__device__ __forceinline__ int64_t next_item(unsigned long long *queue, int64_t &slot_s) {
+    __syncthreads();  // every wave has read the previous item
+    if (threadIdx.x == 0) slot_s = (int64_t)atomicAdd(queue, 1ull);
+    __syncthreads();
+    return slot_s;
+}
+
+__global__ void zero_queue_kernel(unsigned long long *queue) { *queue = 0ull; }
+
 template <bool F64>
 __device__ __forceinline__ void recon_body(const GeomDev &g, const PhaseKnobs &k, int32_t n_maps,
                                            const float2 *__restrict__ V,
                                            const float *__restrict__ wgt, float kappa,
-                                           int32_t n_seg, float *__restrict__ part) {
+                                           int32_t n_seg, float *__restrict__ part,
+                                           unsigned long long *__restrict__ queue) {
     using T = typename PhaseT<F64>::T;
     __shared__ RecHot<F64> hot[kReconStage];
     __shared__ RecCold cold[kReconStage];
@@ -1104,7 +1120,9 @@ __device__ __forceinline__ void recon_body(const GeomDev &g, const PhaseKnobs &k
     const int64_t n_slots = list ? g.n_slots : (int64_t)n_maps * n_seg;
     const int64_t n_items = n_slots * g.n_chunk;
 
-    for (int64_t item = blockIdx.x; item < n_items; item += gridDim.x) {
+    __shared__ int64_t s_item;
+    for (int64_t item = next_item(queue, s_item); item < n_items;
+         item = next_item(queue, s_item)) {
         // the slot varies fastest: with n_slots a multiple of 8 each XCD keeps to its own slots
         const int64_t slot = item % n_slots;
         const int c = (int)(item / n_slots);
@@ -1244,8 +1262,8 @@ __global__ void __launch_bounds__(kBlock) recon_kernel(GeomDev g, PhaseKnobs k, 
                                                        const float2 *__restrict__ V,
                                                        const float *__restrict__ wgt,
                                                        float kappa, int32_t n_seg,
-                                                       float *__restrict__ part) {
-    recon_body<F64>(g, k, n_maps, V, wgt, kappa, n_seg, part);
+                                                       float *__restrict__ part, unsigned long long *__restrict__ queue) {
+    recon_body<F64>(g, k, n_maps, V, wgt, kappa, n_seg, part, queue);
 }
 template <>
 __global__ void __launch_bounds__(kBlock, 5) recon_kernel<true>(GeomDev g, PhaseKnobs k,
@@ -1253,8 +1271,8 @@ __global__ void __launch_bounds__(kBlock, 5) recon_kernel<true>(GeomDev g, Phase
                                                                 const float2 *__restrict__ V,
                                                                 const float *__restrict__ wgt,
                                                                 float kappa, int32_t n_seg,
-                                                                float *__restrict__ part) {
-    recon_body<true>(g, k, n_maps, V, wgt, kappa, n_seg, part);
+                                                                float *__restrict__ part, unsigned long long *__restrict__ queue) {
+    recon_body<true>(g, k, n_maps, V, wgt, kappa, n_seg, part, queue);
 }
 
 // F[map][ring[p]] = sum over the map's slots of part[slot][p].
@@ -1404,13 +1422,16 @@ hipError_t launch_visgen(const GeomDev &g, const PhaseKnobs &k, bool f64, int vt
 
 hipError_t launch_recon(const GeomDev &g, const PhaseKnobs &k, bool f64_phase, int32_t n_maps,
                         const float2 *V, const float *w, float kappa, int32_t n_seg, float *part,
-                        int grid, hipStream_t st) {
+                        unsigned long long *queue, int grid, hipStream_t st) {
+    // the queue counter is cleared by a kernel, not a memset (graph-safe)
+    hipLaunchKernelGGL(zero_queue_kernel, dim3(1), dim3(1), 0, st, queue);
+    if (hipError_t e = hipGetLastError()) return e;
     if (f64_phase)
         hipLaunchKernelGGL(recon_kernel<true>, dim3(grid), dim3(kBlock), 0, st, g, k, n_maps, V,
-                           w, kappa, n_seg, part);
+                           w, kappa, n_seg, part, queue);
     else
         hipLaunchKernelGGL(recon_kernel<false>, dim3(grid), dim3(kBlock), 0, st, g, k, n_maps, V,
-                           w, kappa, n_seg, part);
+                           w, kappa, n_seg, part, queue);
     return hipGetLastError();
 }
 

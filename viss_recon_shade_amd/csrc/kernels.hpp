@@ -84,9 +84,10 @@ hipError_t launch_visgen(const GeomDev &g, const PhaseKnobs &k, bool f64_phase, 
                          int32_t n_maps, const float *Ich, float2 *V, int grid, hipStream_t st);
 
 // Stage 4.  V: [n_maps][n_vis] complex64; w: [n_vis] or null; part: [n_maps][n_seg][n_pad].
+// queue: one device counter, cleared by the launch (a kernel, no memset node).
 hipError_t launch_recon(const GeomDev &g, const PhaseKnobs &k, bool f64_phase, int32_t n_maps,
                         const float2 *V, const float *w, float kappa, int32_t n_seg,
-                        float *part, int grid, hipStream_t st);
+                        float *part, unsigned long long *queue, int grid, hipStream_t st);
 // F[map][ring[p]] = sum_seg part[map][seg][p].
 hipError_t launch_recon_finalize(const GeomDev &g, int32_t n_maps, int32_t n_seg, int64_t n_pix,
                                  const float *part, float *F, hipStream_t st);
