@@ -106,6 +106,7 @@ struct vrs_geom {
     double4 *d_pix64 = nullptr;
     int32_t *d_ring = nullptr;
     ChunkRec *d_chunk = nullptr;
+    ChunkRec *d_sub = nullptr;  // bounding caps of the kSubChunk-pixel groups
     // fine-cell sky (vrs_geom_fine_sky); all null until one is attached
     int64_t n_cells = 0;
     float4 *d_cell32 = nullptr;
@@ -128,6 +129,7 @@ struct vrs_geom {
         g.n_vis = n_vis; g.n_pad = n_pad;
         g.pos = d_pos; g.cap = d_cap; g.uvw64 = d_uvw64; g.vis_cap = d_viscap; g.pairs = d_pairs;
         g.pix32 = d_pix32; g.pix64 = d_pix64; g.ring = d_ring; g.chunk = d_chunk;
+        g.sub = d_sub;
         const double e = kNearLimbRel * r_moon;
         g.limb.R2 = r_moon * r_moon;
         g.limb.lo2 = (r_moon - e) * (r_moon - e);
@@ -544,7 +546,7 @@ void vrs_geom_destroy(vrs_geom *g) {
     DeviceGuard dg(g->device);
     (void)hipDeviceSynchronize();
     void *ptrs[] = {g->d_orbit, g->d_tidx, g->d_pos,  g->d_cap,  g->d_uvw64, g->d_viscap,
-                    g->d_pairs, g->d_pix32, g->d_pix64, g->d_ring, g->d_chunk};
+                    g->d_pairs, g->d_pix32, g->d_pix64, g->d_ring, g->d_chunk, g->d_sub};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     g->free_fine();
@@ -601,26 +603,28 @@ static vrs_status sky_build(vrs_ctx *ctx, vrs_geom *g, int32_t nside, int64_t n_
         pix32[q] = make_float4((float)v[0], (float)v[1], (float)v[2], 0.0f);
         ring[q] = (int32_t)r;
     }
-    std::vector<ChunkRec> chunks(g->n_chunk);
-    for (int32_t c = 0; c < g->n_chunk; ++c) {
+    // Bounding cap of the pixels [q0, q0 + len): every pixel centre lies within
+    // angle rho of (gx, gy, gz); rho is padded, and wide or partly padded groups
+    // are flagged to be tested pixel by pixel.
+    auto bound = [&](int64_t q0, int len) {
+        ChunkRec ch;
         double sx = 0, sy = 0, sz = 0;
         int nv = 0;
         uint32_t h = 0;
-        for (int i = 0; i < kChunk; ++i) {
-            const int64_t q = (int64_t)c * kChunk + i;
+        for (int i = 0; i < len; ++i) {
+            const int64_t q = q0 + i;
             if (ring[q] < 0) continue;
             sx += pix64[q].x; sy += pix64[q].y; sz += pix64[q].z;
             ++nv;
             h ^= pix_hash_u64((uint64_t)ring[q]);
         }
         const double nrm = std::sqrt(sx * sx + sy * sy + sz * sz);
-        ChunkRec &ch = chunks[c];
         double rho = M_PI;
         if (nrm > 1e-6 * nv) {
             sx /= nrm; sy /= nrm; sz /= nrm;
             double maxchord = 0;
-            for (int i = 0; i < kChunk; ++i) {
-                const int64_t q = (int64_t)c * kChunk + i;
+            for (int i = 0; i < len; ++i) {
+                const int64_t q = q0 + i;
                 if (ring[q] < 0) continue;
                 const double dx = pix64[q].x - sx, dy = pix64[q].y - sy, dz = pix64[q].z - sz;
                 maxchord = std::max(maxchord, std::sqrt(dx * dx + dy * dy + dz * dz));
@@ -633,16 +637,23 @@ static vrs_status sky_build(vrs_ctx *ctx, vrs_geom *g, int32_t nside, int64_t n_
         ch.gx = (float)sx; ch.gy = (float)sy; ch.gz = (float)sz;
         ch.crho = (float)std::cos(rho);
         ch.srho = (float)std::sin(rho);
-        ch.flags = (rho > 0.5 || nv < kChunk) ? kChunkAlwaysStraddle : 0;
+        ch.flags = (rho > 0.5 || nv < len) ? kChunkAlwaysStraddle : 0;
         ch.n_valid = nv;
         ch.hash_all = h;
-    }
+        return ch;
+    };
+    std::vector<ChunkRec> chunks(g->n_chunk), subs(g->n_pad / kSubChunk);
+    for (int32_t c = 0; c < g->n_chunk; ++c) chunks[c] = bound((int64_t)c * kChunk, kChunk);
+    for (size_t i = 0; i < subs.size(); ++i) subs[i] = bound((int64_t)i * kSubChunk, kSubChunk);
 
     hipStream_t st = ctx->stream;
     VRS_HIP(ctx, hipMalloc(&g->d_pix32, sizeof(float4) * g->n_pad));
     VRS_HIP(ctx, hipMalloc(&g->d_pix64, sizeof(double4) * g->n_pad));
     VRS_HIP(ctx, hipMalloc(&g->d_ring, sizeof(int32_t) * g->n_pad));
     VRS_HIP(ctx, hipMalloc(&g->d_chunk, sizeof(ChunkRec) * g->n_chunk));
+    VRS_HIP(ctx, hipMalloc(&g->d_sub, sizeof(ChunkRec) * subs.size()));
+    VRS_HIP(ctx, hipMemcpyAsync(g->d_sub, subs.data(), sizeof(ChunkRec) * subs.size(),
+                                hipMemcpyHostToDevice, st));
     VRS_HIP(ctx, hipMemcpyAsync(g->d_pix32, pix32.data(), sizeof(float4) * g->n_pad,
                                 hipMemcpyHostToDevice, st));
     VRS_HIP(ctx, hipMemcpyAsync(g->d_pix64, pix64.data(), sizeof(double4) * g->n_pad,

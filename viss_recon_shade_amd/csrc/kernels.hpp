@@ -36,6 +36,7 @@ struct GeomDev {
     const double4 *pix64;  // [n_pad]
     const int32_t *ring;   // [n_pad]
     const ChunkRec *chunk; // [n_chunk]
+    const ChunkRec *sub;   // [n_pad / kSubChunk] bounding caps of 64-pixel groups (8 x 8 blocks)
     LimbConsts limb;
     // Ragged batch ("list mode"); all null for a single geometry.
     const int2 *vis_sat;       // [n_vis] (sat_a * n_t + t, sat_b * n_t + t); x < 0: padding

@@ -45,6 +45,7 @@ constexpr int kWavesR = 16;       // recon block: two chunks, one block per CU, 
 constexpr int kThreadsR = kWavesR * 64;
 
 static_assert(kChunk % kStage == 0, "stages must tile chunks");
+static_assert(kStage == kSubChunk, "vis-gen stages are classified by their sub-chunk record");
 static_assert(kWavesR * kNT == 2 * kChunk, "a recon block covers two chunks");
 
 __device__ __forceinline__ floatx16 mfma(half8 a, half8 b, floatx16 c) {
@@ -351,8 +352,9 @@ __global__ void __launch_bounds__(kThreads) mfma_visgen_kernel(
 #pragma unroll
             for (int c = 0; c < 2; ++c) acc[a][c] = floatx4{};
         float gre = 0.0f, gim = 0.0f;  // sum of (cos, sin) over shaded pixels
-        int cls = kOut;
-        bool mixed = false;   // some lane's pair shades part of the chunk, or lanes differ
+        int cls = kOut, ccls = kOut;  // class of this lane's pair: stage, chunk
+        bool cmixed = false;  // the wave's chunk classes mix or some lane straddles
+        bool mixed = false;   // some lane's pair shades part of the stage, or lanes differ
         bool all_in = false;  // every visibility of the wave shades the whole chunk: E' = 0
 
         const _Float16 *Ag = A16 + grp * 2 * kMaps * g.n_pad;
@@ -364,12 +366,21 @@ __global__ void __launch_bounds__(kThreads) mfma_visgen_kernel(
         for (int64_t st = 0; st < n_stages; ++st) {
             const int buf = (int)(st & 1);
             const int64_t p0 = st * kStage;
-            // chunk record first, so that waiting for it does not wait for the prefetch
-            const bool new_chunk = (p0 % kChunk) == 0;
-            ChunkRec ch;
-            if (new_chunk) ch = g.chunk[p0 / kChunk];
-            if (new_chunk) {
-                cls = classify_m(ch, sCap[vc][0], sCap[vc][1]);
+            // Classes per chunk; where the wave's chunk is mixed, lanes whose chunk
+            // straddles re-classify each stage against the 64-pixel group's own
+            // bounding cap (records first, so that waiting for them does not wait
+            // for the prefetch).
+            if ((p0 % kChunk) == 0) {
+                const ChunkRec ch = g.chunk[p0 / kChunk];
+                ccls = classify_m(ch, sCap[vc][0], sCap[vc][1]);
+                cmixed = __ballot(ccls != kOut) != 0 && __ballot(ccls == kIn) != ~0ull;
+                cls = ccls;
+                all_in = __ballot(ccls == kIn) == ~0ull;
+                mixed = cmixed;
+            }
+            if (cmixed) {  // wave-uniform
+                cls = ccls;
+                if (ccls == kStraddle) cls = classify_m(g.sub[p0 / kSubChunk], sCap[vc][0], sCap[vc][1]);
                 all_in = __ballot(cls == kIn) == ~0ull;
                 mixed = !all_in && __ballot(cls != kOut) != 0;
             }

@@ -10,6 +10,7 @@
 //   pix64  double4 [n_pad]           (l, m, n, 0) in chunk order, FP64
 //   ring   int32   [n_pad]           RING index of each chunk-ordered pixel, -1 for padding
 //   chunk  ChunkRec[n_chunk]         bounding cap of each chunk of kChunk pixels
+//   sub    ChunkRec[n_pad / 64]      the same for each group of kSubChunk pixels
 // Chunk order is HEALPix NESTED order, so a chunk of 256 pixels is a 16 x 16
 // patch of one base face whenever nside >= 16.  From nside 16 on the second
 // half is the antipodal image of the first (position n/2 + q holds the
@@ -27,6 +28,7 @@ constexpr int kPixPerLane = 4;
 constexpr int kChunk = kWave * kPixPerLane;  // pixels per chunk
 constexpr int kWavesPerBlock = 4;
 constexpr int kBlock = kWave * kWavesPerBlock;
+constexpr int kSubChunk = 64;                // pixels of a sub-chunk (one wave's lane group)
 
 // Shade cap of one satellite at one step: pixel s is shaded iff
 // dot(s, dir) >= c where dir = -r/|r| and c = sqrt(1 - R^2/|r|^2).
