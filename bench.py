@@ -7,6 +7,9 @@ visibilities), HEALPix Nside 128 sky (196 608 pixels, log-normal(0, 1.5) plus
 the device) + vis-gen + recon, inputs resident in HBM.
 value = (vis-gen terms + recon terms) / step time, terms = N_vis x N_pix.
 
+The plain step pairs each pixel with its antipode (one phase and sin/cos serve
+both terms, vrs_knobs.antipodal); the same step with every term on its own is
+reported beside it (plain_direct).
 Also reported: the FP64-phase path on the same workload, the relative error
 of both against the FP64 oracle on sampled outputs, per-kernel times from HIP
 events on the context stream, the dominant kernel's roofline fraction, and the
@@ -31,8 +34,10 @@ FLOPS_PER_TERM = 10               # 3 FMA phase + 2 FMA accumulate
 N_CU, N_XCD = 256, 8              # SQ counters sum over CUs, GRBM_GUI_ACTIVE over XCDs
 
 
-def visgen_name(f64, vt):
-    """Kernel name of a vis-gen tile configuration."""
+def visgen_name(f64, vt, pair=False):
+    """Kernel name of a vis-gen tile configuration (pair: antipodal pixel pairs)."""
+    if pair:
+        return f"visgen_pair_kernel<{'true' if f64 else 'false'}, {vt}>"
     return f"visgen_kernel<{'true' if f64 else 'false'}, {vt}, false>"
 
 
@@ -441,6 +446,11 @@ def main():
     ms_mixed = timed_steps(torch, step_with(km), max(2, args.steps // 2), 1)
     kt_mixed = kernel_avg_ms(ctx, step_with(km), ["visgen", "recon"], 2)
     Vm, Fm = state["V"].cpu().numpy(), state["F"].cpu().numpy()
+    # the same plain step with every term forming its own phase (no antipodal pairs)
+    kd = vrs.knobs(precision="plain", antipodal=0)
+    tile_direct = g.autotune(It, kd)
+    ms_direct = timed_steps(torch, step_with(kd), max(2, args.steps // 2), 1)
+    kt_direct = kernel_avg_ms(ctx, step_with(kd), ["visgen", "recon"], 2)
 
     value = terms / (ms_plain * 1e-3) / 1e9
     # the same step with host (numpy) buffers at the boundary: PCIe staging included
@@ -457,7 +467,9 @@ def main():
     issue = None
     dom = max(("visgen", "recon"), key=lambda n: kt_plain[n])
     dom_ms = kt_plain[dom]
-    kname = {"visgen": visgen_name(False, tile_plain[0]), "recon": "recon_kernel<false>"}[dom]
+    paired = wl.nside >= 16   # HEALPix skies from Nside 16 on take the antipodal layout
+    kname = {"visgen": visgen_name(False, tile_plain[0], paired),
+             "recon": "recon_kernel<false>"}[dom]
     tfile = os.path.join(ROOT, "profiles", "traffic_c2.json")
     if wl.name == "c2" and os.path.exists(tfile):
         kern = json.load(open(tfile))["kernels"]
@@ -524,6 +536,13 @@ def main():
         "mixed_f64phase": {"value": round(terms / (ms_mixed * 1e-3) / 1e9, 2),
                            "ms_per_step": round(ms_mixed, 3),
                            "kernels_ms": {k: round(v, 3) for k, v in kt_mixed.items()}},
+        "plain_direct": {"value": round(terms / (ms_direct * 1e-3) / 1e9, 2),
+                         "ms_per_step": round(ms_direct, 3),
+                         "kernels_ms": {k: round(v, 3) for k, v in kt_direct.items()},
+                         "visgen_tile": tile_direct[0],
+                         "note": "plain FP32 with antipodal=0: every (visibility, pixel) term "
+                                 "forms its own phase and sin/cos (the headline pairs a pixel "
+                                 "with its antipode, one sin/cos for both)"},
     }
     if not args.no_mfma:
         out["mfma_split16"] = mfma_legs(torch, ctx, vrs, W, wl, g, pos, uvw, lmn, args)

@@ -1021,8 +1021,12 @@ vrs_status vrs_autotune_visgen(vrs_ctx *ctx, const vrs_geom *g, const vrs_knobs 
     double best = 1e300;
     int best_vt = visgen_tile_vis();
     vrs_status st = VRS_OK;
+    // the antipodal-pair kernel holds four sums per visibility: 32-visibility
+    // tiles spill, so it is tuned over 8 and 16
+    const bool pair = k->antipodal && g->n_half > 0;
+    const std::vector<int> tiles = pair ? std::vector<int>{8, 16} : std::vector<int>{8, 16, 32};
     for (int rep = 0; rep < 2 && st == VRS_OK; ++rep)     // first round warms up
-        for (int vt : {8, 16, 32}) {
+        for (int vt : tiles) {
             ctx->tuned_vt[key] = vt;
             if (hipEventRecord(e0, ctx->stream) != hipSuccess) { st = VRS_E_HIP; break; }
             st = vrs_visgen(ctx, g, k, n_maps, I, (float *)V);
@@ -1468,7 +1472,9 @@ vrs_status vrs_batch_create(vrs_ctx *ctx, const vrs_geom *g, const vrs_batch_des
         double best = 1e300, ms = 0;
         int best_vt = 16;
         if (vrs_status s = batch_time(b, pk, true, &ms)) return cleanup(s);  // warm-up
-        for (int vt : {8, 16, 32}) {
+        const std::vector<int> tiles =
+            g->n_half > 0 ? std::vector<int>{8, 16} : std::vector<int>{8, 16, 32};
+        for (int vt : tiles) {
             b->vt = vt;
             if (vrs_status s = batch_time(b, pk, true, &ms)) return cleanup(s);
             if (ms < best) { best = ms; best_vt = vt; }

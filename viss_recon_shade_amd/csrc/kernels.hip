@@ -1395,9 +1395,9 @@ hipError_t launch_to_chunk_order(const GeomDev &g, int32_t n_maps, int64_t n_pix
 template <bool F64, int VT>
 static hipError_t launch_visgen_t(const GeomDev &g, const PhaseKnobs &k, bool pair, int32_t n_maps,
                                   const float *Ich, float2 *V, int grid, hipStream_t st) {
-    if (pair)
-        hipLaunchKernelGGL((visgen_pair_kernel<F64, VT>), dim3(grid), dim3(kBlock), 0, st, g, k,
-                           n_maps, Ich, V);
+    if (pair)  // tiles of 8 or 16 (see visgen_kernel_ptr)
+        hipLaunchKernelGGL((visgen_pair_kernel<F64, VT == 8 ? 8 : 16>), dim3(grid), dim3(kBlock),
+                           0, st, g, k, n_maps, Ich, V);
     else
         hipLaunchKernelGGL((visgen_kernel<F64, VT>), dim3(grid), dim3(kBlock), 0, st, g, k,
                            n_maps, Ich, V);
@@ -1504,12 +1504,9 @@ int visgen_tile_vis() { return 16; }
 int list_pad() { return kListPad; }
 int recon_stage_vis() { return kReconStage; }
 const void *visgen_kernel_ptr(bool f64, int vt, bool pair) {
-    if (pair) {
-        switch (vt) {
-            case 8: return f64 ? (const void *)visgen_pair_kernel<true, 8> : (const void *)visgen_pair_kernel<false, 8>;
-            case 32: return f64 ? (const void *)visgen_pair_kernel<true, 32> : (const void *)visgen_pair_kernel<false, 32>;
-            default: return f64 ? (const void *)visgen_pair_kernel<true, 16> : (const void *)visgen_pair_kernel<false, 16>;
-        }
+    if (pair) {  // tiles of 8 or 16 (32 would spill its four sums per visibility)
+        if (vt == 8) return f64 ? (const void *)visgen_pair_kernel<true, 8> : (const void *)visgen_pair_kernel<false, 8>;
+        return f64 ? (const void *)visgen_pair_kernel<true, 16> : (const void *)visgen_pair_kernel<false, 16>;
     }
     switch (vt) {
         case 8: return f64 ? (const void *)visgen_kernel<true, 8> : (const void *)visgen_kernel<false, 8>;
