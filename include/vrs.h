@@ -125,6 +125,13 @@ typedef struct vrs_knobs {
                             * own phase.  Shading is decided per pixel either way; recon
                             * (most pixel pairs have one pixel wholly shaded) always runs
                             * per pixel. */
+    int32_t sat_factors;   /* VRS_MFMA_SPLIT16 only.  1 (default): on a geometry built from
+                            * orbits whose u v w are satellite position differences (no
+                            * VRS_UVW_CEIL, at most 16 satellites) the matrix-core kernels form
+                            * each pair's phase factor as E_b conj(E_a) from per-satellite
+                            * factors (one FP64 phase and sin/cos per satellite and step,
+                            * each satellite's own occultation zeroing its factor); 0: every
+                            * (visibility, pixel) entry forms its own FP64 phase and sin/cos. */
 } vrs_knobs;
 
 typedef struct vrs_shade_stats {

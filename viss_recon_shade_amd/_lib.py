@@ -57,7 +57,8 @@ class GeomInfo(C.Structure):
 
 class Knobs(C.Structure):
     _fields_ = [("phase_scale", C.c_double), ("wfac", C.c_int32), ("t_occ", C.c_float),
-                ("kappa", C.c_double), ("precision", C.c_int32), ("antipodal", C.c_int32)]
+                ("kappa", C.c_double), ("precision", C.c_int32), ("antipodal", C.c_int32),
+                ("sat_factors", C.c_int32)]
 
 
 class BatchDesc(C.Structure):

@@ -284,6 +284,8 @@ vrs_status check_knobs(vrs_ctx *ctx, const vrs_knobs *k) {
     if (k->precision < VRS_PLAIN_F32 || k->precision > VRS_MFMA_SPLIT16)
         return fail(ctx, VRS_E_ARG, "unknown precision %d", k->precision);
     if (k->antipodal != 0 && k->antipodal != 1) return fail(ctx, VRS_E_ARG, "antipodal must be 0 or 1");
+    if (k->sat_factors != 0 && k->sat_factors != 1)
+        return fail(ctx, VRS_E_ARG, "sat_factors must be 0 or 1");
     return VRS_OK;
 }
 
@@ -313,6 +315,7 @@ void vrs_knobs_default(vrs_knobs *k) {
     k->kappa = 0.0;
     k->precision = VRS_PLAIN_F32;
     k->antipodal = 1;
+    k->sat_factors = 1;
 }
 
 void vrs_knobs_reference(vrs_knobs *k) {
@@ -323,6 +326,7 @@ void vrs_knobs_reference(vrs_knobs *k) {
     k->kappa = 0.0;
     k->precision = VRS_PLAIN_F32;
     k->antipodal = 1;
+    k->sat_factors = 1;
 }
 
 int vrs_device_count(void) {
@@ -953,6 +957,13 @@ static vrs_status check_geom(vrs_ctx *ctx, const vrs_geom *g) {
     return VRS_OK;
 }
 
+// Steps per block of the per-satellite-factor matrix-core vis-gen, 0 where it
+// does not apply: u v w must be the (unrounded) satellite position differences.
+static int sat_factor_tb(const vrs_geom *g, const vrs_knobs *k) {
+    if (!k->sat_factors || g->from_uvw || (g->uvw_flags & VRS_UVW_CEIL)) return 0;
+    return mfma_sat_tb(g->n_sat, g->n_base);
+}
+
 vrs_status vrs_visgen(vrs_ctx *ctx, const vrs_geom *g, const vrs_knobs *k, int32_t n_maps,
                       const float *I, float *V) {
     if (vrs_status s = check_ctx(ctx)) return s;
@@ -970,6 +981,19 @@ vrs_status vrs_visgen(vrs_ctx *ctx, const vrs_geom *g, const vrs_knobs *k, int32
         if (vrs_status s = ws_get(ctx, 0, mfma_a16_bytes_visgen(n_maps, g->n_pad), &a16)) return s;
         if (vrs_status s = ws_get(ctx, 3, 8 * (size_t)n_maps + 256, &small)) return s;
         const int64_t groups = (n_maps + 63) / 64;
+        const int tb = sat_factor_tb(g, k);
+        if (tb > 0) {
+            const int64_t n_tiles = groups * ((g->n_t + tb - 1) / tb);
+            const int grid = persistent_grid(ctx, mfma_visgen_sat_kernel_ptr(tb), n_tiles,
+                                             mfma_block_threads());
+            Timer tm(ctx, "mfma_visgen");
+            VRS_HIP(ctx, launch_mfma_visgen_sat(g->dev(), phase_knobs(k), g->orb.inv_lambda_km,
+                                                n_maps, g->n_pix, (const float *)sI.dev,
+                                                (unsigned int *)small, (float *)small + n_maps,
+                                                a16, (float2 *)sV.dev, grid, ctx->stream));
+            tm.stop();
+            return sV.finish();
+        }
         const int64_t n_tiles = groups * ((g->n_vis + mfma_vis_per_block() - 1) / mfma_vis_per_block());
         const int grid = persistent_grid(ctx, mfma_visgen_kernel_ptr(), n_tiles, mfma_block_threads());
         Timer tm(ctx, "mfma_visgen");

@@ -182,12 +182,13 @@ __device__ __forceinline__ int64_t kstep_pos(int64_t p) {
 }
 
 // I [n_maps][n_pix] RING -> hi/lo FP16 [group][2][kMaps][n_pad] chunk order
-// (permuted by kstep_pos inside each K-step), scaled.
+// (permuted by kstep_pos inside each K-step when `permute`), scaled.
 __global__ void __launch_bounds__(256) split_maps_kernel(GeomDev g, int32_t n_maps, int64_t n_pix,
                                                          const float *__restrict__ I,
                                                          const unsigned int *__restrict__ amax,
                                                          _Float16 *__restrict__ out,
-                                                         float *__restrict__ inv_scale) {
+                                                         float *__restrict__ inv_scale,
+                                                         int32_t permute) {
     const int64_t n_groups = (n_maps + kMaps - 1) / kMaps;
     const int64_t total = n_groups * kMaps * g.n_pad;
     for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
@@ -205,7 +206,7 @@ __global__ void __launch_bounds__(256) split_maps_kernel(GeomDev g, int32_t n_ma
         }
         const _Float16 h = (_Float16)x;  // nearest
         const _Float16 l = (_Float16)(x - (float)h);
-        const int64_t kp = kstep_pos(pi);
+        const int64_t kp = permute ? kstep_pos(pi) : pi;
         out[((grp * 2 + 0) * kMaps + row) * g.n_pad + kp] = h;
         out[((grp * 2 + 1) * kMaps + row) * g.n_pad + kp] = l;
     }
@@ -707,6 +708,440 @@ __global__ void __launch_bounds__(kThreadsR, 4) mfma_recon_kernel(
     }
 }
 
+// ------------------------------------------------------------------ per-satellite factors
+// On an orbit geometry u v w = (r_y - r_x) / lambda for pair (x, y) at one step,
+// so the phase s (u l + v m + w (n - 1)) of every pair is phi_y - phi_x with
+// phi_x = s r_x . (l, m, n - 1) / lambda (n for the trailing-w convention), and
+// E_xy = E_y conj(E_x): with (c, s) = (cos, sin) 2 pi phi,
+//   cos(phi_y - phi_x) = c_y c_x + s_y s_x,   sin(phi_y - phi_x) = s_y c_x - c_y s_x.
+// The shade mask factorises the same way: (x, y) is clear for a pixel iff
+// neither satellite's ray to it is occulted, so zeroing E_x on the pixels that
+// satellite x cannot see zeroes exactly the shaded entries of every product.
+// Each per-satellite factor (FP64 phase and fract, FP32 sin/cos, its own
+// bit-exact occultation test) serves every pair that satellite is in: n_sat
+// phases and sin/cos per step instead of n_base.
+
+// Largest number of (step, satellite) rows one vis-gen block holds per stage.
+template <int TB>
+struct SatRows {
+    static constexpr int value = TB == 2 ? 32 : 44;
+};
+constexpr int kSatLd = kStage + 2;      // float2 per LDS row: 64 pixels + 16 bytes (rows on shifted banks)
+constexpr int kSatSlots = (kWaves - 1) * kNTV;  // visibilities per block: 15 waves x 16
+constexpr int kGramLd = 33;             // floats per row of a staged 32 x 32 Gram matrix
+
+// x = hi + lo in FP16: hi rounded toward zero, lo = x - hi rounded to nearest.
+// neg1 is -1 hidden from the optimiser (opaque_neg1): fma(hi, neg1, x) is then
+// formed by v_fma_mix_f32, which widens the FP16 operand exactly, instead of a
+// widening conversion and a subtraction (4 VALU per pair instead of 6).
+__device__ __forceinline__ float opaque_neg1() {
+    float m;
+    asm("" : "=s"(m) : "0"(-1.0f));
+    return m;
+}
+__device__ __forceinline__ void split2m(float x0, float x1, float neg1, uint32_t &hi, uint32_t &lo) {
+    typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+    const half2_t h = __builtin_bit_cast(half2_t, __builtin_amdgcn_cvt_pkrtz(x0, x1));
+    half2_t l;
+    l[0] = (_Float16)__builtin_fmaf((float)h[0], neg1, x0);
+    l[1] = (_Float16)__builtin_fmaf((float)h[1], neg1, x1);
+    hi = __builtin_bit_cast(uint32_t, h);
+    lo = __builtin_bit_cast(uint32_t, l);
+}
+
+// The map tile of stage st (both FP16 planes, 64 maps x 64 pixels, 16 KiB) as
+// one 16-byte piece per thread of a 1024-thread block, loaded into registers;
+// tile_store puts it at the same place stage_load_lds would (the swizzled
+// LDS layout).  Staging through registers keeps the LDS reads of the current
+// stage free of waits: a read that may alias an in-flight global->LDS load
+// waits for every such load (vmcnt(0)), which exposed the next stage's whole
+// load latency at the first K-step.
+__device__ __forceinline__ uint4 tile_load(const _Float16 *__restrict__ Ag, int64_t ld, int64_t st) {
+    const int i = threadIdx.x;                       // 16-byte slot of the tile
+    const int hl = i / (kMaps * (kStage / 8));
+    const int rem = i % (kMaps * (kStage / 8));
+    const int row = rem / (kStage / 8), slot = rem % (kStage / 8);
+    const int seg = a_slot(row, slot);
+    return *reinterpret_cast<const uint4 *>(Ag + ((int64_t)hl * kMaps + row) * ld + st * kStage + seg * 8);
+}
+__device__ __forceinline__ void tile_store(_Float16 *sAbuf, uint4 x) {
+    reinterpret_cast<uint4 *>(sAbuf)[threadIdx.x] = x;
+}
+static_assert(2 * kMaps * kStage * 2 == kThreads * 16, "one 16-byte piece of the map tile per thread");
+
+// A row's satellite position (FP64; w = |r|^2, for the exact predicate), the
+// same scaled by s / lambda (the phase of a pixel is one dot product with it)
+// and the row's shade cap: staged in LDS once per tile, so that forming a stage
+// reads no global memory (LDS reads do not queue behind the vector-memory
+// prefetches of the next stage).
+struct SatRow {
+    double4 pos;
+    double4 spos;
+    CapRec cap;
+};
+static_assert(sizeof(SatRow) == 96, "row record layout");
+
+// A stage's pixel inputs, staged in LDS: pixel_piece loads one 16-byte piece of
+// them into a register at the top of the stage two before their use, and
+// pixel_store puts it in place at the end of that stage.  Pieces: 128 of the FP64
+// vectors, 64 of the FP32 vectors, 16 of the RING indices, 2 of the 64-pixel
+// group's bounding cap (210 of the block's threads take one each).
+struct PixStage {
+    double4 p64[kStage];
+    float4 p32[kStage];
+    int32_t ring[kStage];
+    ChunkRec sub;
+};
+constexpr int kPixPieces = kStage * 2 + kStage + kStage / 4 + 2;
+static_assert(sizeof(PixStage) == 16 * kPixPieces, "pixel pieces tile the record");
+
+// One load per thread from an address picked without branches (a branch per
+// source would serialise the loads into one register behind waits).  Threads
+// past the pieces reload piece 0 and store nothing.
+__device__ __forceinline__ uint4 pixel_piece(const GeomDev &g, int64_t st) {
+    const int i = threadIdx.x;
+    const int64_t p0 = st * kStage;
+    const uint4 *src = reinterpret_cast<const uint4 *>(g.pix64 + p0) + (i < 2 * kStage ? i : 0);
+    src = i >= 2 * kStage && i < 3 * kStage
+              ? reinterpret_cast<const uint4 *>(g.pix32 + p0) + (i - 2 * kStage) : src;
+    src = i >= 3 * kStage && i < 3 * kStage + kStage / 4
+              ? reinterpret_cast<const uint4 *>(g.ring + p0) + (i - 3 * kStage) : src;
+    src = i >= 3 * kStage + kStage / 4 && i < kPixPieces
+              ? reinterpret_cast<const uint4 *>(g.sub + st) + (i - 3 * kStage - kStage / 4) : src;
+    return *src;
+}
+__device__ __forceinline__ void pixel_store(PixStage *d, uint4 x) {
+    if (threadIdx.x < kPixPieces) reinterpret_cast<uint4 *>(d)[threadIdx.x] = x;
+}
+
+// Rows r = wave + kWaves i (i < MR, bit i of `live`) of one stage: for (step
+// t0 + r / n_sat, satellite r % n_sat) and the stage's 64 pixels (lane = pixel),
+// E_x = (cos, sin) of 2 pi phi_x with the pixels that satellite cannot see
+// zeroed into sEm, and with FULL the unzeroed factor into sEf (the Gram sums
+// need it; with FULL, padding pixels are zero in both, while vis-gen alone
+// needs no padding test: the map operand is zero on them).  Occultation per pixel: the FP32 test
+// against the guard band's edges, the exact FP64 predicate inside the band (the
+// rule of every kernel here; a per-pixel test costs fewer instructions than
+// classifying the row's cap against the 64-pixel group first).
+template <bool FULL, int MR>
+__device__ __forceinline__ void sat_rows(const GeomDev &g, double dzo, int wave, uint32_t live,
+                                         const PixStage *ps, const SatRow *sRow,
+                                         float2 (*sEm)[kSatLd], float2 (*sEf)[kSatLd]) {
+    const int lane = threadIdx.x & 63;
+    const double4 s = ps->p64[lane];
+    const float4 s32 = ps->p32[lane];
+    const double szd = s.z - dzo;
+#pragma unroll
+    for (int i = 0; i < MR; ++i) {
+        if (!((live >> i) & 1u)) continue;  // wave-uniform
+        const int r = wave + kWaves * i;
+        const double4 sp = sRow[r].spos;
+        const CapRec cap = sRow[r].cap;
+        const double ph = fma(sp.x, s.x, fma(sp.y, s.y, sp.z * szd));
+        const float rv = (float)__builtin_amdgcn_fract(ph);
+        float2 ef = make_float2(__builtin_amdgcn_cosf(rv), __builtin_amdgcn_sinf(rv));
+        if (FULL && ps->ring[lane] < 0) ef = make_float2(0.f, 0.f);
+        const float x = fmaf(s32.z, cap.dz, fmaf(s32.y, cap.dy, s32.x * cap.dx));
+        bool occ = x >= cap.hi;
+        if (x >= cap.lo && !occ) occ = occ64m(sRow[r].pos, s.x, s.y, s.z, g.limb);
+        sEm[r][lane] = occ ? make_float2(0.f, 0.f) : ef;
+        if (FULL) sEf[r][lane] = ef;
+    }
+}
+
+// 16 x 16 Gram blocks of one step, staged for the consumers: [P1, P2] (full-sky
+// sums; see mfma_visgen_sat_kernel).
+constexpr int kGramLd16 = 17;
+
+// Vis-gen on per-satellite factors.  Block: TB consecutive steps of every
+// baseline (TB * n_base <= 240 visibilities, slot j = step (j / n_base),
+// baseline (j % n_base)); waves 0-14 hold 16 slots each on the 16x16x32
+// instruction (the products of mfma_visgen_kernel, with E'_xy = E'_y conj(E'_x)
+// formed from the two staged rows of each lane's pair); every wave forms its
+// share of the next stage's rows after its K-steps.
+//
+// Shaded-pixel term (TOCC): t_occ sum_{p shaded} E_xy = t_occ (U_xy - U'_xy),
+// with U the full-sky sum sum_p E_y conj(E_x) and U' the same over clear pixels.
+// U' is the sum of the lane's own clear entries (two adds per entry); U is a
+// Gram matrix of the unmasked rows of a step, which wave 15 contracts on the
+// matrix cores: with C, S the cos and
+// sin rows of the 16 satellites, Re U_xy = P1[y][x] = (C C^T + S S^T)[y][x],
+// and sum_p sin(phi_y - phi_x) = P2[x][y] - P2[y][x], P2 = C S^T.  Split FP16:
+// P1 = K + K^T with K = Hc Hc^T / 2 + Hc Lc^T + Hs Hs^T / 2 + Hs Ls^T, and
+// P2 = Hc Hs^T + Hc Ls^T + Lc Hs^T.  Each wave role runs its own stage loop
+// (with the same barriers), so the Gram accumulators and the consumers'
+// accumulators never share registers.
+template <int TB, bool TOCC>
+__global__ void __launch_bounds__(kThreads) mfma_visgen_sat_kernel(
+    GeomDev g, PhaseKnobs k, double sc, int32_t n_maps, const _Float16 *__restrict__ A16,
+    const float *__restrict__ inv_scale, float2 *__restrict__ V) {
+    constexpr int R = SatRows<TB>::value;
+    __shared__ _Float16 sA[2][2][kMaps][kStage];  // [buffer][hi/lo][map][pixel]
+    __shared__ float2 sE[2][TOCC ? 2 : 1][R][kSatLd];  // [buffer][masked, full][row][pixel]
+    __shared__ float sG[TOCC ? TB : 1][2][16][kGramLd16];
+    __shared__ SatRow sRow[R];
+    __shared__ PixStage sPix[2];  // [stage & 1]
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int col = lane & 15, kg = lane >> 4;
+    const int n_sat = g.n_sat, n_base = g.n_base;
+    const int n_rows = TB * n_sat;
+    const int n_slot = TB * n_base;
+    const int n_ct = (n_slot + kNTV - 1) / kNTV;  // consumer waves
+    const int64_t n_tb = (g.n_t + TB - 1) / TB;
+    const int64_t n_groups = (n_maps + kMaps - 1) / kMaps;
+    const int64_t n_tiles = n_tb * n_groups;
+    const int64_t n_stages = g.n_pad / kStage;
+    const double dzo = k.wfac ? 0.0 : 1.0;
+    const float t_occ = k.t_occ;
+    const bool consumer = wave < n_ct;             // wave-uniform
+    const bool gram = TOCC && wave == kWaves - 1;  // wave-uniform
+    constexpr int MR = (R + kWaves - 1) / kWaves;  // most rows one wave forms
+    const int nr = wave < n_rows ? (n_rows - 1 - wave) / kWaves + 1 : 0;  // rows this wave forms
+    int rtl[MR];  // step (within the block) of each of this wave's rows
+#pragma unroll
+    for (int i = 0; i < MR; ++i) rtl[i] = (wave + kWaves * i) / n_sat;
+    // this lane's slot (fixed over the tiles)
+    const int j = wave * kNTV + col;
+    const int tlv = j / n_base, bi = j - tlv * n_base;
+    int rx = 0, ry = 0, px = 0, py = 0;
+    if (consumer && j < n_slot) {
+        const int2 pr = g.pairs[bi];
+        px = pr.x;
+        py = pr.y;
+        rx = tlv * n_sat + px;
+        ry = tlv * n_sat + py;
+    }
+    const float neg1 = opaque_neg1();
+
+    for (int64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+        const int64_t grp = tile / n_tb;
+        const int t0 = (int)(tile % n_tb) * TB;
+        const bool vok = consumer && j < n_slot && t0 + tlv < g.n_t;
+        const int64_t v = (int64_t)bi * g.n_t + t0 + tlv;
+        const _Float16 *Ag = A16 + grp * 2 * kMaps * g.n_pad;
+        uint32_t live = 0;  // this wave's rows whose step exists
+#pragma unroll
+        for (int i = 0; i < MR; ++i)
+            if (i < nr && t0 + rtl[i] < g.n_t) live |= 1u << i;
+        {
+            // The buffers, sRow and sPix are free: every wave passed the previous
+            // tile's last barrier, and the last stage forms no rows.
+            if (threadIdx.x < n_rows) {
+                const int r = threadIdx.x;
+                const int tl = r / n_sat, sat = r - tl * n_sat;
+                SatRow e;
+                if (t0 + tl < g.n_t) {
+                    const int64_t row = (int64_t)sat * g.n_t + t0 + tl;
+                    e.pos = g.pos[row];
+                    e.cap = g.cap[row];
+                } else {
+                    e.pos = make_double4(0.0, 0.0, 0.0, 0.0);
+                    e.cap = {0.f, 0.f, 0.f, 2.0f, 0.f, kGuardMin, 2.0f - kGuardMin, 2.0f + kGuardMin};
+                }
+                e.spos = make_double4(e.pos.x * sc, e.pos.y * sc, e.pos.z * sc, 0.0);
+                sRow[r] = e;
+            }
+            tile_store(&sA[0][0][0][0], tile_load(Ag, g.n_pad, 0));
+            pixel_store(&sPix[0], pixel_piece(g, 0));
+            if (n_stages > 1) pixel_store(&sPix[1], pixel_piece(g, 1));
+            __syncthreads();
+            sat_rows<TOCC, MR>(g, dzo, wave, live, &sPix[0], sRow, sE[0][0], sE[0][TOCC ? 1 : 0]);
+            __syncthreads();
+        }
+        // Prefetch into registers, stored at the end of each stage: the map tile
+        // of stage st + 1 and the pixel inputs of stage st + 2 (loaded without a
+        // branch: the last stages reload their own).  Nothing in a stage waits
+        // for them.
+        auto stage_end = [&](int64_t st, uint4 tnext, uint4 pnext) {
+            const int buf = (int)(st & 1);
+            // sA[buf ^ 1] was last read before the previous barrier; sPix[buf] holds
+            // stage st's inputs, last read while forming stage st
+            asm volatile("" ::: "memory");
+            if (st + 1 < n_stages) tile_store(&sA[buf ^ 1][0][0][0], tnext);
+            if (st + 2 < n_stages) pixel_store(&sPix[buf], pnext);
+            __syncthreads();
+        };
+        float2 vout[kMaps / kMTV][4];  // consumer results, written after the shaded term
+        float clr_c = 0.0f, clr_s = 0.0f;  // sum of the lane's clear entries (TOCC)
+        if (consumer) {
+            floatx4 acc[kMaps / kMTV][2];
+#pragma unroll
+            for (int a = 0; a < kMaps / kMTV; ++a)
+#pragma unroll
+                for (int c = 0; c < 2; ++c) acc[a][c] = floatx4{};
+            for (int64_t st = 0; st < n_stages; ++st) {
+                const int buf = (int)(st & 1);
+                const uint4 tnext = tile_load(Ag, g.n_pad, st + 1 < n_stages ? st + 1 : st);
+                const uint4 pnext = pixel_piece(g, st + 2 < n_stages ? st + 2 : st);
+                // E of K-step ks for this lane's 8 pixels (ks * 32 + 8 kg + i, identity
+                // K order), read one K-step ahead so that the reads of K-step ks + 1
+                // are in flight while the products of K-step ks issue
+                float4 ea[4], eb[4];
+                auto read_e = [&](int ks) {
+                    const float2 *ex = &sE[buf][0][rx][ks * kKSV + kg * 8];
+                    const float2 *ey = &sE[buf][0][ry][ks * kKSV + kg * 8];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        ea[q] = *reinterpret_cast<const float4 *>(ex + 2 * q);
+                        eb[q] = *reinterpret_cast<const float4 *>(ey + 2 * q);
+                    }
+                };
+                read_e(0);
+#pragma unroll
+                for (int ks = 0; ks < kStage / kKSV; ++ks) {
+                    float cs[8], sn[8];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const float4 a = ea[q], b = eb[q];
+                        cs[2 * q] = fmaf(b.x, a.x, b.y * a.y);
+                        sn[2 * q] = fmaf(b.y, a.x, -(b.x * a.y));
+                        cs[2 * q + 1] = fmaf(b.z, a.z, b.w * a.w);
+                        sn[2 * q + 1] = fmaf(b.w, a.z, -(b.z * a.w));
+                    }
+                    if (ks + 1 < kStage / kKSV) read_e(ks + 1);
+                    if (TOCC) {
+#pragma unroll
+                        for (int t = 0; t < 8; ++t) {
+                            clr_c += cs[t];
+                            clr_s += sn[t];
+                        }
+                    }
+                    H8 bre_h, bre_l, bim_h, bim_l;
+#pragma unroll
+                    for (int t2 = 0; t2 < 4; ++t2) {
+                        split2m(cs[2 * t2], cs[2 * t2 + 1], neg1, bre_h.u[t2], bre_l.u[t2]);
+                        split2m(sn[2 * t2], sn[2 * t2 + 1], neg1, bim_h.u[t2], bim_l.u[t2]);
+                    }
+#pragma unroll
+                    for (int mt = 0; mt < kMaps / kMTV; ++mt) {
+                        const int row = mt * kMTV + col, so = a_slot(row, ks * 4 + kg) * 8;
+                        const half8 ah = *reinterpret_cast<const half8 *>(&sA[buf][0][row][so]);
+                        const half8 al = *reinterpret_cast<const half8 *>(&sA[buf][1][row][so]);
+                        acc[mt][0] = mfma16(ah, bre_h.v, acc[mt][0]);
+                        acc[mt][0] = mfma16(ah, bre_l.v, acc[mt][0]);
+                        acc[mt][0] = mfma16(al, bre_h.v, acc[mt][0]);
+                        acc[mt][1] = mfma16(ah, bim_h.v, acc[mt][1]);
+                        acc[mt][1] = mfma16(ah, bim_l.v, acc[mt][1]);
+                        acc[mt][1] = mfma16(al, bim_h.v, acc[mt][1]);
+                    }
+                }
+                if (st + 1 < n_stages)
+                    sat_rows<TOCC, MR>(g, dzo, wave, live, &sPix[buf ^ 1], sRow, sE[buf ^ 1][0],
+                             sE[buf ^ 1][TOCC ? 1 : 0]);
+                stage_end(st, tnext, pnext);
+            }
+#pragma unroll
+            for (int mt = 0; mt < kMaps / kMTV; ++mt)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int64_t m = grp * kMaps + mt * kMTV + 4 * kg + r;
+                    const float is = m < n_maps ? inv_scale[m] : 0.0f;
+                    vout[mt][r] = make_float2(acc[mt][0][r] * is, -(acc[mt][1][r] * is));
+                }
+        } else if (gram) {
+            // this lane: satellite gs = lane % 16 (its row and column of the Gram
+            // blocks), pixels 8 kg + i of each 32-pixel K-step
+            floatx4 p1[TB], p2[TB];
+#pragma unroll
+            for (int q = 0; q < TB; ++q) p1[q] = p2[q] = floatx4{};
+            const int gs = lane & 15;
+            for (int64_t st = 0; st < n_stages; ++st) {
+                const int buf = (int)(st & 1);
+                const uint4 tnext = tile_load(Ag, g.n_pad, st + 1 < n_stages ? st + 1 : st);
+                const uint4 pnext = pixel_piece(g, st + 2 < n_stages ? st + 2 : st);
+#pragma unroll
+                for (int q = 0; q < TB; ++q) {
+                    if (t0 + q >= g.n_t) continue;  // wave-uniform
+#pragma unroll 1
+                    for (int kk = 0; kk < kStage / kKSV; ++kk) {
+                        // one Gram block pair from one staged row set (full, then masked)
+                        auto gram_rows = [&](const float2(*rows)[kSatLd], floatx4 &a1, floatx4 &a2) {
+                            float cc[8], ss[8];
+                            if (gs < n_sat) {
+                                const float2 *e = &rows[q * n_sat + gs][kk * kKSV + kg * 8];
+#pragma unroll
+                                for (int i = 0; i < 4; ++i) {
+                                    const float4 a = *reinterpret_cast<const float4 *>(e + 2 * i);
+                                    cc[2 * i] = a.x; ss[2 * i] = a.y; cc[2 * i + 1] = a.z; ss[2 * i + 1] = a.w;
+                                }
+                            } else {
+#pragma unroll
+                                for (int i = 0; i < 8; ++i) cc[i] = ss[i] = 0.0f;
+                            }
+                            H8 hc, lc, hs, ls, hc2, hs2;
+#pragma unroll
+                            for (int i = 0; i < 4; ++i) {
+                                split2m(cc[2 * i], cc[2 * i + 1], neg1, hc.u[i], lc.u[i]);
+                                split2m(ss[2 * i], ss[2 * i + 1], neg1, hs.u[i], ls.u[i]);
+                            }
+                            hc2.v = hc.v * (_Float16)0.5f;
+                            hs2.v = hs.v * (_Float16)0.5f;
+                            a1 = mfma16(hc2.v, hc.v, a1);
+                            a1 = mfma16(hc.v, lc.v, a1);
+                            a1 = mfma16(hs2.v, hs.v, a1);
+                            a1 = mfma16(hs.v, ls.v, a1);
+                            a2 = mfma16(hc.v, hs.v, a2);
+                            a2 = mfma16(hc.v, ls.v, a2);
+                            a2 = mfma16(lc.v, hs.v, a2);
+                        };
+                        gram_rows(sE[buf][TOCC ? 1 : 0], p1[q], p2[q]);
+                    }
+                }
+                if (st + 1 < n_stages)
+                    sat_rows<TOCC, MR>(g, dzo, wave, live, &sPix[buf ^ 1], sRow, sE[buf ^ 1][0],
+                             sE[buf ^ 1][TOCC ? 1 : 0]);
+                stage_end(st, tnext, pnext);
+            }
+            if (TOCC) {
+                // D of 16x16x32: lane holds rows 4 (lane / 16) + r of column lane % 16
+#pragma unroll
+                for (int q = 0; q < TB; ++q)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int a = 4 * kg + r;
+                        sG[q][0][a][gs] = p1[q][r];
+                        sG[q][1][a][gs] = p2[q][r];
+                    }
+            }
+        } else {
+            for (int64_t st = 0; st < n_stages; ++st) {
+                const int buf = (int)(st & 1);
+                const uint4 tnext = tile_load(Ag, g.n_pad, st + 1 < n_stages ? st + 1 : st);
+                const uint4 pnext = pixel_piece(g, st + 2 < n_stages ? st + 2 : st);
+                if (st + 1 < n_stages)
+                    sat_rows<TOCC, MR>(g, dzo, wave, live, &sPix[buf ^ 1], sRow, sE[buf ^ 1][0],
+                             sE[buf ^ 1][TOCC ? 1 : 0]);
+                stage_end(st, tnext, pnext);
+            }
+        }
+        if (TOCC) __syncthreads();  // sG whole
+        if (TOCC) {
+            clr_c += __shfl_xor(clr_c, 16, 64);
+            clr_s += __shfl_xor(clr_s, 16, 64);
+            clr_c += __shfl_xor(clr_c, 32, 64);
+            clr_s += __shfl_xor(clr_s, 32, 64);
+        }
+        if (vok) {
+            float shc = 0.0f, shs = 0.0f;
+            if (TOCC) {
+                // shaded sums: the pair's full-sky Gram sums minus its clear entries
+                const float(*G)[16][kGramLd16] = sG[tlv];
+                shc = t_occ * (G[0][py][px] + G[0][px][py] - clr_c);
+                shs = t_occ * (G[1][px][py] - G[1][py][px] - clr_s);
+            }
+#pragma unroll
+            for (int mt = 0; mt < kMaps / kMTV; ++mt)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int64_t m = grp * kMaps + mt * kMTV + 4 * kg + r;
+                    // V = sum_clear I E + t_occ sum_shaded E, E = cos - i sin
+                    if (m < n_maps)
+                        V[m * g.n_vis + v] = make_float2(vout[mt][r].x + shc, vout[mt][r].y - shs);
+                }
+        }
+    }
+}
 }  // namespace
 
 int mfma_vis_pad(int64_t n_vis, int64_t *n_vis_pad) {
@@ -734,11 +1169,60 @@ hipError_t launch_mfma_visgen(const GeomDev &g, const PhaseKnobs &k, int32_t n_m
                        (const float *)nullptr, amax);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(split_maps_kernel, dim3(2048), dim3(256), 0, st, g, n_maps, n_pix, I, amax,
-                       (_Float16 *)a16, inv_scale);
+                       (_Float16 *)a16, inv_scale, 1);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(mfma_visgen_kernel, dim3(grid), dim3(kThreads), 0, st, g, k, n_maps,
                        (const _Float16 *)a16, inv_scale, V);
     return hipGetLastError();
+}
+
+int mfma_sat_tb(int32_t n_sat, int32_t n_base) {
+    if (n_sat < 2 || n_sat > 16 || n_base < 1) return 0;
+    if (2 * n_base <= kSatSlots && 2 * n_sat <= SatRows<2>::value &&
+        (4 * n_base > kSatSlots || 4 * n_sat > SatRows<4>::value))
+        return 2;
+    if (4 * n_base <= kSatSlots && 4 * n_sat <= SatRows<4>::value) return 4;
+    return 0;
+}
+
+template <int TB, bool TOCC>
+static void launch_sat_visgen_t(const GeomDev &g, const PhaseKnobs &k, double sc, int32_t n_maps,
+                                const void *a16, const float *inv_scale, float2 *V, int grid,
+                                hipStream_t st) {
+    hipLaunchKernelGGL((mfma_visgen_sat_kernel<TB, TOCC>), dim3(grid), dim3(kThreads), 0, st, g, k,
+                       sc, n_maps, (const _Float16 *)a16, inv_scale, V);
+}
+
+hipError_t launch_mfma_visgen_sat(const GeomDev &g, const PhaseKnobs &k, double inv_lambda_km,
+                                  int32_t n_maps, int64_t n_pix, const float *I,
+                                  unsigned int *amax, float *inv_scale, void *a16, float2 *V,
+                                  int grid, hipStream_t st) {
+    const int tb = mfma_sat_tb(g.n_sat, g.n_base);
+    if (tb == 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(zero_u32_kernel, dim3((n_maps + 255) / 256), dim3(256), 0, st, amax, n_maps);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(row_absmax_kernel, dim3(64, n_maps), dim3(256), 0, st, I, n_pix, n_pix, 0,
+                       (const float *)nullptr, amax);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(split_maps_kernel, dim3(2048), dim3(256), 0, st, g, n_maps, n_pix, I, amax,
+                       (_Float16 *)a16, inv_scale, 0);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    const double sc = k.s * inv_lambda_km;
+    const bool tocc = k.t_occ != 0.0f;
+    if (tb == 2) {
+        if (tocc) launch_sat_visgen_t<2, true>(g, k, sc, n_maps, a16, inv_scale, V, grid, st);
+        else launch_sat_visgen_t<2, false>(g, k, sc, n_maps, a16, inv_scale, V, grid, st);
+    } else {
+        if (tocc) launch_sat_visgen_t<4, true>(g, k, sc, n_maps, a16, inv_scale, V, grid, st);
+        else launch_sat_visgen_t<4, false>(g, k, sc, n_maps, a16, inv_scale, V, grid, st);
+    }
+    return hipGetLastError();
+}
+
+const void *mfma_visgen_sat_kernel_ptr(int tb) {
+    return tb == 2 ? (const void *)mfma_visgen_sat_kernel<2, true>
+                   : (const void *)mfma_visgen_sat_kernel<4, true>;
 }
 
 hipError_t launch_mfma_recon(const GeomDev &g, const PhaseKnobs &k, int32_t n_maps,

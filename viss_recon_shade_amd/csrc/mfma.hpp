@@ -24,6 +24,16 @@ hipError_t launch_mfma_recon(const GeomDev &g, const PhaseKnobs &k, int32_t n_ma
                              unsigned int *amax, float *inv_scale, void *a16, int32_t n_seg,
                              float *part, int grid, hipStream_t st);
 
+// Per-satellite phase factors (orbit geometries whose u v w are satellite
+// position differences): steps per vis-gen block for this array, 0 if the
+// satellite kernels do not apply (n_sat outside [2, 16]).
+int mfma_sat_tb(int32_t n_sat, int32_t n_base);
+hipError_t launch_mfma_visgen_sat(const GeomDev &g, const PhaseKnobs &k, double inv_lambda_km,
+                                  int32_t n_maps, int64_t n_pix, const float *I,
+                                  unsigned int *amax, float *inv_scale, void *a16, float2 *V,
+                                  int grid, hipStream_t st);
+const void *mfma_visgen_sat_kernel_ptr(int tb);
+
 const void *mfma_visgen_kernel_ptr();
 const void *mfma_recon_kernel_ptr();
 int mfma_block_threads();         // vis-gen block
