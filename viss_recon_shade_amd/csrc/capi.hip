@@ -964,6 +964,20 @@ static int sat_factor_tb(const vrs_geom *g, const vrs_knobs *k) {
     return mfma_sat_tb(g->n_sat, g->n_base);
 }
 
+// Satellite count of the per-satellite-factor matrix-core recon, 0 where it does
+// not apply: factorisable u v w (as for vis-gen), 8 or 16 satellites, and the
+// lexicographic pair list (its operand layout is built on it).
+static int sat_recon_ns(const vrs_geom *g, const vrs_knobs *k) {
+    if (!k->sat_factors || g->from_uvw || (g->uvw_flags & VRS_UVW_CEIL)) return 0;
+    const int ns = mfma_recon_sat_ns(g->n_sat);
+    if (ns == 0 || g->n_base != ns * (ns - 1) / 2) return 0;
+    int b = 0;
+    for (int a = 0; a < ns; ++a)
+        for (int c = a + 1; c < ns; ++c, ++b)
+            if (g->pairs[2 * b] != a || g->pairs[2 * b + 1] != c) return 0;
+    return ns;
+}
+
 vrs_status vrs_visgen(vrs_ctx *ctx, const vrs_geom *g, const vrs_knobs *k, int32_t n_maps,
                       const float *I, float *V) {
     if (vrs_status s = check_ctx(ctx)) return s;
@@ -1093,6 +1107,36 @@ static vrs_status recon_common(vrs_ctx *ctx, const vrs_geom *g, const vrs_knobs 
     if (vrs_status s = sV.prepare()) return s;
     if (vrs_status s = sw.prepare()) return s;
     if (vrs_status s = sF.prepare()) return s;
+    if (k->precision == VRS_MFMA_SPLIT16 && sat_recon_ns(g, k) > 0) {
+        const int64_t n_slots = mfma_recon_sat_slots(g->n_sat, g->n_t);
+        const int64_t groups = (n_maps + 63) / 64;
+        const int64_t ptiles = (g->n_pad + mfma_recon_sat_pix_per_block() - 1) / mfma_recon_sat_pix_per_block();
+        const int64_t n_tp = ((int64_t)g->n_t + 1) / 2;
+        int64_t n_seg = (2048 + ptiles * groups - 1) / (ptiles * groups);
+        n_seg = std::max<int64_t>(1, std::min<int64_t>(n_seg, n_tp));
+        if (n_seg > 8) n_seg = std::min<int64_t>((n_seg + 7) / 8 * 8, n_tp);
+        const size_t small_b = (8 * (size_t)n_maps + 256 + 255) / 256 * 256;
+        const size_t spos_b = sizeof(double) * 4 * (size_t)g->n_sat * g->n_t;
+        void *a16, *small, *part;
+        if (vrs_status s = ws_get(ctx, 2, (size_t)groups * 4 * 64 * n_slots * 2, &a16)) return s;
+        if (vrs_status s = ws_get(ctx, 3, small_b + spos_b, &small)) return s;
+        if (vrs_status s = ws_get(ctx, 1, sizeof(float) * n_seg * n_maps * g->n_pad, &part)) return s;
+        const int ns = sat_recon_ns(g, k);
+        const int grid = persistent_grid(ctx, mfma_recon_sat_kernel_ptr(ns), ptiles * groups * n_seg,
+                                         mfma_recon_sat_threads());
+        const GeomDev gd = g->dev();
+        {
+            Timer tm(ctx, "mfma_recon");
+            VRS_HIP(ctx, launch_mfma_recon_sat(gd, phase_knobs(k), g->orb.inv_lambda_km, n_maps,
+                                               (const float2 *)sV.dev, (const float *)sw.dev,
+                                               (float)kappa, n_slots, (unsigned int *)small,
+                                               (float *)small + n_maps, a16, (char *)small + small_b,
+                                               (int32_t)n_seg, (float *)part, grid, ctx->stream));
+        }
+        VRS_HIP(ctx, launch_recon_finalize(gd, n_maps, (int32_t)n_seg, g->n_pix,
+                                           (const float *)part, (float *)sF.dev, ctx->stream));
+        return sF.finish();
+    }
     if (k->precision == VRS_MFMA_SPLIT16) {
         int64_t n_vis_pad;
         mfma_vis_pad(g->n_vis, &n_vis_pad);

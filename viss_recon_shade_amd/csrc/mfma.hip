@@ -1142,6 +1142,313 @@ __global__ void __launch_bounds__(kThreads) mfma_visgen_sat_kernel(
         }
     }
 }
+// ------------------------------------------------------------------ recon on per-satellite factors
+// Pairs of an n-satellite array (n even) in star order: star c (c = 0 .. n - 2)
+// holds the pairs of satellite c with satellite n - 1 and with c + 1 .. c +
+// (n - 2) / 2 (mod n - 1), so n - 1 stars of n / 2 pairs cover every pair once
+// (a regular tournament on the first n - 1 satellites, each beating the last).
+// Entry j of the list is pair (star_x, star_y), x < y; star_center its star's
+// centre.  Past the n (n - 1) / 2 pairs the entries are padding (-1).
+__host__ __device__ constexpr int star_center(int n, int j) { return j / (n / 2); }
+__host__ __device__ constexpr int star_leaf(int n, int j) {
+    return j % (n / 2) == 0 ? n - 1 : (star_center(n, j) + j % (n / 2)) % (n - 1);
+}
+__host__ __device__ constexpr int star_x(int n, int j) {
+    return j >= n * (n - 1) / 2 ? -1
+                                : (star_center(n, j) < star_leaf(n, j) ? star_center(n, j) : star_leaf(n, j));
+}
+__host__ __device__ constexpr int star_y(int n, int j) {
+    return j >= n * (n - 1) / 2 ? -1
+                                : (star_center(n, j) < star_leaf(n, j) ? star_leaf(n, j) : star_center(n, j));
+}
+// Index of pair (x, y), x < y, in the lexicographic pair list (0,1), (0,2), ...
+__host__ __device__ constexpr int lex_pair(int n, int x, int y) { return x * n - x * (x + 1) / 2 + (y - x - 1); }
+
+// Layout of the recon operand over (step pair, pair): per step pair tp (steps
+// 2 tp and 2 tp + 1) kKTP K-steps of 16 slots; slot 16 ks + 8 h + e holds pair
+// 8 ks + e (star order) at step 2 tp + h.  kKTP is a multiple of 8, so each
+// step pair fills an even number of 64-slot stages.
+template <int NS>
+struct SatRecon {
+    static constexpr int kPairs = NS * (NS - 1) / 2;
+    static constexpr int kKT = (kPairs + 7) / 8;
+    static constexpr int kKTP = (kKT + 7) / 8 * 8;
+    static constexpr int kSlots = 16 * kKTP;  // per step pair
+    static constexpr int kStages = kKTP / 4;  // per step pair: even, so a stage's
+                                              // buffer parity is known at compile time
+};
+
+// V [n_maps][n_vis] complex -> hi/lo FP16 [group][re, -im][hi, lo][kMaps][n_tp * kSlots]
+// in the step-pair layout above, weighted by w * kappa and scaled per map.
+template <int NS>
+__global__ void __launch_bounds__(256) split_vis_sat_kernel(int64_t n_vis, int32_t n_t, int64_t n_slots,
+                                                            int32_t n_maps, const float2 *__restrict__ V,
+                                                            const float *__restrict__ w, float kappa,
+                                                            const unsigned int *__restrict__ amax,
+                                                            _Float16 *__restrict__ out,
+                                                            float *__restrict__ inv_scale) {
+    using L = SatRecon<NS>;
+    const int64_t n_groups = (n_maps + kMaps - 1) / kMaps;
+    const int64_t total = n_groups * kMaps * n_slots;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t slot = idx % n_slots;
+        const int64_t mrow = idx / n_slots;
+        const int64_t grp = mrow / kMaps, row = mrow % kMaps;
+        const int64_t m = grp * kMaps + row;
+        const int64_t tp = slot / L::kSlots;
+        const int rem = (int)(slot % L::kSlots);
+        const int j = 8 * (rem / 16) + rem % 8;
+        const int64_t t = 2 * tp + (rem / 8) % 2;
+        float xr = 0.0f, xi = 0.0f;
+        if (m < n_maps) {
+            const float sc = row_scale(amax[m], fabsf(kappa));
+            if (j < L::kPairs && t < n_t) {
+                const int64_t v = (int64_t)lex_pair(NS, star_x(NS, j), star_y(NS, j)) * n_t + t;
+                const float2 q = V[m * n_vis + v];
+                const float ww = (w ? w[v] : 1.0f) * kappa * sc;
+                xr = q.x * ww;
+                xi = -(q.y * ww);
+            }
+            if (slot == 0) inv_scale[m] = 1.0f / sc;
+        }
+        const _Float16 hr = (_Float16)xr, hi_ = (_Float16)xi;
+        const _Float16 lr = (_Float16)(xr - (float)hr), li = (_Float16)(xi - (float)hi_);
+        const int64_t base = grp * 4 * kMaps;
+        out[((base + 0 * kMaps) + row) * n_slots + slot] = hr;
+        out[((base + 1 * kMaps) + row) * n_slots + slot] = lr;
+        out[((base + 2 * kMaps) + row) * n_slots + slot] = hi_;
+        out[((base + 3 * kMaps) + row) * n_slots + slot] = li;
+    }
+}
+
+// Satellite positions scaled by s / lambda (the phase of a pixel is one dot
+// product with them); zero-filled samples stay zero.
+__global__ void __launch_bounds__(256) scale_pos_kernel(int64_t n, const double4 *__restrict__ pos,
+                                                        double sc, double4 *__restrict__ spos) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const double4 p = pos[i];
+        spos[i] = make_double4(p.x * sc, p.y * sc, p.z * sc, 0.0);
+    }
+}
+
+// Recon block on per-satellite factors: 8 waves (two per SIMD: the factors of
+// 16 satellites take 32 registers, and at three waves per SIMD the kernel
+// spilled, its reloads waiting behind the stage's loads in flight) x 32 pixels.
+constexpr int kWavesRS = 8;
+constexpr int kThreadsRS = kWavesRS * 64;
+
+// The weighted-V tile of stage st (4 FP16 planes x 64 maps x 64 slots, 32 KiB)
+// straight from global memory into LDS (global_load_lds_dwordx4), placed as
+// stage_load_lds<4> places it: 32 blocks of 1 KiB, wave w moving blocks w,
+// w + kWavesRS, ...  The two stage buffers are distinct LDS arrays, chosen at
+// compile time, so that the compiler can tell the K-steps' reads of one from
+// the loads in flight into the other (reads it cannot tell apart would wait
+// for every load in flight).
+__device__ __forceinline__ void vtile_dma(const _Float16 *__restrict__ Ag, int64_t ld, int64_t st,
+                                          _Float16 *sAbuf) {
+    constexpr int kBlocks = 4 * kMaps * (kStage / 8) / 64;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int r = 0; r < (kBlocks + kWavesRS - 1) / kWavesRS; ++r) {
+        const int blk = wave + r * kWavesRS;
+        if (blk < kBlocks) {
+            const int i = blk * 64 + lane;
+            const int plane = i / (kMaps * (kStage / 8));
+            const int rem = i % (kMaps * (kStage / 8));
+            const int row = rem / (kStage / 8), slot = rem % (kStage / 8);
+            const _Float16 *src = Ag + ((int64_t)plane * kMaps + row) * ld + st * kStage + a_slot(row, slot) * 8;
+            __builtin_amdgcn_global_load_lds(src, (lds_ptr_t)(sAbuf + blk * 512), 16, 0, 0);
+        }
+    }
+}
+
+// A satellite's scaled position and shade cap at one step (64 bytes).
+struct RecRow {
+    double4 spos;
+    CapRec cap;
+};
+
+// Recon on per-satellite factors (NS satellites, the lexicographic pair list).
+// Block: 8 waves x 32 pixels; K runs over step pairs, 4 K-steps of 16 slots per
+// stage: lanes 0-31 hold step 2 tp, lanes 32-63 step 2 tp + 1, and each lane's 8
+// entries of a K-step are the 8 pairs of one star (16 satellites; two stars for
+// 8).  At the start of each step pair every lane forms its pixel's factors of
+// all NS satellites at its step in registers (FP64 phase and fract, FP32 sin and
+// cos, zeroed where that satellite's ray to the pixel is occulted: the exact
+// per-pixel rule), so each entry is E'_xy = E'_y conj(E'_x), one complex
+// product, and a shaded entry is zero by construction.  A K-step whose star
+// centres are occulted for every lane of the wave is all zero and skipped.
+template <int NS>
+__global__ void __launch_bounds__(kThreadsRS) mfma_recon_sat_kernel(
+    GeomDev g, PhaseKnobs k, int32_t n_maps, int64_t n_slots, const _Float16 *__restrict__ A16,
+    const float *__restrict__ inv_scale, const double4 *__restrict__ spos, int32_t n_seg,
+    float *__restrict__ part) {
+    using L = SatRecon<NS>;
+    __shared__ _Float16 sA0[4][kMaps][kStage];  // even stages: [re hi, re lo, im hi, im lo][map][slot]
+    __shared__ _Float16 sA1[4][kMaps][kStage];  // odd stages
+    __shared__ RecRow sRow[2][2 * NS];          // [step pair & 1][h * NS + satellite]
+    __shared__ double4 sPix[kWavesRS * kNT];     // the block's pixels (FP64; w: the FP32 copy's
+                                                 // z is not needed, x y z FP32 from pix32)
+    __shared__ float4 sPix32[kWavesRS * kNT];
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int col = lane & 31, kg = lane >> 5;
+    constexpr int kPix = kWavesRS * kNT;  // pixels per block
+    const int64_t n_ptiles = (g.n_pad + kPix - 1) / kPix;
+    const int64_t n_groups = (n_maps + kMaps - 1) / kMaps;
+    const int64_t n_tp = (g.n_t + 1) / 2;
+    const int64_t per_seg = (n_tp + n_seg - 1) / n_seg;
+    const int64_t n_items = n_ptiles * n_groups * n_seg;
+    const double dzo = k.wfac ? 0.0 : 1.0;
+    const float neg1 = opaque_neg1();
+
+    // the rows of step pair tp: 2 NS rows of 4 pieces, threads 0 .. 8 NS - 1;
+    // this thread's piece: row (satellite rs, step 2 tp + rh), 16-byte part rq
+    const int rr_ = threadIdx.x >> 2, rq = threadIdx.x & 3;
+    const int rh = rr_ / NS, rs = rr_ % NS;
+    const uint4 *rsrc = (rq < 2 ? reinterpret_cast<const uint4 *>(spos + (int64_t)rs * g.n_t + rh)
+                                : reinterpret_cast<const uint4 *>(g.cap + (int64_t)rs * g.n_t + rh)) +
+                        (rq & 1);
+    auto rows_load = [&](int64_t tp) {
+        uint4 x = make_uint4(0u, 0u, 0u, 0u);
+        if (threadIdx.x < 8 * NS && 2 * tp + rh < g.n_t) x = rsrc[2 * (2 * tp)];  // 2 pieces per row
+        return x;
+    };
+    auto rows_store = [&](RecRow *dst, uint4 x) {
+        if (threadIdx.x < 8 * NS) reinterpret_cast<uint4 *>(dst)[threadIdx.x] = x;
+    };
+
+    for (int64_t item = blockIdx.x; item < n_items; item += gridDim.x) {
+        const int64_t seg = item % n_seg;
+        const int64_t rest = item / n_seg;
+        const int64_t pt = rest % n_ptiles;
+        const int64_t grp = rest / n_ptiles;
+        const int64_t pi = pt * kPix + wave * kNT + col;
+        const bool live = pt * kPix + wave * kNT < g.n_pad;  // wave-uniform
+        // the block's pixels into LDS (read at each step pair's factors: kept in
+        // registers they would hold 11 of them across every K-step)
+        if (kg == 0) {
+            sPix[wave * kNT + col] = live ? g.pix64[pi] : make_double4(0.0, 0.0, 0.0, 0.0);
+            sPix32[wave * kNT + col] = live ? g.pix32[pi] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        const int64_t tpb = seg * per_seg;
+        const int64_t tpe = tpb + per_seg < n_tp ? tpb + per_seg : n_tp;
+        const _Float16 *Ag = A16 + grp * 4 * kMaps * n_slots;
+        floatx16 acc[2];
+        acc[0] = floatx16{};
+        acc[1] = floatx16{};
+        if (tpb < tpe) {
+            // Both buffers are free: every wave passed the previous item's last barrier.
+            vtile_dma(Ag, n_slots, tpb * L::kStages, &sA0[0][0][0]);
+            rows_store(&sRow[tpb & 1][0], rows_load(tpb));
+            wait_vm();
+            __syncthreads();
+        }
+        for (int64_t tp = tpb; tp < tpe; ++tp) {
+            // this lane's factors at step 2 tp + kg
+            float ec[NS], es[NS];
+            uint32_t occm = 0;
+            {
+                const double4 s64 = sPix[wave * kNT + col];
+                const float4 s32 = sPix32[wave * kNT + col];
+                const double szd = s64.z - dzo;
+                const RecRow *rr = &sRow[tp & 1][kg * NS];
+#pragma unroll
+                for (int s = 0; s < NS; ++s) {
+                    const double4 sp = rr[s].spos;
+                    const CapRec cap = rr[s].cap;
+                    const double ph = fma(sp.x, s64.x, fma(sp.y, s64.y, sp.z * szd));
+                    const float rv = (float)__builtin_amdgcn_fract(ph);
+                    const float x = fmaf(s32.z, cap.dz, fmaf(s32.y, cap.dy, s32.x * cap.dx));
+                    bool occ = x >= cap.hi;
+                    if (x >= cap.lo && !occ) {  // inside the guard band: the exact predicate
+                        const int64_t t = 2 * tp + kg;
+                        occ = t < g.n_t &&
+                              occ64m(g.pos[(int64_t)s * g.n_t + t], s64.x, s64.y, s64.z, g.limb);
+                    }
+                    ec[s] = occ ? 0.0f : __builtin_amdgcn_cosf(rv);
+                    es[s] = occ ? 0.0f : __builtin_amdgcn_sinf(rv);
+                    occm |= (uint32_t)occ << s;
+                    // one satellite at a time: hoisting the row reads of later
+                    // satellites would hold their 64 bytes per lane in registers
+                    if (s % 4 == 3) __builtin_amdgcn_sched_barrier(0);
+                }
+            }
+#pragma unroll
+            for (int sq = 0; sq < L::kStages; ++sq) {
+                const int64_t st = (tp - tpb) * L::kStages + sq;
+                // stage parity = sq & 1 (kStages is even)
+                const _Float16(*sAc)[kMaps][kStage] = (sq & 1) ? sA1 : sA0;
+                const bool more = tp + 1 < tpe || sq + 1 < L::kStages;
+                // the next step pair's rows at its last stage (stored at the end of
+                // the stage), then the next stage's V tile into the other buffer
+                const bool rows_next = sq + 1 == L::kStages && tp + 1 < tpe;
+                const uint4 rn = rows_load(rows_next ? tp + 1 : tp);
+                if (more) vtile_dma(Ag, n_slots, tpb * L::kStages + st + 1, (sq & 1) ? &sA0[0][0][0] : &sA1[0][0][0]);
+                {
+#pragma unroll
+                    for (int kq = 0; kq < 4; ++kq) {
+                        const int ks = sq * 4 + kq;
+                        if (ks >= L::kKT) continue;
+                        float cs[8], sn[8];
+#pragma unroll
+                        for (int e = 0; e < 8; ++e) {
+                            const int j = 8 * ks + e;
+                            const int x = star_x(NS, j), y = star_y(NS, j);
+                            if (x < 0) {
+                                cs[e] = 0.0f;
+                                sn[e] = 0.0f;
+                            } else {
+                                cs[e] = fmaf(ec[y], ec[x], es[y] * es[x]);
+                                sn[e] = fmaf(es[y], ec[x], -(ec[y] * es[x]));
+                            }
+                        }
+                        H8 bre_h, bre_l, bim_h, bim_l;
+#pragma unroll
+                        for (int t2 = 0; t2 < 4; ++t2) {
+                            split2m(cs[2 * t2], cs[2 * t2 + 1], neg1, bre_h.u[t2], bre_l.u[t2]);
+                            split2m(sn[2 * t2], sn[2 * t2 + 1], neg1, bim_h.u[t2], bim_l.u[t2]);
+                        }
+#pragma unroll
+                        for (int mt = 0; mt < 2; ++mt) {
+                            const int row = mt * kMT + col, so = a_slot(row, kq * 2 + kg) * 8;
+                            const half8 arh = *reinterpret_cast<const half8 *>(&sAc[0][row][so]);
+                            const half8 arl = *reinterpret_cast<const half8 *>(&sAc[1][row][so]);
+                            const half8 aih = *reinterpret_cast<const half8 *>(&sAc[2][row][so]);
+                            const half8 ail = *reinterpret_cast<const half8 *>(&sAc[3][row][so]);
+                            acc[mt] = mfma(arh, bre_h.v, acc[mt]);
+                            acc[mt] = mfma(arh, bre_l.v, acc[mt]);
+                            acc[mt] = mfma(arl, bre_h.v, acc[mt]);
+                            acc[mt] = mfma(aih, bim_h.v, acc[mt]);
+                            acc[mt] = mfma(aih, bim_l.v, acc[mt]);
+                            acc[mt] = mfma(ail, bim_h.v, acc[mt]);
+                        }
+                        // K-steps stay apart: interleaving the unrolled K-steps of a stage
+                        // (their operand reads hoisted) costs more registers than it hides
+                        __builtin_amdgcn_sched_barrier(0);
+                    }
+                }
+                // the other buffer was last read before the previous barrier; sRow[(tp + 1) & 1]
+                // was last read when step pair tp - 1 formed its factors
+                if (rows_next) rows_store(&sRow[(tp + 1) & 1][0], rn);
+                wait_vm();
+                __syncthreads();
+            }
+        }
+        if (live) {
+#pragma unroll
+            for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    const int64_t m = grp * kMaps + mt * kMT + acc_row(r, lane);
+                    if (m < n_maps) part[(m * n_seg + seg) * g.n_pad + pi] = acc[mt][r] * inv_scale[m];
+                }
+        }
+    }
+}
+
 }  // namespace
 
 int mfma_vis_pad(int64_t n_vis, int64_t *n_vis_pad) {
@@ -1218,6 +1525,57 @@ hipError_t launch_mfma_visgen_sat(const GeomDev &g, const PhaseKnobs &k, double 
         else launch_sat_visgen_t<4, false>(g, k, sc, n_maps, a16, inv_scale, V, grid, st);
     }
     return hipGetLastError();
+}
+
+int mfma_recon_sat_ns(int32_t n_sat) { return (n_sat == 16 || n_sat == 8) ? n_sat : 0; }
+
+int64_t mfma_recon_sat_slots(int32_t n_sat, int32_t n_t) {
+    const int64_t n_tp = ((int64_t)n_t + 1) / 2;
+    return n_sat == 16 ? n_tp * SatRecon<16>::kSlots : n_tp * SatRecon<8>::kSlots;
+}
+
+template <int NS>
+static hipError_t launch_recon_sat_t(const GeomDev &g, const PhaseKnobs &k, double sc, int32_t n_maps,
+                                     const float2 *V, const float *w, float kappa, int64_t n_slots,
+                                     unsigned int *amax, float *inv_scale, void *a16, double4 *spos,
+                                     int32_t n_seg, float *part, int grid, hipStream_t st) {
+    const int64_t nst = (int64_t)g.n_sat * g.n_t;
+    hipLaunchKernelGGL(scale_pos_kernel, dim3((unsigned)((nst + 255) / 256)), dim3(256), 0, st, nst, g.pos,
+                       sc, spos);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(zero_u32_kernel, dim3((n_maps + 255) / 256), dim3(256), 0, st, amax, n_maps);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(row_absmax_kernel, dim3(64, n_maps), dim3(256), 0, st,
+                       (const float *)V, g.n_vis, 2 * g.n_vis, 1, w, amax);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL((split_vis_sat_kernel<NS>), dim3(2048), dim3(256), 0, st, g.n_vis, g.n_t, n_slots,
+                       n_maps, V, w, kappa, amax, (_Float16 *)a16, inv_scale);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL((mfma_recon_sat_kernel<NS>), dim3(grid), dim3(kThreadsRS), 0, st, g, k, n_maps,
+                       n_slots, (const _Float16 *)a16, inv_scale, (const double4 *)spos, n_seg, part);
+    return hipGetLastError();
+}
+
+hipError_t launch_mfma_recon_sat(const GeomDev &g, const PhaseKnobs &k, double inv_lambda_km,
+                                 int32_t n_maps, const float2 *V, const float *w, float kappa,
+                                 int64_t n_slots, unsigned int *amax, float *inv_scale, void *a16,
+                                 void *spos, int32_t n_seg, float *part, int grid, hipStream_t st) {
+    const double sc = k.s * inv_lambda_km;
+    if (g.n_sat == 16)
+        return launch_recon_sat_t<16>(g, k, sc, n_maps, V, w, kappa, n_slots, amax, inv_scale, a16,
+                                      (double4 *)spos, n_seg, part, grid, st);
+    if (g.n_sat == 8)
+        return launch_recon_sat_t<8>(g, k, sc, n_maps, V, w, kappa, n_slots, amax, inv_scale, a16,
+                                     (double4 *)spos, n_seg, part, grid, st);
+    return hipErrorInvalidValue;
+}
+
+int mfma_recon_sat_threads() { return kThreadsRS; }
+int mfma_recon_sat_pix_per_block() { return kWavesRS * kNT; }
+
+const void *mfma_recon_sat_kernel_ptr(int ns) {
+    return ns == 16 ? (const void *)mfma_recon_sat_kernel<16> : (const void *)mfma_recon_sat_kernel<8>;
 }
 
 const void *mfma_visgen_sat_kernel_ptr(int tb) {

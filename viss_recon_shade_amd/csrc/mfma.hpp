@@ -34,6 +34,19 @@ hipError_t launch_mfma_visgen_sat(const GeomDev &g, const PhaseKnobs &k, double 
                                   int grid, hipStream_t st);
 const void *mfma_visgen_sat_kernel_ptr(int tb);
 
+// Recon on per-satellite factors: the satellite counts it is built for (16, 8;
+// 0 otherwise; the pair list must be the lexicographic one), the operand's slot
+// count, and the launch (spos: n_sat * n_t double4 of workspace).
+int mfma_recon_sat_ns(int32_t n_sat);
+int64_t mfma_recon_sat_slots(int32_t n_sat, int32_t n_t);
+hipError_t launch_mfma_recon_sat(const GeomDev &g, const PhaseKnobs &k, double inv_lambda_km,
+                                 int32_t n_maps, const float2 *V, const float *w, float kappa,
+                                 int64_t n_slots, unsigned int *amax, float *inv_scale, void *a16,
+                                 void *spos, int32_t n_seg, float *part, int grid, hipStream_t st);
+const void *mfma_recon_sat_kernel_ptr(int ns);
+int mfma_recon_sat_threads();
+int mfma_recon_sat_pix_per_block();
+
 const void *mfma_visgen_kernel_ptr();
 const void *mfma_recon_kernel_ptr();
 int mfma_block_threads();         // vis-gen block
