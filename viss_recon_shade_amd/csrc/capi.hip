@@ -1001,10 +1001,11 @@ vrs_status vrs_visgen(vrs_ctx *ctx, const vrs_geom *g, const vrs_knobs *k, int32
             const int grid = persistent_grid(ctx, mfma_visgen_sat_kernel_ptr(tb), n_tiles,
                                              mfma_block_threads());
             Timer tm(ctx, "mfma_visgen");
+            const bool star = sat_recon_ns(g, k) > 0 && mfma_visgen_sat_star(g->n_sat, g->n_base);
             VRS_HIP(ctx, launch_mfma_visgen_sat(g->dev(), phase_knobs(k), g->orb.inv_lambda_km,
                                                 n_maps, g->n_pix, (const float *)sI.dev,
                                                 (unsigned int *)small, (float *)small + n_maps,
-                                                a16, (float2 *)sV.dev, grid, ctx->stream));
+                                                a16, (float2 *)sV.dev, star, grid, ctx->stream));
             tm.stop();
             return sV.finish();
         }

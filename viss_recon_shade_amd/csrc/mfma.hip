@@ -142,11 +142,13 @@ __global__ void zero_u32_kernel(unsigned int *__restrict__ x, int n) {
     if (i < n) x[i] = 0u;
 }
 
-// Per-row scale exponents: e[m] such that max |x[m][:]| * 2^e[m] <= 2^14.
+// Per-row scale exponents: e[m] such that max |x[m][:] - shift| * 2^e[m] <= 2^14
+// (shift applies to real rows only).
 __global__ void __launch_bounds__(256) row_absmax_kernel(const float *__restrict__ x, int64_t n,
                                                          int64_t stride_row, int complex_in,
                                                          const float *__restrict__ w,
-                                                         unsigned int *__restrict__ amax) {
+                                                         unsigned int *__restrict__ amax,
+                                                         float shift) {
     const int64_t row = blockIdx.y;
     const float *xr = x + row * stride_row;
     float m = 0.0f;
@@ -157,7 +159,7 @@ __global__ void __launch_bounds__(256) row_absmax_kernel(const float *__restrict
             m = fmaxf(m, fabsf(xr[2 * i]) * ww);
             m = fmaxf(m, fabsf(xr[2 * i + 1]) * ww);
         } else {
-            m = fmaxf(m, fabsf(xr[i]) * ww);
+            m = fmaxf(m, fabsf(xr[i] - shift) * ww);
         }
     }
     for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
@@ -181,14 +183,15 @@ __device__ __forceinline__ int64_t kstep_pos(int64_t p) {
     return (p & ~int64_t(31)) | ((p & 3) << 3) | ((p >> 2) & 7);
 }
 
-// I [n_maps][n_pix] RING -> hi/lo FP16 [group][2][kMaps][n_pad] chunk order
-// (permuted by kstep_pos inside each K-step when `permute`), scaled.
+// I - shift [n_maps][n_pix] RING -> hi/lo FP16 [group][2][kMaps][n_pad] chunk
+// order (permuted by kstep_pos inside each K-step when `permute`), scaled;
+// padding pixels are zero.
 __global__ void __launch_bounds__(256) split_maps_kernel(GeomDev g, int32_t n_maps, int64_t n_pix,
                                                          const float *__restrict__ I,
                                                          const unsigned int *__restrict__ amax,
                                                          _Float16 *__restrict__ out,
                                                          float *__restrict__ inv_scale,
-                                                         int32_t permute) {
+                                                         int32_t permute, float shift) {
     const int64_t n_groups = (n_maps + kMaps - 1) / kMaps;
     const int64_t total = n_groups * kMaps * g.n_pad;
     for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
@@ -201,7 +204,7 @@ __global__ void __launch_bounds__(256) split_maps_kernel(GeomDev g, int32_t n_ma
         if (m < n_maps) {
             sc = row_scale(amax[m], 1.0f);
             const int r = g.ring[pi];
-            if (r >= 0) x = I[m * n_pix + r] * sc;
+            if (r >= 0) x = (I[m * n_pix + r] - shift) * sc;
             if (pi == 0) inv_scale[m] = 1.0f / sc;
         }
         const _Float16 h = (_Float16)x;  // nearest
@@ -721,6 +724,27 @@ __global__ void __launch_bounds__(kThreadsR, 4) mfma_recon_kernel(
 // bit-exact occultation test) serves every pair that satellite is in: n_sat
 // phases and sin/cos per step instead of n_base.
 
+// Pairs of an n-satellite array (n even) in star order: star c (c = 0 .. n - 2)
+// holds the pairs of satellite c with satellite n - 1 and with c + 1 .. c +
+// (n - 2) / 2 (mod n - 1), so n - 1 stars of n / 2 pairs cover every pair once
+// (a regular tournament on the first n - 1 satellites, each beating the last).
+// Entry j of the list is pair (star_x, star_y), x < y; star_center its star's
+// centre.  Past the n (n - 1) / 2 pairs the entries are padding (-1).
+__host__ __device__ constexpr int star_center(int n, int j) { return j / (n / 2); }
+__host__ __device__ constexpr int star_leaf(int n, int j) {
+    return j % (n / 2) == 0 ? n - 1 : (star_center(n, j) + j % (n / 2)) % (n - 1);
+}
+__host__ __device__ constexpr int star_x(int n, int j) {
+    return j >= n * (n - 1) / 2 ? -1
+                                : (star_center(n, j) < star_leaf(n, j) ? star_center(n, j) : star_leaf(n, j));
+}
+__host__ __device__ constexpr int star_y(int n, int j) {
+    return j >= n * (n - 1) / 2 ? -1
+                                : (star_center(n, j) < star_leaf(n, j) ? star_leaf(n, j) : star_center(n, j));
+}
+// Index of pair (x, y), x < y, in the lexicographic pair list (0,1), (0,2), ...
+__host__ __device__ constexpr int lex_pair(int n, int x, int y) { return x * n - x * (x + 1) / 2 + (y - x - 1); }
+
 // Largest number of (step, satellite) rows one vis-gen block holds per stage.
 template <int TB>
 struct SatRows {
@@ -733,7 +757,9 @@ constexpr int kGramLd = 33;             // floats per row of a staged 32 x 32 Gr
 // x = hi + lo in FP16: hi rounded toward zero, lo = x - hi rounded to nearest.
 // neg1 is -1 hidden from the optimiser (opaque_neg1): fma(hi, neg1, x) is then
 // formed by v_fma_mix_f32, which widens the FP16 operand exactly, instead of a
-// widening conversion and a subtraction (4 VALU per pair instead of 6).
+// widening conversion and a subtraction (4 VALU per pair instead of 6).  Forming
+// each half of lo in place (v_fma_mixlo / v_fma_mixhi, 3 VALU per pair) measured
+// slower: the second half waits for the first.
 __device__ __forceinline__ float opaque_neg1() {
     float m;
     asm("" : "=s"(m) : "0"(-1.0f));
@@ -795,20 +821,29 @@ struct PixStage {
 constexpr int kPixPieces = kStage * 2 + kStage + kStage / 4 + 2;
 static_assert(sizeof(PixStage) == 16 * kPixPieces, "pixel pieces tile the record");
 
-// One load per thread from an address picked without branches (a branch per
-// source would serialise the loads into one register behind waits).  Threads
-// past the pieces reload piece 0 and store nothing.
-__device__ __forceinline__ uint4 pixel_piece(const GeomDev &g, int64_t st) {
+// Where this thread's piece of every stage lies: stage st's piece at base + st *
+// stride (threads past the pieces reload piece 0 and store nothing).  Picked
+// once per kernel, so a stage's load is one address computation (a branch per
+// source at each stage would serialise the loads into one register behind waits).
+struct PixSrc {
+    const char *base;
+    int32_t stride;
+};
+__device__ __forceinline__ PixSrc pixel_src(const GeomDev &g) {
     const int i = threadIdx.x;
-    const int64_t p0 = st * kStage;
-    const uint4 *src = reinterpret_cast<const uint4 *>(g.pix64 + p0) + (i < 2 * kStage ? i : 0);
-    src = i >= 2 * kStage && i < 3 * kStage
-              ? reinterpret_cast<const uint4 *>(g.pix32 + p0) + (i - 2 * kStage) : src;
-    src = i >= 3 * kStage && i < 3 * kStage + kStage / 4
-              ? reinterpret_cast<const uint4 *>(g.ring + p0) + (i - 3 * kStage) : src;
-    src = i >= 3 * kStage + kStage / 4 && i < kPixPieces
-              ? reinterpret_cast<const uint4 *>(g.sub + st) + (i - 3 * kStage - kStage / 4) : src;
-    return *src;
+    if (i < 2 * kStage) return {reinterpret_cast<const char *>(g.pix64) + 16 * i, kStage * 32};
+    if (i < 3 * kStage) return {reinterpret_cast<const char *>(g.pix32) + 16 * (i - 2 * kStage), kStage * 16};
+    if (i < 3 * kStage + kStage / 4)
+        return {reinterpret_cast<const char *>(g.ring) + 16 * (i - 3 * kStage), kStage * 4};
+    if (i < kPixPieces)
+        return {reinterpret_cast<const char *>(g.sub) + 16 * (i - 3 * kStage - kStage / 4),
+                (int32_t)sizeof(ChunkRec)};
+    return {reinterpret_cast<const char *>(g.pix64), kStage * 32};
+}
+static_assert(sizeof(double4) == 32 && sizeof(float4) == 16 && sizeof(ChunkRec) == 32,
+              "pixel piece strides");
+__device__ __forceinline__ uint4 pixel_piece(const PixSrc &src, int64_t st) {
+    return *reinterpret_cast<const uint4 *>(src.base + st * src.stride);
 }
 __device__ __forceinline__ void pixel_store(PixStage *d, uint4 x) {
     if (threadIdx.x < kPixPieces) reinterpret_cast<uint4 *>(d)[threadIdx.x] = x;
@@ -822,14 +857,18 @@ __device__ __forceinline__ void pixel_store(PixStage *d, uint4 x) {
 // needs no padding test: the map operand is zero on them).  Occultation per pixel: the FP32 test
 // against the guard band's edges, the exact FP64 predicate inside the band (the
 // rule of every kernel here; a per-pixel test costs fewer instructions than
-// classifying the row's cap against the 64-pixel group first).
-template <bool FULL, int MR>
+// classifying the row's cap against the 64-pixel group first).  With OCC, bit p
+// of sOc[r] is set where pixel p of the stage is occulted for the row or is
+// padding (where the row contributes nothing to any product).
+template <bool FULL, bool OCC, int MR>
 __device__ __forceinline__ void sat_rows(const GeomDev &g, double dzo, int wave, uint32_t live,
                                          const PixStage *ps, const SatRow *sRow,
-                                         float2 (*sEm)[kSatLd], float2 (*sEf)[kSatLd]) {
+                                         float2 (*sEm)[kSatLd], float2 (*sEf)[kSatLd],
+                                         uint64_t *sOc) {
     const int lane = threadIdx.x & 63;
     const double4 s = ps->p64[lane];
     const float4 s32 = ps->p32[lane];
+    const bool pad = (FULL || OCC) && ps->ring[lane] < 0;
     const double szd = s.z - dzo;
 #pragma unroll
     for (int i = 0; i < MR; ++i) {
@@ -840,12 +879,16 @@ __device__ __forceinline__ void sat_rows(const GeomDev &g, double dzo, int wave,
         const double ph = fma(sp.x, s.x, fma(sp.y, s.y, sp.z * szd));
         const float rv = (float)__builtin_amdgcn_fract(ph);
         float2 ef = make_float2(__builtin_amdgcn_cosf(rv), __builtin_amdgcn_sinf(rv));
-        if (FULL && ps->ring[lane] < 0) ef = make_float2(0.f, 0.f);
+        if (FULL && pad) ef = make_float2(0.f, 0.f);
         const float x = fmaf(s32.z, cap.dz, fmaf(s32.y, cap.dy, s32.x * cap.dx));
         bool occ = x >= cap.hi;
         if (x >= cap.lo && !occ) occ = occ64m(sRow[r].pos, s.x, s.y, s.z, g.limb);
         sEm[r][lane] = occ ? make_float2(0.f, 0.f) : ef;
         if (FULL) sEf[r][lane] = ef;
+        if (OCC) {
+            const uint64_t bm = __ballot(occ || pad);
+            if (lane == 0) sOc[r] = bm;
+        }
     }
 }
 
@@ -854,24 +897,27 @@ __device__ __forceinline__ void sat_rows(const GeomDev &g, double dzo, int wave,
 constexpr int kGramLd16 = 17;
 
 // Vis-gen on per-satellite factors.  Block: TB consecutive steps of every
-// baseline (TB * n_base <= 240 visibilities, slot j = step (j / n_base),
-// baseline (j % n_base)); waves 0-14 hold 16 slots each on the 16x16x32
-// instruction (the products of mfma_visgen_kernel, with E'_xy = E'_y conj(E'_x)
-// formed from the two staged rows of each lane's pair); every wave forms its
-// share of the next stage's rows after its K-steps.
+// baseline (TB * n_base <= 240 visibilities); waves 0-14 hold 16 slots each on
+// the 16x16x32 instruction (the products of mfma_visgen_kernel, with E'_xy =
+// E'_y conj(E'_x) formed from the two staged rows of each lane's pair); every
+// wave forms its share of the next stage's rows after its K-steps.  Slots: j =
+// step (j / n_base), baseline (j % n_base); with STAR (every pair of 16 or 8
+// satellites, TB n_sat = 32), wave w holds star w of the star order at each of
+// the TB steps, so all its pairs hold satellite w, and it skips the K-steps on
+// whose 32 pixels satellite w is occulted at every step: all its products
+// there are zero.
 //
-// Shaded-pixel term (TOCC): t_occ sum_{p shaded} E_xy = t_occ (U_xy - U'_xy),
-// with U the full-sky sum sum_p E_y conj(E_x) and U' the same over clear pixels.
-// U' is the sum of the lane's own clear entries (two adds per entry); U is a
-// Gram matrix of the unmasked rows of a step, which wave 15 contracts on the
-// matrix cores: with C, S the cos and
-// sin rows of the 16 satellites, Re U_xy = P1[y][x] = (C C^T + S S^T)[y][x],
-// and sum_p sin(phi_y - phi_x) = P2[x][y] - P2[y][x], P2 = C S^T.  Split FP16:
-// P1 = K + K^T with K = Hc Hc^T / 2 + Hc Lc^T + Hs Hs^T / 2 + Hs Ls^T, and
-// P2 = Hc Hs^T + Hc Ls^T + Lc Hs^T.  Each wave role runs its own stage loop
-// (with the same barriers), so the Gram accumulators and the consumers'
-// accumulators never share registers.
-template <int TB, bool TOCC>
+// Shaded-pixel term (TOCC): V = sum_clear I E + t_occ sum_shaded E =
+// sum_clear (I - t_occ) E + t_occ U, with U_xy = sum_p E_y conj(E_x) over every
+// pixel; the map operand holds I - t_occ.  U is a Gram matrix of the unmasked
+// rows of a step, which wave 15 contracts on the matrix cores: with C, S the
+// cos and sin rows of the 16 satellites, Re U_xy = P1[y][x] = (C C^T + S
+// S^T)[y][x], and sum_p sin(phi_y - phi_x) = P2[x][y] - P2[y][x], P2 = C S^T.
+// Split FP16: P1 = K + K^T with K = Hc Hc^T / 2 + Hc Lc^T + Hs Hs^T / 2 + Hs
+// Ls^T, and P2 = Hc Hs^T + Hc Ls^T + Lc Hs^T.  Each wave role runs its own
+// stage loop (with the same barriers), so the Gram accumulators and the
+// consumers' accumulators never share registers.
+template <int TB, bool TOCC, bool STAR>
 __global__ void __launch_bounds__(kThreads) mfma_visgen_sat_kernel(
     GeomDev g, PhaseKnobs k, double sc, int32_t n_maps, const _Float16 *__restrict__ A16,
     const float *__restrict__ inv_scale, float2 *__restrict__ V) {
@@ -881,6 +927,7 @@ __global__ void __launch_bounds__(kThreads) mfma_visgen_sat_kernel(
     __shared__ float sG[TOCC ? TB : 1][2][16][kGramLd16];
     __shared__ SatRow sRow[R];
     __shared__ PixStage sPix[2];  // [stage & 1]
+    __shared__ uint64_t sOcc[STAR ? 2 : 1][R];  // [buffer][row]: occultation bits (STAR)
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int col = lane & 15, kg = lane >> 4;
@@ -903,68 +950,90 @@ __global__ void __launch_bounds__(kThreads) mfma_visgen_sat_kernel(
     for (int i = 0; i < MR; ++i) rtl[i] = (wave + kWaves * i) / n_sat;
     // this lane's slot (fixed over the tiles)
     const int j = wave * kNTV + col;
-    const int tlv = j / n_base, bi = j - tlv * n_base;
-    int rx = 0, ry = 0, px = 0, py = 0;
+    int tlv, bi, rx = 0, ry = 0, px = 0, py = 0;
+    if (STAR) {
+        const int h = n_sat / 2;  // pairs per star
+        tlv = col / h;
+        const int js = wave * h + col % h;
+        px = consumer ? star_x(n_sat, js) : 0;
+        py = consumer ? star_y(n_sat, js) : 0;
+        bi = lex_pair(n_sat, px, py);
+    } else {
+        tlv = j / n_base;
+        bi = j - tlv * n_base;
+        if (consumer && j < n_slot) {
+            const int2 pr = g.pairs[bi];
+            px = pr.x;
+            py = pr.y;
+        }
+    }
     if (consumer && j < n_slot) {
-        const int2 pr = g.pairs[bi];
-        px = pr.x;
-        py = pr.y;
         rx = tlv * n_sat + px;
         ry = tlv * n_sat + py;
     }
     const float neg1 = opaque_neg1();
+    const PixSrc psrc = pixel_src(g);
 
-    for (int64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
-        const int64_t grp = tile / n_tb;
-        const int t0 = (int)(tile % n_tb) * TB;
-        const bool vok = consumer && j < n_slot && t0 + tlv < g.n_t;
-        const int64_t v = (int64_t)bi * g.n_t + t0 + tlv;
-        const _Float16 *Ag = A16 + grp * 2 * kMaps * g.n_pad;
-        uint32_t live = 0;  // this wave's rows whose step exists
+    // Each wave role runs its own tile loop with the same barriers (two per tile
+    // set-up, one per stage, one before the results with TOCC), so that the
+    // registers of one role are not held live through another's.
+    auto tile_of = [&](int64_t tile, int64_t &grp, int &t0, uint32_t &live) {
+        grp = tile / n_tb;
+        t0 = (int)(tile % n_tb) * TB;
+        live = 0;  // this wave's rows whose step exists
 #pragma unroll
         for (int i = 0; i < MR; ++i)
             if (i < nr && t0 + rtl[i] < g.n_t) live |= 1u << i;
-        {
-            // The buffers, sRow and sPix are free: every wave passed the previous
-            // tile's last barrier, and the last stage forms no rows.
-            if (threadIdx.x < n_rows) {
-                const int r = threadIdx.x;
-                const int tl = r / n_sat, sat = r - tl * n_sat;
-                SatRow e;
-                if (t0 + tl < g.n_t) {
-                    const int64_t row = (int64_t)sat * g.n_t + t0 + tl;
-                    e.pos = g.pos[row];
-                    e.cap = g.cap[row];
-                } else {
-                    e.pos = make_double4(0.0, 0.0, 0.0, 0.0);
-                    e.cap = {0.f, 0.f, 0.f, 2.0f, 0.f, kGuardMin, 2.0f - kGuardMin, 2.0f + kGuardMin};
-                }
-                e.spos = make_double4(e.pos.x * sc, e.pos.y * sc, e.pos.z * sc, 0.0);
-                sRow[r] = e;
+    };
+    auto tile_begin = [&](int t0, uint32_t live, const _Float16 *Ag) {
+        // The buffers, sRow and sPix are free: every wave passed the previous
+        // tile's last barrier, and the last stage forms no rows.
+        if (threadIdx.x < n_rows) {
+            const int r = threadIdx.x;
+            const int tl = r / n_sat, sat = r - tl * n_sat;
+            SatRow e;
+            if (t0 + tl < g.n_t) {
+                const int64_t row = (int64_t)sat * g.n_t + t0 + tl;
+                e.pos = g.pos[row];
+                e.cap = g.cap[row];
+            } else {
+                e.pos = make_double4(0.0, 0.0, 0.0, 0.0);
+                e.cap = {0.f, 0.f, 0.f, 2.0f, 0.f, kGuardMin, 2.0f - kGuardMin, 2.0f + kGuardMin};
+                if (STAR) sOcc[0][r] = sOcc[STAR ? 1 : 0][r] = ~0ull;  // no step: skippable
             }
-            tile_store(&sA[0][0][0][0], tile_load(Ag, g.n_pad, 0));
-            pixel_store(&sPix[0], pixel_piece(g, 0));
-            if (n_stages > 1) pixel_store(&sPix[1], pixel_piece(g, 1));
-            __syncthreads();
-            sat_rows<TOCC, MR>(g, dzo, wave, live, &sPix[0], sRow, sE[0][0], sE[0][TOCC ? 1 : 0]);
-            __syncthreads();
+            e.spos = make_double4(e.pos.x * sc, e.pos.y * sc, e.pos.z * sc, 0.0);
+            sRow[r] = e;
         }
-        // Prefetch into registers, stored at the end of each stage: the map tile
-        // of stage st + 1 and the pixel inputs of stage st + 2 (loaded without a
-        // branch: the last stages reload their own).  Nothing in a stage waits
-        // for them.
-        auto stage_end = [&](int64_t st, uint4 tnext, uint4 pnext) {
-            const int buf = (int)(st & 1);
-            // sA[buf ^ 1] was last read before the previous barrier; sPix[buf] holds
-            // stage st's inputs, last read while forming stage st
-            asm volatile("" ::: "memory");
-            if (st + 1 < n_stages) tile_store(&sA[buf ^ 1][0][0][0], tnext);
-            if (st + 2 < n_stages) pixel_store(&sPix[buf], pnext);
-            __syncthreads();
-        };
-        float2 vout[kMaps / kMTV][4];  // consumer results, written after the shaded term
-        float clr_c = 0.0f, clr_s = 0.0f;  // sum of the lane's clear entries (TOCC)
-        if (consumer) {
+        tile_store(&sA[0][0][0][0], tile_load(Ag, g.n_pad, 0));
+        pixel_store(&sPix[0], pixel_piece(psrc, 0));
+        if (n_stages > 1) pixel_store(&sPix[1], pixel_piece(psrc, 1));
+        __syncthreads();
+        sat_rows<TOCC, STAR, MR>(g, dzo, wave, live, &sPix[0], sRow, sE[0][0], sE[0][TOCC ? 1 : 0],
+                                 sOcc[0]);
+        __syncthreads();
+    };
+    // Prefetch into registers, stored at the end of each stage: the map tile
+    // of stage st + 1 and the pixel inputs of stage st + 2 (loaded without a
+    // branch: the last stages reload their own).  Nothing in a stage waits
+    // for them.
+    auto stage_end = [&](int64_t st, uint4 tnext, uint4 pnext) {
+        const int buf = (int)(st & 1);
+        // sA[buf ^ 1] was last read before the previous barrier; sPix[buf] holds
+        // stage st's inputs, last read while forming stage st
+        asm volatile("" ::: "memory");
+        if (st + 1 < n_stages) tile_store(&sA[buf ^ 1][0][0][0], tnext);
+        if (st + 2 < n_stages) pixel_store(&sPix[buf], pnext);
+        __syncthreads();
+    };
+
+    if (consumer) {
+        for (int64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+            int64_t grp;
+            int t0;
+            uint32_t live;
+            tile_of(tile, grp, t0, live);
+            const _Float16 *Ag = A16 + grp * 2 * kMaps * g.n_pad;
+            tile_begin(t0, live, Ag);
             floatx4 acc[kMaps / kMTV][2];
 #pragma unroll
             for (int a = 0; a < kMaps / kMTV; ++a)
@@ -973,7 +1042,16 @@ __global__ void __launch_bounds__(kThreads) mfma_visgen_sat_kernel(
             for (int64_t st = 0; st < n_stages; ++st) {
                 const int buf = (int)(st & 1);
                 const uint4 tnext = tile_load(Ag, g.n_pad, st + 1 < n_stages ? st + 1 : st);
-                const uint4 pnext = pixel_piece(g, st + 2 < n_stages ? st + 2 : st);
+                const uint4 pnext = pixel_piece(psrc, st + 2 < n_stages ? st + 2 : st);
+                // K-steps whose products are all zero (STAR): bit ks
+                uint32_t skip = 0;
+                if (STAR) {
+                    uint64_t m = ~0ull;
+#pragma unroll
+                    for (int tl = 0; tl < TB; ++tl) m &= sOcc[STAR ? buf : 0][tl * n_sat + wave];
+                    const uint32_t sk = ((uint32_t)m == ~0u ? 1u : 0u) | ((uint32_t)(m >> 32) == ~0u ? 2u : 0u);
+                    skip = __builtin_amdgcn_readfirstlane(sk);
+                }
                 // E of K-step ks for this lane's 8 pixels (ks * 32 + 8 kg + i, identity
                 // K order), read one K-step ahead so that the reads of K-step ks + 1
                 // are in flight while the products of K-step ks issue
@@ -990,6 +1068,10 @@ __global__ void __launch_bounds__(kThreads) mfma_visgen_sat_kernel(
                 read_e(0);
 #pragma unroll
                 for (int ks = 0; ks < kStage / kKSV; ++ks) {
+                    if ((skip >> ks) & 1u) {  // wave-uniform
+                        if (ks + 1 < kStage / kKSV) read_e(ks + 1);
+                        continue;
+                    }
                     float cs[8], sn[8];
 #pragma unroll
                     for (int q = 0; q < 4; ++q) {
@@ -1000,13 +1082,6 @@ __global__ void __launch_bounds__(kThreads) mfma_visgen_sat_kernel(
                         sn[2 * q + 1] = fmaf(b.w, a.z, -(b.z * a.w));
                     }
                     if (ks + 1 < kStage / kKSV) read_e(ks + 1);
-                    if (TOCC) {
-#pragma unroll
-                        for (int t = 0; t < 8; ++t) {
-                            clr_c += cs[t];
-                            clr_s += sn[t];
-                        }
-                    }
                     H8 bre_h, bre_l, bim_h, bim_l;
 #pragma unroll
                     for (int t2 = 0; t2 < 4; ++t2) {
@@ -1027,142 +1102,125 @@ __global__ void __launch_bounds__(kThreads) mfma_visgen_sat_kernel(
                     }
                 }
                 if (st + 1 < n_stages)
-                    sat_rows<TOCC, MR>(g, dzo, wave, live, &sPix[buf ^ 1], sRow, sE[buf ^ 1][0],
-                             sE[buf ^ 1][TOCC ? 1 : 0]);
+                    sat_rows<TOCC, STAR, MR>(g, dzo, wave, live, &sPix[buf ^ 1], sRow, sE[buf ^ 1][0],
+                                             sE[buf ^ 1][TOCC ? 1 : 0], sOcc[STAR ? buf ^ 1 : 0]);
                 stage_end(st, tnext, pnext);
             }
-#pragma unroll
-            for (int mt = 0; mt < kMaps / kMTV; ++mt)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int64_t m = grp * kMaps + mt * kMTV + 4 * kg + r;
-                    const float is = m < n_maps ? inv_scale[m] : 0.0f;
-                    vout[mt][r] = make_float2(acc[mt][0][r] * is, -(acc[mt][1][r] * is));
+            if (TOCC) __syncthreads();  // sG whole
+            if (consumer && j < n_slot && t0 + tlv < g.n_t) {
+                const int64_t v = (int64_t)bi * g.n_t + t0 + tlv;
+                float shc = 0.0f, shs = 0.0f;
+                if (TOCC) {
+                    // t_occ times the pair's full-sky sum
+                    const float(*G)[16][kGramLd16] = sG[tlv];
+                    shc = t_occ * (G[0][py][px] + G[0][px][py]);
+                    shs = t_occ * (G[1][px][py] - G[1][py][px]);
                 }
-        } else if (gram) {
-            // this lane: satellite gs = lane % 16 (its row and column of the Gram
-            // blocks), pixels 8 kg + i of each 32-pixel K-step
+#pragma unroll
+                for (int mt = 0; mt < kMaps / kMTV; ++mt)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int64_t m = grp * kMaps + mt * kMTV + 4 * kg + r;
+                        // V = sum_clear (I - t_occ) E + t_occ U, E = cos - i sin
+                        if (m < n_maps) {
+                            const float is = inv_scale[m];
+                            V[m * g.n_vis + v] = make_float2(acc[mt][0][r] * is + shc,
+                                                             -(acc[mt][1][r] * is) - shs);
+                        }
+                    }
+            }
+        }
+    } else if (gram) {
+        // this lane: satellite gs = lane % 16 (its row and column of the Gram
+        // blocks), pixels 8 kg + i of each 32-pixel K-step
+        const int gs = lane & 15;
+        for (int64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+            int64_t grp;
+            int t0;
+            uint32_t live;
+            tile_of(tile, grp, t0, live);
+            const _Float16 *Ag = A16 + grp * 2 * kMaps * g.n_pad;
+            tile_begin(t0, live, Ag);
             floatx4 p1[TB], p2[TB];
 #pragma unroll
             for (int q = 0; q < TB; ++q) p1[q] = p2[q] = floatx4{};
-            const int gs = lane & 15;
             for (int64_t st = 0; st < n_stages; ++st) {
                 const int buf = (int)(st & 1);
                 const uint4 tnext = tile_load(Ag, g.n_pad, st + 1 < n_stages ? st + 1 : st);
-                const uint4 pnext = pixel_piece(g, st + 2 < n_stages ? st + 2 : st);
+                const uint4 pnext = pixel_piece(psrc, st + 2 < n_stages ? st + 2 : st);
 #pragma unroll
                 for (int q = 0; q < TB; ++q) {
                     if (t0 + q >= g.n_t) continue;  // wave-uniform
 #pragma unroll 1
                     for (int kk = 0; kk < kStage / kKSV; ++kk) {
-                        // one Gram block pair from one staged row set (full, then masked)
-                        auto gram_rows = [&](const float2(*rows)[kSatLd], floatx4 &a1, floatx4 &a2) {
-                            float cc[8], ss[8];
-                            if (gs < n_sat) {
-                                const float2 *e = &rows[q * n_sat + gs][kk * kKSV + kg * 8];
-#pragma unroll
-                                for (int i = 0; i < 4; ++i) {
-                                    const float4 a = *reinterpret_cast<const float4 *>(e + 2 * i);
-                                    cc[2 * i] = a.x; ss[2 * i] = a.y; cc[2 * i + 1] = a.z; ss[2 * i + 1] = a.w;
-                                }
-                            } else {
-#pragma unroll
-                                for (int i = 0; i < 8; ++i) cc[i] = ss[i] = 0.0f;
-                            }
-                            H8 hc, lc, hs, ls, hc2, hs2;
+                        // one Gram block pair from the step's unmasked rows
+                        float cc[8], ss[8];
+                        if (gs < n_sat) {
+                            const float2 *e = &sE[buf][TOCC ? 1 : 0][q * n_sat + gs][kk * kKSV + kg * 8];
 #pragma unroll
                             for (int i = 0; i < 4; ++i) {
-                                split2m(cc[2 * i], cc[2 * i + 1], neg1, hc.u[i], lc.u[i]);
-                                split2m(ss[2 * i], ss[2 * i + 1], neg1, hs.u[i], ls.u[i]);
+                                const float4 a = *reinterpret_cast<const float4 *>(e + 2 * i);
+                                cc[2 * i] = a.x; ss[2 * i] = a.y; cc[2 * i + 1] = a.z; ss[2 * i + 1] = a.w;
                             }
-                            hc2.v = hc.v * (_Float16)0.5f;
-                            hs2.v = hs.v * (_Float16)0.5f;
-                            a1 = mfma16(hc2.v, hc.v, a1);
-                            a1 = mfma16(hc.v, lc.v, a1);
-                            a1 = mfma16(hs2.v, hs.v, a1);
-                            a1 = mfma16(hs.v, ls.v, a1);
-                            a2 = mfma16(hc.v, hs.v, a2);
-                            a2 = mfma16(hc.v, ls.v, a2);
-                            a2 = mfma16(lc.v, hs.v, a2);
-                        };
-                        gram_rows(sE[buf][TOCC ? 1 : 0], p1[q], p2[q]);
+                        } else {
+#pragma unroll
+                            for (int i = 0; i < 8; ++i) cc[i] = ss[i] = 0.0f;
+                        }
+                        H8 hc, lc, hs, ls, hc2, hs2;
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) {
+                            split2m(cc[2 * i], cc[2 * i + 1], neg1, hc.u[i], lc.u[i]);
+                            split2m(ss[2 * i], ss[2 * i + 1], neg1, hs.u[i], ls.u[i]);
+                        }
+                        hc2.v = hc.v * (_Float16)0.5f;
+                        hs2.v = hs.v * (_Float16)0.5f;
+                        p1[q] = mfma16(hc2.v, hc.v, p1[q]);
+                        p1[q] = mfma16(hc.v, lc.v, p1[q]);
+                        p1[q] = mfma16(hs2.v, hs.v, p1[q]);
+                        p1[q] = mfma16(hs.v, ls.v, p1[q]);
+                        p2[q] = mfma16(hc.v, hs.v, p2[q]);
+                        p2[q] = mfma16(hc.v, ls.v, p2[q]);
+                        p2[q] = mfma16(lc.v, hs.v, p2[q]);
                     }
                 }
                 if (st + 1 < n_stages)
-                    sat_rows<TOCC, MR>(g, dzo, wave, live, &sPix[buf ^ 1], sRow, sE[buf ^ 1][0],
-                             sE[buf ^ 1][TOCC ? 1 : 0]);
+                    sat_rows<TOCC, STAR, MR>(g, dzo, wave, live, &sPix[buf ^ 1], sRow, sE[buf ^ 1][0],
+                                             sE[buf ^ 1][TOCC ? 1 : 0], sOcc[STAR ? buf ^ 1 : 0]);
                 stage_end(st, tnext, pnext);
             }
-            if (TOCC) {
-                // D of 16x16x32: lane holds rows 4 (lane / 16) + r of column lane % 16
+            // D of 16x16x32: lane holds rows 4 (lane / 16) + r of column lane % 16
 #pragma unroll
-                for (int q = 0; q < TB; ++q)
+            for (int q = 0; q < TB; ++q)
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        const int a = 4 * kg + r;
-                        sG[q][0][a][gs] = p1[q][r];
-                        sG[q][1][a][gs] = p2[q][r];
-                    }
-            }
-        } else {
+                for (int r = 0; r < 4; ++r) {
+                    const int a = 4 * kg + r;
+                    sG[q][0][a][gs] = p1[q][r];
+                    sG[q][1][a][gs] = p2[q][r];
+                }
+            __syncthreads();  // sG whole
+        }
+    } else {
+        for (int64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+            int64_t grp;
+            int t0;
+            uint32_t live;
+            tile_of(tile, grp, t0, live);
+            const _Float16 *Ag = A16 + grp * 2 * kMaps * g.n_pad;
+            tile_begin(t0, live, Ag);
             for (int64_t st = 0; st < n_stages; ++st) {
                 const int buf = (int)(st & 1);
                 const uint4 tnext = tile_load(Ag, g.n_pad, st + 1 < n_stages ? st + 1 : st);
-                const uint4 pnext = pixel_piece(g, st + 2 < n_stages ? st + 2 : st);
+                const uint4 pnext = pixel_piece(psrc, st + 2 < n_stages ? st + 2 : st);
                 if (st + 1 < n_stages)
-                    sat_rows<TOCC, MR>(g, dzo, wave, live, &sPix[buf ^ 1], sRow, sE[buf ^ 1][0],
-                             sE[buf ^ 1][TOCC ? 1 : 0]);
+                    sat_rows<TOCC, STAR, MR>(g, dzo, wave, live, &sPix[buf ^ 1], sRow, sE[buf ^ 1][0],
+                                             sE[buf ^ 1][TOCC ? 1 : 0], sOcc[STAR ? buf ^ 1 : 0]);
                 stage_end(st, tnext, pnext);
             }
-        }
-        if (TOCC) __syncthreads();  // sG whole
-        if (TOCC) {
-            clr_c += __shfl_xor(clr_c, 16, 64);
-            clr_s += __shfl_xor(clr_s, 16, 64);
-            clr_c += __shfl_xor(clr_c, 32, 64);
-            clr_s += __shfl_xor(clr_s, 32, 64);
-        }
-        if (vok) {
-            float shc = 0.0f, shs = 0.0f;
-            if (TOCC) {
-                // shaded sums: the pair's full-sky Gram sums minus its clear entries
-                const float(*G)[16][kGramLd16] = sG[tlv];
-                shc = t_occ * (G[0][py][px] + G[0][px][py] - clr_c);
-                shs = t_occ * (G[1][px][py] - G[1][py][px] - clr_s);
-            }
-#pragma unroll
-            for (int mt = 0; mt < kMaps / kMTV; ++mt)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int64_t m = grp * kMaps + mt * kMTV + 4 * kg + r;
-                    // V = sum_clear I E + t_occ sum_shaded E, E = cos - i sin
-                    if (m < n_maps)
-                        V[m * g.n_vis + v] = make_float2(vout[mt][r].x + shc, vout[mt][r].y - shs);
-                }
+            if (TOCC) __syncthreads();  // sG whole
         }
     }
 }
 // ------------------------------------------------------------------ recon on per-satellite factors
-// Pairs of an n-satellite array (n even) in star order: star c (c = 0 .. n - 2)
-// holds the pairs of satellite c with satellite n - 1 and with c + 1 .. c +
-// (n - 2) / 2 (mod n - 1), so n - 1 stars of n / 2 pairs cover every pair once
-// (a regular tournament on the first n - 1 satellites, each beating the last).
-// Entry j of the list is pair (star_x, star_y), x < y; star_center its star's
-// centre.  Past the n (n - 1) / 2 pairs the entries are padding (-1).
-__host__ __device__ constexpr int star_center(int n, int j) { return j / (n / 2); }
-__host__ __device__ constexpr int star_leaf(int n, int j) {
-    return j % (n / 2) == 0 ? n - 1 : (star_center(n, j) + j % (n / 2)) % (n - 1);
-}
-__host__ __device__ constexpr int star_x(int n, int j) {
-    return j >= n * (n - 1) / 2 ? -1
-                                : (star_center(n, j) < star_leaf(n, j) ? star_center(n, j) : star_leaf(n, j));
-}
-__host__ __device__ constexpr int star_y(int n, int j) {
-    return j >= n * (n - 1) / 2 ? -1
-                                : (star_center(n, j) < star_leaf(n, j) ? star_leaf(n, j) : star_center(n, j));
-}
-// Index of pair (x, y), x < y, in the lexicographic pair list (0,1), (0,2), ...
-__host__ __device__ constexpr int lex_pair(int n, int x, int y) { return x * n - x * (x + 1) / 2 + (y - x - 1); }
 
 // Layout of the recon operand over (step pair, pair): per step pair tp (steps
 // 2 tp and 2 tp + 1) kKTP K-steps of 16 slots; slot 16 ks + 8 h + e holds pair
@@ -1473,10 +1531,10 @@ hipError_t launch_mfma_visgen(const GeomDev &g, const PhaseKnobs &k, int32_t n_m
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(row_absmax_kernel, dim3(64, n_maps), dim3(256), 0, st, I, n_pix, n_pix, 0,
-                       (const float *)nullptr, amax);
+                       (const float *)nullptr, amax, 0.0f);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(split_maps_kernel, dim3(2048), dim3(256), 0, st, g, n_maps, n_pix, I, amax,
-                       (_Float16 *)a16, inv_scale, 1);
+                       (_Float16 *)a16, inv_scale, 1, 0.0f);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(mfma_visgen_kernel, dim3(grid), dim3(kThreads), 0, st, g, k, n_maps,
                        (const _Float16 *)a16, inv_scale, V);
@@ -1494,35 +1552,46 @@ int mfma_sat_tb(int32_t n_sat, int32_t n_base) {
 
 template <int TB, bool TOCC>
 static void launch_sat_visgen_t(const GeomDev &g, const PhaseKnobs &k, double sc, int32_t n_maps,
-                                const void *a16, const float *inv_scale, float2 *V, int grid,
-                                hipStream_t st) {
-    hipLaunchKernelGGL((mfma_visgen_sat_kernel<TB, TOCC>), dim3(grid), dim3(kThreads), 0, st, g, k,
-                       sc, n_maps, (const _Float16 *)a16, inv_scale, V);
+                                const void *a16, const float *inv_scale, float2 *V, bool star,
+                                int grid, hipStream_t st) {
+    if (star)
+        hipLaunchKernelGGL((mfma_visgen_sat_kernel<TB, TOCC, true>), dim3(grid), dim3(kThreads), 0, st,
+                           g, k, sc, n_maps, (const _Float16 *)a16, inv_scale, V);
+    else
+        hipLaunchKernelGGL((mfma_visgen_sat_kernel<TB, TOCC, false>), dim3(grid), dim3(kThreads), 0, st,
+                           g, k, sc, n_maps, (const _Float16 *)a16, inv_scale, V);
+}
+
+int mfma_visgen_sat_star(int32_t n_sat, int32_t n_base) {
+    return (n_sat == 16 || n_sat == 8) && n_base == n_sat * (n_sat - 1) / 2 &&
+           mfma_sat_tb(n_sat, n_base) * n_sat == 2 * kNTV;
 }
 
 hipError_t launch_mfma_visgen_sat(const GeomDev &g, const PhaseKnobs &k, double inv_lambda_km,
                                   int32_t n_maps, int64_t n_pix, const float *I,
                                   unsigned int *amax, float *inv_scale, void *a16, float2 *V,
-                                  int grid, hipStream_t st) {
+                                  bool star, int grid, hipStream_t st) {
     const int tb = mfma_sat_tb(g.n_sat, g.n_base);
-    if (tb == 0) return hipErrorInvalidValue;
+    if (tb == 0 || (star && !mfma_visgen_sat_star(g.n_sat, g.n_base))) return hipErrorInvalidValue;
+    // maps I - t_occ: V = sum_clear (I - t_occ) E + t_occ sum_all E
+    const float shift = k.t_occ;
     hipLaunchKernelGGL(zero_u32_kernel, dim3((n_maps + 255) / 256), dim3(256), 0, st, amax, n_maps);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(row_absmax_kernel, dim3(64, n_maps), dim3(256), 0, st, I, n_pix, n_pix, 0,
-                       (const float *)nullptr, amax);
+                       (const float *)nullptr, amax, shift);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(split_maps_kernel, dim3(2048), dim3(256), 0, st, g, n_maps, n_pix, I, amax,
-                       (_Float16 *)a16, inv_scale, 0);
+                       (_Float16 *)a16, inv_scale, 0, shift);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     const double sc = k.s * inv_lambda_km;
     const bool tocc = k.t_occ != 0.0f;
     if (tb == 2) {
-        if (tocc) launch_sat_visgen_t<2, true>(g, k, sc, n_maps, a16, inv_scale, V, grid, st);
-        else launch_sat_visgen_t<2, false>(g, k, sc, n_maps, a16, inv_scale, V, grid, st);
+        if (tocc) launch_sat_visgen_t<2, true>(g, k, sc, n_maps, a16, inv_scale, V, star, grid, st);
+        else launch_sat_visgen_t<2, false>(g, k, sc, n_maps, a16, inv_scale, V, star, grid, st);
     } else {
-        if (tocc) launch_sat_visgen_t<4, true>(g, k, sc, n_maps, a16, inv_scale, V, grid, st);
-        else launch_sat_visgen_t<4, false>(g, k, sc, n_maps, a16, inv_scale, V, grid, st);
+        if (tocc) launch_sat_visgen_t<4, true>(g, k, sc, n_maps, a16, inv_scale, V, star, grid, st);
+        else launch_sat_visgen_t<4, false>(g, k, sc, n_maps, a16, inv_scale, V, star, grid, st);
     }
     return hipGetLastError();
 }
@@ -1547,7 +1616,7 @@ static hipError_t launch_recon_sat_t(const GeomDev &g, const PhaseKnobs &k, doub
     hipLaunchKernelGGL(zero_u32_kernel, dim3((n_maps + 255) / 256), dim3(256), 0, st, amax, n_maps);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(row_absmax_kernel, dim3(64, n_maps), dim3(256), 0, st,
-                       (const float *)V, g.n_vis, 2 * g.n_vis, 1, w, amax);
+                       (const float *)V, g.n_vis, 2 * g.n_vis, 1, w, amax, 0.0f);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL((split_vis_sat_kernel<NS>), dim3(2048), dim3(256), 0, st, g.n_vis, g.n_t, n_slots,
                        n_maps, V, w, kappa, amax, (_Float16 *)a16, inv_scale);
@@ -1579,8 +1648,8 @@ const void *mfma_recon_sat_kernel_ptr(int ns) {
 }
 
 const void *mfma_visgen_sat_kernel_ptr(int tb) {
-    return tb == 2 ? (const void *)mfma_visgen_sat_kernel<2, true>
-                   : (const void *)mfma_visgen_sat_kernel<4, true>;
+    return tb == 2 ? (const void *)mfma_visgen_sat_kernel<2, true, true>
+                   : (const void *)mfma_visgen_sat_kernel<4, true, true>;
 }
 
 hipError_t launch_mfma_recon(const GeomDev &g, const PhaseKnobs &k, int32_t n_maps,
@@ -1591,7 +1660,7 @@ hipError_t launch_mfma_recon(const GeomDev &g, const PhaseKnobs &k, int32_t n_ma
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(row_absmax_kernel, dim3(64, n_maps), dim3(256), 0, st,
-                       (const float *)V, g.n_vis, 2 * g.n_vis, 1, w, amax);
+                       (const float *)V, g.n_vis, 2 * g.n_vis, 1, w, amax, 0.0f);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(split_vis_kernel, dim3(2048), dim3(256), 0, st, g.n_vis, n_vis_pad, n_maps,
                        V, w, kappa, amax, (_Float16 *)a16, inv_scale);
