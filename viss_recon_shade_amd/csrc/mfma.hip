@@ -745,6 +745,12 @@ __host__ __device__ constexpr int star_y(int n, int j) {
 // Index of pair (x, y), x < y, in the lexicographic pair list (0,1), (0,2), ...
 __host__ __device__ constexpr int lex_pair(int n, int x, int y) { return x * n - x * (x + 1) / 2 + (y - x - 1); }
 
+// A buffer index as a type, for stage loops unrolled by two.
+template <int B>
+struct BufC {
+    static constexpr int value = B;
+};
+
 // Largest number of (step, satellite) rows one vis-gen block holds per stage.
 template <int TB>
 struct SatRows {
@@ -857,19 +863,20 @@ __device__ __forceinline__ void pixel_store(PixStage *d, uint4 x) {
 // needs no padding test: the map operand is zero on them).  Occultation per pixel: the FP32 test
 // against the guard band's edges, the exact FP64 predicate inside the band (the
 // rule of every kernel here; a per-pixel test costs fewer instructions than
-// classifying the row's cap against the 64-pixel group first).  With OCC, bit p
-// of sOc[r] is set where pixel p of the stage is occulted for the row or is
-// padding (where the row contributes nothing to any product).
-template <bool FULL, bool OCC, int MR>
-__device__ __forceinline__ void sat_rows(const GeomDev &g, double dzo, int wave, uint32_t live,
-                                         const PixStage *ps, const SatRow *sRow,
-                                         float2 (*sEm)[kSatLd], float2 (*sEf)[kSatLd],
-                                         uint64_t *sOc) {
+// classifying the row's cap against the 64-pixel group first).  PADS: the
+// stage may hold padding pixels.  Returns the AND over the wave's live rows of
+// the lane bits of pixels occulted for the row or padding (where the row
+// contributes nothing to any product); all ones if no row is live.
+template <bool FULL, bool PADS, int MR>
+__device__ __forceinline__ uint64_t sat_rows(const GeomDev &g, double dzo, int wave, uint32_t live,
+                                             const PixStage *ps, const SatRow *sRow,
+                                             float2 (*sEm)[kSatLd], float2 (*sEf)[kSatLd]) {
     const int lane = threadIdx.x & 63;
     const double4 s = ps->p64[lane];
     const float4 s32 = ps->p32[lane];
-    const bool pad = (FULL || OCC) && ps->ring[lane] < 0;
+    const bool pad = PADS && ps->ring[lane] < 0;
     const double szd = s.z - dzo;
+    uint64_t none = ~0ull;
 #pragma unroll
     for (int i = 0; i < MR; ++i) {
         if (!((live >> i) & 1u)) continue;  // wave-uniform
@@ -879,17 +886,15 @@ __device__ __forceinline__ void sat_rows(const GeomDev &g, double dzo, int wave,
         const double ph = fma(sp.x, s.x, fma(sp.y, s.y, sp.z * szd));
         const float rv = (float)__builtin_amdgcn_fract(ph);
         float2 ef = make_float2(__builtin_amdgcn_cosf(rv), __builtin_amdgcn_sinf(rv));
-        if (FULL && pad) ef = make_float2(0.f, 0.f);
+        if (FULL && PADS && pad) ef = make_float2(0.f, 0.f);
         const float x = fmaf(s32.z, cap.dz, fmaf(s32.y, cap.dy, s32.x * cap.dx));
         bool occ = x >= cap.hi;
         if (x >= cap.lo && !occ) occ = occ64m(sRow[r].pos, s.x, s.y, s.z, g.limb);
         sEm[r][lane] = occ ? make_float2(0.f, 0.f) : ef;
         if (FULL) sEf[r][lane] = ef;
-        if (OCC) {
-            const uint64_t bm = __ballot(occ || pad);
-            if (lane == 0) sOc[r] = bm;
-        }
+        none &= __ballot(PADS ? occ || pad : occ);
     }
+    return none;
 }
 
 // 16 x 16 Gram blocks of one step, staged for the consumers: [P1, P2] (full-sky
@@ -901,11 +906,12 @@ constexpr int kGramLd16 = 17;
 // the 16x16x32 instruction (the products of mfma_visgen_kernel, with E'_xy =
 // E'_y conj(E'_x) formed from the two staged rows of each lane's pair); every
 // wave forms its share of the next stage's rows after its K-steps.  Slots: j =
-// step (j / n_base), baseline (j % n_base); with STAR (every pair of 16 or 8
-// satellites, TB n_sat = 32), wave w holds star w of the star order at each of
-// the TB steps, so all its pairs hold satellite w, and it skips the K-steps on
-// whose 32 pixels satellite w is occulted at every step: all its products
-// there are zero.
+// step (j / n_base), baseline (j % n_base); with STAR (every pair of 16
+// satellites, two steps per block), wave w holds star w of the star order at
+// both steps, so all its pairs hold satellite w, and it skips the K-steps on
+// whose 32 pixels satellite w is occulted at both steps: all its products there
+// are zero.  Wave w forms rows w and w + 16 itself (satellite w at both steps),
+// so the pixels to skip come out of its own row formation.
 //
 // Shaded-pixel term (TOCC): V = sum_clear I E + t_occ sum_shaded E =
 // sum_clear (I - t_occ) E + t_occ U, with U_xy = sum_p E_y conj(E_x) over every
@@ -920,14 +926,13 @@ constexpr int kGramLd16 = 17;
 template <int TB, bool TOCC, bool STAR>
 __global__ void __launch_bounds__(kThreads) mfma_visgen_sat_kernel(
     GeomDev g, PhaseKnobs k, double sc, int32_t n_maps, const _Float16 *__restrict__ A16,
-    const float *__restrict__ inv_scale, float2 *__restrict__ V) {
+    const float *__restrict__ inv_scale, float2 *__restrict__ V, int32_t pads) {
     constexpr int R = SatRows<TB>::value;
     __shared__ _Float16 sA[2][2][kMaps][kStage];  // [buffer][hi/lo][map][pixel]
     __shared__ float2 sE[2][TOCC ? 2 : 1][R][kSatLd];  // [buffer][masked, full][row][pixel]
     __shared__ float sG[TOCC ? TB : 1][2][16][kGramLd16];
     __shared__ SatRow sRow[R];
     __shared__ PixStage sPix[2];  // [stage & 1]
-    __shared__ uint64_t sOcc[STAR ? 2 : 1][R];  // [buffer][row]: occultation bits (STAR)
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int col = lane & 15, kg = lane >> 4;
@@ -985,6 +990,11 @@ __global__ void __launch_bounds__(kThreads) mfma_visgen_sat_kernel(
         for (int i = 0; i < MR; ++i)
             if (i < nr && t0 + rtl[i] < g.n_t) live |= 1u << i;
     };
+    // Rows of one stage (pads: the geometry has padding pixels)
+    auto form_rows = [&](const PixStage *ps, uint32_t live, float2(*em)[kSatLd], float2(*ef)[kSatLd]) {
+        return pads ? sat_rows<TOCC, true, MR>(g, dzo, wave, live, ps, sRow, em, ef)
+                    : sat_rows<TOCC, false, MR>(g, dzo, wave, live, ps, sRow, em, ef);
+    };
     auto tile_begin = [&](int t0, uint32_t live, const _Float16 *Ag) {
         // The buffers, sRow and sPix are free: every wave passed the previous
         // tile's last barrier, and the last stage forms no rows.
@@ -999,7 +1009,6 @@ __global__ void __launch_bounds__(kThreads) mfma_visgen_sat_kernel(
             } else {
                 e.pos = make_double4(0.0, 0.0, 0.0, 0.0);
                 e.cap = {0.f, 0.f, 0.f, 2.0f, 0.f, kGuardMin, 2.0f - kGuardMin, 2.0f + kGuardMin};
-                if (STAR) sOcc[0][r] = sOcc[STAR ? 1 : 0][r] = ~0ull;  // no step: skippable
             }
             e.spos = make_double4(e.pos.x * sc, e.pos.y * sc, e.pos.z * sc, 0.0);
             sRow[r] = e;
@@ -1008,16 +1017,16 @@ __global__ void __launch_bounds__(kThreads) mfma_visgen_sat_kernel(
         pixel_store(&sPix[0], pixel_piece(psrc, 0));
         if (n_stages > 1) pixel_store(&sPix[1], pixel_piece(psrc, 1));
         __syncthreads();
-        sat_rows<TOCC, STAR, MR>(g, dzo, wave, live, &sPix[0], sRow, sE[0][0], sE[0][TOCC ? 1 : 0],
-                                 sOcc[0]);
+        const uint64_t none = form_rows(&sPix[0], live, sE[0][0], sE[0][TOCC ? 1 : 0]);
         __syncthreads();
+        return none;
     };
     // Prefetch into registers, stored at the end of each stage: the map tile
     // of stage st + 1 and the pixel inputs of stage st + 2 (loaded without a
     // branch: the last stages reload their own).  Nothing in a stage waits
     // for them.
-    auto stage_end = [&](int64_t st, uint4 tnext, uint4 pnext) {
-        const int buf = (int)(st & 1);
+    auto stage_end = [&](int64_t st, uint4 tnext, uint4 pnext, auto bufc) {
+        constexpr int buf = decltype(bufc)::value;
         // sA[buf ^ 1] was last read before the previous barrier; sPix[buf] holds
         // stage st's inputs, last read while forming stage st
         asm volatile("" ::: "memory");
@@ -1033,25 +1042,20 @@ __global__ void __launch_bounds__(kThreads) mfma_visgen_sat_kernel(
             uint32_t live;
             tile_of(tile, grp, t0, live);
             const _Float16 *Ag = A16 + grp * 2 * kMaps * g.n_pad;
-            tile_begin(t0, live, Ag);
+            uint64_t none = tile_begin(t0, live, Ag);  // of the next stage to consume
             floatx4 acc[kMaps / kMTV][2];
 #pragma unroll
             for (int a = 0; a < kMaps / kMTV; ++a)
 #pragma unroll
                 for (int c = 0; c < 2; ++c) acc[a][c] = floatx4{};
-            for (int64_t st = 0; st < n_stages; ++st) {
-                const int buf = (int)(st & 1);
+            // the stage loop unrolled by two: buffer indices known at compile time
+            auto cstage = [&](int64_t st, auto bufc) {
+                constexpr int buf = decltype(bufc)::value;
                 const uint4 tnext = tile_load(Ag, g.n_pad, st + 1 < n_stages ? st + 1 : st);
                 const uint4 pnext = pixel_piece(psrc, st + 2 < n_stages ? st + 2 : st);
                 // K-steps whose products are all zero (STAR): bit ks
                 uint32_t skip = 0;
-                if (STAR) {
-                    uint64_t m = ~0ull;
-#pragma unroll
-                    for (int tl = 0; tl < TB; ++tl) m &= sOcc[STAR ? buf : 0][tl * n_sat + wave];
-                    const uint32_t sk = ((uint32_t)m == ~0u ? 1u : 0u) | ((uint32_t)(m >> 32) == ~0u ? 2u : 0u);
-                    skip = __builtin_amdgcn_readfirstlane(sk);
-                }
+                if (STAR) skip = ((uint32_t)none == ~0u ? 1u : 0u) | ((uint32_t)(none >> 32) == ~0u ? 2u : 0u);
                 // E of K-step ks for this lane's 8 pixels (ks * 32 + 8 kg + i, identity
                 // K order), read one K-step ahead so that the reads of K-step ks + 1
                 // are in flight while the products of K-step ks issue
@@ -1102,10 +1106,15 @@ __global__ void __launch_bounds__(kThreads) mfma_visgen_sat_kernel(
                     }
                 }
                 if (st + 1 < n_stages)
-                    sat_rows<TOCC, STAR, MR>(g, dzo, wave, live, &sPix[buf ^ 1], sRow, sE[buf ^ 1][0],
-                                             sE[buf ^ 1][TOCC ? 1 : 0], sOcc[STAR ? buf ^ 1 : 0]);
-                stage_end(st, tnext, pnext);
+                    none = form_rows(&sPix[buf ^ 1], live, sE[buf ^ 1][0], sE[buf ^ 1][TOCC ? 1 : 0]);
+                stage_end(st, tnext, pnext, bufc);
+            };
+            int64_t st = 0;
+            for (; st + 1 < n_stages; st += 2) {
+                cstage(st, BufC<0>{});
+                cstage(st + 1, BufC<1>{});
             }
+            if (st < n_stages) cstage(st, BufC<0>{});
             if (TOCC) __syncthreads();  // sG whole
             if (consumer && j < n_slot && t0 + tlv < g.n_t) {
                 const int64_t v = (int64_t)bi * g.n_t + t0 + tlv;
@@ -1140,12 +1149,12 @@ __global__ void __launch_bounds__(kThreads) mfma_visgen_sat_kernel(
             uint32_t live;
             tile_of(tile, grp, t0, live);
             const _Float16 *Ag = A16 + grp * 2 * kMaps * g.n_pad;
-            tile_begin(t0, live, Ag);
+            (void)tile_begin(t0, live, Ag);
             floatx4 p1[TB], p2[TB];
 #pragma unroll
             for (int q = 0; q < TB; ++q) p1[q] = p2[q] = floatx4{};
-            for (int64_t st = 0; st < n_stages; ++st) {
-                const int buf = (int)(st & 1);
+            auto gstage = [&](int64_t st, auto bufc) {
+                constexpr int buf = decltype(bufc)::value;
                 const uint4 tnext = tile_load(Ag, g.n_pad, st + 1 < n_stages ? st + 1 : st);
                 const uint4 pnext = pixel_piece(psrc, st + 2 < n_stages ? st + 2 : st);
 #pragma unroll
@@ -1184,10 +1193,15 @@ __global__ void __launch_bounds__(kThreads) mfma_visgen_sat_kernel(
                     }
                 }
                 if (st + 1 < n_stages)
-                    sat_rows<TOCC, STAR, MR>(g, dzo, wave, live, &sPix[buf ^ 1], sRow, sE[buf ^ 1][0],
-                                             sE[buf ^ 1][TOCC ? 1 : 0], sOcc[STAR ? buf ^ 1 : 0]);
-                stage_end(st, tnext, pnext);
+                    (void)form_rows(&sPix[buf ^ 1], live, sE[buf ^ 1][0], sE[buf ^ 1][TOCC ? 1 : 0]);
+                stage_end(st, tnext, pnext, bufc);
+            };
+            int64_t st = 0;
+            for (; st + 1 < n_stages; st += 2) {
+                gstage(st, BufC<0>{});
+                gstage(st + 1, BufC<1>{});
             }
+            if (st < n_stages) gstage(st, BufC<0>{});
             // D of 16x16x32: lane holds rows 4 (lane / 16) + r of column lane % 16
 #pragma unroll
             for (int q = 0; q < TB; ++q)
@@ -1206,16 +1220,21 @@ __global__ void __launch_bounds__(kThreads) mfma_visgen_sat_kernel(
             uint32_t live;
             tile_of(tile, grp, t0, live);
             const _Float16 *Ag = A16 + grp * 2 * kMaps * g.n_pad;
-            tile_begin(t0, live, Ag);
-            for (int64_t st = 0; st < n_stages; ++st) {
-                const int buf = (int)(st & 1);
+            (void)tile_begin(t0, live, Ag);
+            auto istage = [&](int64_t st, auto bufc) {
+                constexpr int buf = decltype(bufc)::value;
                 const uint4 tnext = tile_load(Ag, g.n_pad, st + 1 < n_stages ? st + 1 : st);
                 const uint4 pnext = pixel_piece(psrc, st + 2 < n_stages ? st + 2 : st);
                 if (st + 1 < n_stages)
-                    sat_rows<TOCC, STAR, MR>(g, dzo, wave, live, &sPix[buf ^ 1], sRow, sE[buf ^ 1][0],
-                                             sE[buf ^ 1][TOCC ? 1 : 0], sOcc[STAR ? buf ^ 1 : 0]);
-                stage_end(st, tnext, pnext);
+                    (void)form_rows(&sPix[buf ^ 1], live, sE[buf ^ 1][0], sE[buf ^ 1][TOCC ? 1 : 0]);
+                stage_end(st, tnext, pnext, bufc);
+            };
+            int64_t st = 0;
+            for (; st + 1 < n_stages; st += 2) {
+                istage(st, BufC<0>{});
+                istage(st + 1, BufC<1>{});
             }
+            if (st < n_stages) istage(st, BufC<0>{});
             if (TOCC) __syncthreads();  // sG whole
         }
     }
@@ -1553,18 +1572,22 @@ int mfma_sat_tb(int32_t n_sat, int32_t n_base) {
 template <int TB, bool TOCC>
 static void launch_sat_visgen_t(const GeomDev &g, const PhaseKnobs &k, double sc, int32_t n_maps,
                                 const void *a16, const float *inv_scale, float2 *V, bool star,
-                                int grid, hipStream_t st) {
-    if (star)
-        hipLaunchKernelGGL((mfma_visgen_sat_kernel<TB, TOCC, true>), dim3(grid), dim3(kThreads), 0, st,
-                           g, k, sc, n_maps, (const _Float16 *)a16, inv_scale, V);
-    else
-        hipLaunchKernelGGL((mfma_visgen_sat_kernel<TB, TOCC, false>), dim3(grid), dim3(kThreads), 0, st,
-                           g, k, sc, n_maps, (const _Float16 *)a16, inv_scale, V);
+                                int32_t pads, int grid, hipStream_t st) {
+    if constexpr (TB == 2) {  // the star layout holds two steps
+        if (star) {
+            hipLaunchKernelGGL((mfma_visgen_sat_kernel<TB, TOCC, true>), dim3(grid), dim3(kThreads), 0, st,
+                               g, k, sc, n_maps, (const _Float16 *)a16, inv_scale, V, pads);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((mfma_visgen_sat_kernel<TB, TOCC, false>), dim3(grid), dim3(kThreads), 0, st,
+                           g, k, sc, n_maps, (const _Float16 *)a16, inv_scale, V, pads);
 }
 
 int mfma_visgen_sat_star(int32_t n_sat, int32_t n_base) {
-    return (n_sat == 16 || n_sat == 8) && n_base == n_sat * (n_sat - 1) / 2 &&
-           mfma_sat_tb(n_sat, n_base) * n_sat == 2 * kNTV;
+    // two steps of 15 stars of 8 pairs: one star per consumer wave, whose rows
+    // (satellite w at both steps: rows w and w + 16) that wave forms itself
+    return n_sat == 16 && n_base == 120 && mfma_sat_tb(n_sat, n_base) == 2;
 }
 
 hipError_t launch_mfma_visgen_sat(const GeomDev &g, const PhaseKnobs &k, double inv_lambda_km,
@@ -1586,12 +1609,13 @@ hipError_t launch_mfma_visgen_sat(const GeomDev &g, const PhaseKnobs &k, double 
     if ((e = hipGetLastError()) != hipSuccess) return e;
     const double sc = k.s * inv_lambda_km;
     const bool tocc = k.t_occ != 0.0f;
+    const int32_t pads = n_pix != g.n_pad;  // padding pixels: the pixel count is not whole chunks
     if (tb == 2) {
-        if (tocc) launch_sat_visgen_t<2, true>(g, k, sc, n_maps, a16, inv_scale, V, star, grid, st);
-        else launch_sat_visgen_t<2, false>(g, k, sc, n_maps, a16, inv_scale, V, star, grid, st);
+        if (tocc) launch_sat_visgen_t<2, true>(g, k, sc, n_maps, a16, inv_scale, V, star, pads, grid, st);
+        else launch_sat_visgen_t<2, false>(g, k, sc, n_maps, a16, inv_scale, V, star, pads, grid, st);
     } else {
-        if (tocc) launch_sat_visgen_t<4, true>(g, k, sc, n_maps, a16, inv_scale, V, star, grid, st);
-        else launch_sat_visgen_t<4, false>(g, k, sc, n_maps, a16, inv_scale, V, star, grid, st);
+        if (tocc) launch_sat_visgen_t<4, true>(g, k, sc, n_maps, a16, inv_scale, V, star, pads, grid, st);
+        else launch_sat_visgen_t<4, false>(g, k, sc, n_maps, a16, inv_scale, V, star, pads, grid, st);
     }
     return hipGetLastError();
 }
@@ -1649,7 +1673,7 @@ const void *mfma_recon_sat_kernel_ptr(int ns) {
 
 const void *mfma_visgen_sat_kernel_ptr(int tb) {
     return tb == 2 ? (const void *)mfma_visgen_sat_kernel<2, true, true>
-                   : (const void *)mfma_visgen_sat_kernel<4, true, true>;
+                   : (const void *)mfma_visgen_sat_kernel<4, true, false>;
 }
 
 hipError_t launch_mfma_recon(const GeomDev &g, const PhaseKnobs &k, int32_t n_maps,

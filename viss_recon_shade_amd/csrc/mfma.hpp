@@ -28,10 +28,10 @@ hipError_t launch_mfma_recon(const GeomDev &g, const PhaseKnobs &k, int32_t n_ma
 // position differences): steps per vis-gen block for this array, 0 if the
 // satellite kernels do not apply (n_sat outside [2, 16]).
 int mfma_sat_tb(int32_t n_sat, int32_t n_base);
-// 1 if the array's blocks can hold star-ordered baselines (16 or 8 satellites,
-// every pair; the pair list must be the lexicographic one): each wave's
-// visibilities then share one satellite, and a wave skips the K-steps on which
-// that satellite is occulted at every step of the block.
+// 1 if the array's blocks can hold star-ordered baselines (16 satellites, every
+// pair; the pair list must be the lexicographic one): each wave's visibilities
+// then share one satellite, and a wave skips the K-steps on which that
+// satellite is occulted at both steps of the block.
 int mfma_visgen_sat_star(int32_t n_sat, int32_t n_base);
 hipError_t launch_mfma_visgen_sat(const GeomDev &g, const PhaseKnobs &k, double inv_lambda_km,
                                   int32_t n_maps, int64_t n_pix, const float *I,
