@@ -1341,11 +1341,15 @@ __device__ __forceinline__ void vtile_dma(const _Float16 *__restrict__ Ag, int64
     }
 }
 
-// A satellite's scaled position and shade cap at one step (64 bytes).
+// A satellite's scaled position, shade cap and position (for the exact
+// predicate) at one step (96 bytes): forming the factors reads no global memory,
+// whose loads would wait behind the V tile in flight.
 struct RecRow {
     double4 spos;
     CapRec cap;
+    double4 pos;
 };
+static_assert(sizeof(RecRow) == 96, "recon row layout");
 
 // Recon on per-satellite factors (NS satellites, the lexicographic pair list).
 // Block: 8 waves x 32 pixels; K runs over step pairs, 4 K-steps of 16 slots per
@@ -1355,8 +1359,7 @@ struct RecRow {
 // all NS satellites at its step in registers (FP64 phase and fract, FP32 sin and
 // cos, zeroed where that satellite's ray to the pixel is occulted: the exact
 // per-pixel rule), so each entry is E'_xy = E'_y conj(E'_x), one complex
-// product, and a shaded entry is zero by construction.  A K-step whose star
-// centres are occulted for every lane of the wave is all zero and skipped.
+// product, and a shaded entry is zero by construction.
 template <int NS>
 __global__ void __launch_bounds__(kThreadsRS) mfma_recon_sat_kernel(
     GeomDev g, PhaseKnobs k, int32_t n_maps, int64_t n_slots, const _Float16 *__restrict__ A16,
@@ -1381,20 +1384,23 @@ __global__ void __launch_bounds__(kThreadsRS) mfma_recon_sat_kernel(
     const double dzo = k.wfac ? 0.0 : 1.0;
     const float neg1 = opaque_neg1();
 
-    // the rows of step pair tp: 2 NS rows of 4 pieces, threads 0 .. 8 NS - 1;
+    // the rows of step pair tp: 2 NS rows of 6 pieces, threads 0 .. 12 NS - 1;
     // this thread's piece: row (satellite rs, step 2 tp + rh), 16-byte part rq
-    const int rr_ = threadIdx.x >> 2, rq = threadIdx.x & 3;
+    constexpr int kRowPieces = sizeof(RecRow) / 16;
+    const int rr_ = threadIdx.x / kRowPieces, rq = threadIdx.x % kRowPieces;
     const int rh = rr_ / NS, rs = rr_ % NS;
-    const uint4 *rsrc = (rq < 2 ? reinterpret_cast<const uint4 *>(spos + (int64_t)rs * g.n_t + rh)
-                                : reinterpret_cast<const uint4 *>(g.cap + (int64_t)rs * g.n_t + rh)) +
+    const int64_t re = (int64_t)rs * g.n_t + rh;
+    const uint4 *rsrc = (rq < 2   ? reinterpret_cast<const uint4 *>(spos + re)
+                         : rq < 4 ? reinterpret_cast<const uint4 *>(g.cap + re)
+                                  : reinterpret_cast<const uint4 *>(g.pos + re)) +
                         (rq & 1);
     auto rows_load = [&](int64_t tp) {
         uint4 x = make_uint4(0u, 0u, 0u, 0u);
-        if (threadIdx.x < 8 * NS && 2 * tp + rh < g.n_t) x = rsrc[2 * (2 * tp)];  // 2 pieces per row
+        if (threadIdx.x < 2 * NS * kRowPieces && 2 * tp + rh < g.n_t) x = rsrc[2 * (2 * tp)];  // 2 pieces per element
         return x;
     };
     auto rows_store = [&](RecRow *dst, uint4 x) {
-        if (threadIdx.x < 8 * NS) reinterpret_cast<uint4 *>(dst)[threadIdx.x] = x;
+        if (threadIdx.x < 2 * NS * kRowPieces) reinterpret_cast<uint4 *>(dst)[threadIdx.x] = x;
     };
 
     for (int64_t item = blockIdx.x; item < n_items; item += gridDim.x) {
@@ -1442,8 +1448,7 @@ __global__ void __launch_bounds__(kThreadsRS) mfma_recon_sat_kernel(
                     bool occ = x >= cap.hi;
                     if (x >= cap.lo && !occ) {  // inside the guard band: the exact predicate
                         const int64_t t = 2 * tp + kg;
-                        occ = t < g.n_t &&
-                              occ64m(g.pos[(int64_t)s * g.n_t + t], s64.x, s64.y, s64.z, g.limb);
+                        occ = t < g.n_t && occ64m(rr[s].pos, s64.x, s64.y, s64.z, g.limb);
                     }
                     ec[s] = occ ? 0.0f : __builtin_amdgcn_cosf(rv);
                     es[s] = occ ? 0.0f : __builtin_amdgcn_sinf(rv);
