@@ -1257,6 +1257,9 @@ struct SatRecon {
 
 // V [n_maps][n_vis] complex -> hi/lo FP16 [group][re, -im][hi, lo][kMaps][n_tp * kSlots]
 // in the step-pair layout above, weighted by w * kappa and scaled per map.
+// Block item: one map row and kTPB step pairs.  The 2 kTPB steps of every pair
+// (contiguous in V) are staged in LDS, weighted, then written slot by slot, so
+// both the reads of V and the writes of each plane are contiguous runs.
 template <int NS>
 __global__ void __launch_bounds__(256) split_vis_sat_kernel(int64_t n_vis, int32_t n_t, int64_t n_slots,
                                                             int32_t n_maps, const float2 *__restrict__ V,
@@ -1265,37 +1268,54 @@ __global__ void __launch_bounds__(256) split_vis_sat_kernel(int64_t n_vis, int32
                                                             _Float16 *__restrict__ out,
                                                             float *__restrict__ inv_scale) {
     using L = SatRecon<NS>;
+    constexpr int kTPB = 8, kT = 2 * kTPB;  // step pairs and steps per item
+    __shared__ float2 sv[L::kPairs][kT];    // [lexicographic pair][step]: (Re, -Im) weighted
+    const int64_t n_tp = n_slots / L::kSlots;
+    const int64_t n_tb = (n_tp + kTPB - 1) / kTPB;
     const int64_t n_groups = (n_maps + kMaps - 1) / kMaps;
-    const int64_t total = n_groups * kMaps * n_slots;
-    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
-         idx += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t slot = idx % n_slots;
-        const int64_t mrow = idx / n_slots;
+    const int64_t n_items = n_groups * kMaps * n_tb;
+    for (int64_t item = blockIdx.x; item < n_items; item += gridDim.x) {
+        const int64_t tb = item % n_tb, mrow = item / n_tb;
         const int64_t grp = mrow / kMaps, row = mrow % kMaps;
         const int64_t m = grp * kMaps + row;
-        const int64_t tp = slot / L::kSlots;
-        const int rem = (int)(slot % L::kSlots);
-        const int j = 8 * (rem / 16) + rem % 8;
-        const int64_t t = 2 * tp + (rem / 8) % 2;
-        float xr = 0.0f, xi = 0.0f;
-        if (m < n_maps) {
-            const float sc = row_scale(amax[m], fabsf(kappa));
-            if (j < L::kPairs && t < n_t) {
-                const int64_t v = (int64_t)lex_pair(NS, star_x(NS, j), star_y(NS, j)) * n_t + t;
-                const float2 q = V[m * n_vis + v];
+        const int64_t t0 = tb * kT;
+        const float sc = m < n_maps ? row_scale(amax[m], fabsf(kappa)) : 1.0f;
+        __syncthreads();  // the previous item's reads of sv are done
+        for (int i = threadIdx.x; i < L::kPairs * kT; i += blockDim.x) {
+            const int p = i / kT, ss = i % kT;
+            const int64_t t = t0 + ss;
+            float2 q = make_float2(0.0f, 0.0f);
+            if (m < n_maps && t < n_t) {
+                const int64_t v = (int64_t)p * n_t + t;
+                const float2 x = V[m * n_vis + v];
                 const float ww = (w ? w[v] : 1.0f) * kappa * sc;
-                xr = q.x * ww;
-                xi = -(q.y * ww);
+                q = make_float2(x.x * ww, -(x.y * ww));
             }
-            if (slot == 0) inv_scale[m] = 1.0f / sc;
+            sv[p][ss] = q;
         }
-        const _Float16 hr = (_Float16)xr, hi_ = (_Float16)xi;
-        const _Float16 lr = (_Float16)(xr - (float)hr), li = (_Float16)(xi - (float)hi_);
+        __syncthreads();
+        const int64_t s0 = tb * kTPB * L::kSlots;
+        const int nsl = (int)(n_slots - s0 < kTPB * L::kSlots ? n_slots - s0 : kTPB * L::kSlots);
         const int64_t base = grp * 4 * kMaps;
-        out[((base + 0 * kMaps) + row) * n_slots + slot] = hr;
-        out[((base + 1 * kMaps) + row) * n_slots + slot] = lr;
-        out[((base + 2 * kMaps) + row) * n_slots + slot] = hi_;
-        out[((base + 3 * kMaps) + row) * n_slots + slot] = li;
+        for (int i = threadIdx.x; i < nsl; i += blockDim.x) {
+            const int tpl = i / L::kSlots, rem = i % L::kSlots;
+            const int j = 8 * (rem / 16) + rem % 8;  // pair (star order)
+            const int h = (rem / 8) % 2;             // step 2 tp + h
+            float xr = 0.0f, xi = 0.0f;
+            if (j < L::kPairs) {
+                const float2 q = sv[lex_pair(NS, star_x(NS, j), star_y(NS, j))][2 * tpl + h];
+                xr = q.x;
+                xi = q.y;
+            }
+            const _Float16 hr = (_Float16)xr, hi_ = (_Float16)xi;
+            const _Float16 lr = (_Float16)(xr - (float)hr), li = (_Float16)(xi - (float)hi_);
+            const int64_t o = s0 + i;
+            out[((base + 0 * kMaps) + row) * n_slots + o] = hr;
+            out[((base + 1 * kMaps) + row) * n_slots + o] = lr;
+            out[((base + 2 * kMaps) + row) * n_slots + o] = hi_;
+            out[((base + 3 * kMaps) + row) * n_slots + o] = li;
+        }
+        if (tb == 0 && threadIdx.x == 0 && m < n_maps) inv_scale[m] = 1.0f / sc;
     }
 }
 
