@@ -130,8 +130,10 @@ typedef struct vrs_knobs {
                             * VRS_UVW_CEIL, at most 16 satellites) the matrix-core kernels form
                             * each pair's phase factor as E_b conj(E_a) from per-satellite
                             * factors (one FP64 phase and sin/cos per satellite and step,
-                            * each satellite's own occultation zeroing its factor); 0: every
-                            * (visibility, pixel) entry forms its own FP64 phase and sin/cos. */
+                            * each satellite's own occultation zeroing its factor; with
+                            * t_occ != 0 the shaded term is t_occ times each pair's full-sky
+                            * sum, the maps entering as I - t_occ); 0: every (visibility,
+                            * pixel) entry forms its own FP64 phase and sin/cos. */
 } vrs_knobs;
 
 typedef struct vrs_shade_stats {
