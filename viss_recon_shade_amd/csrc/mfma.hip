@@ -1255,8 +1255,11 @@ struct SatRecon {
                                               // buffer parity is known at compile time
 };
 
-// V [n_maps][n_vis] complex -> hi/lo FP16 [group][re, -im][hi, lo][kMaps][n_tp * kSlots]
-// in the step-pair layout above, weighted by w * kappa and scaled per map.
+// V [n_maps][n_vis] complex -> hi/lo FP16 in the step-pair slot order above,
+// weighted by w * kappa and scaled per map, stored as the recon kernel's LDS
+// tiles: [group][tile of 64 slots][re hi, re lo, im hi, im lo][map][64 slots],
+// each 64-slot row in the a_slot swizzle, so a tile is one contiguous 32 KiB run
+// that the kernel copies into LDS unchanged.
 // Block item: one map row and kTPB step pairs.  The 2 kTPB steps of every pair
 // (contiguous in V) are staged in LDS, weighted, then written slot by slot, so
 // both the reads of V and the writes of each plane are contiguous runs.
@@ -1296,7 +1299,6 @@ __global__ void __launch_bounds__(256) split_vis_sat_kernel(int64_t n_vis, int32
         __syncthreads();
         const int64_t s0 = tb * kTPB * L::kSlots;
         const int nsl = (int)(n_slots - s0 < kTPB * L::kSlots ? n_slots - s0 : kTPB * L::kSlots);
-        const int64_t base = grp * 4 * kMaps;
         for (int i = threadIdx.x; i < nsl; i += blockDim.x) {
             const int tpl = i / L::kSlots, rem = i % L::kSlots;
             const int j = 8 * (rem / 16) + rem % 8;  // pair (star order)
@@ -1310,10 +1312,14 @@ __global__ void __launch_bounds__(256) split_vis_sat_kernel(int64_t n_vis, int32
             const _Float16 hr = (_Float16)xr, hi_ = (_Float16)xi;
             const _Float16 lr = (_Float16)(xr - (float)hr), li = (_Float16)(xi - (float)hi_);
             const int64_t o = s0 + i;
-            out[((base + 0 * kMaps) + row) * n_slots + o] = hr;
-            out[((base + 1 * kMaps) + row) * n_slots + o] = lr;
-            out[((base + 2 * kMaps) + row) * n_slots + o] = hi_;
-            out[((base + 3 * kMaps) + row) * n_slots + o] = li;
+            const int64_t tile = o / kStage;
+            const int c = (int)(o % kStage);
+            _Float16 *t = out + (grp * (n_slots / kStage) + tile) * (4 * kMaps * kStage) + row * kStage +
+                          a_slot((int)row, c >> 3) * 8 + (c & 7);
+            t[0 * kMaps * kStage] = hr;
+            t[1 * kMaps * kStage] = lr;
+            t[2 * kMaps * kStage] = hi_;
+            t[3 * kMaps * kStage] = li;
         }
         if (tb == 0 && threadIdx.x == 0 && m < n_maps) inv_scale[m] = 1.0f / sc;
     }
@@ -1336,28 +1342,22 @@ __global__ void __launch_bounds__(256) scale_pos_kernel(int64_t n, const double4
 constexpr int kWavesRS = 8;
 constexpr int kThreadsRS = kWavesRS * 64;
 
-// The weighted-V tile of stage st (4 FP16 planes x 64 maps x 64 slots, 32 KiB)
-// straight from global memory into LDS (global_load_lds_dwordx4), placed as
-// stage_load_lds<4> places it: 32 blocks of 1 KiB, wave w moving blocks w,
+// The weighted-V tile of stage st (4 FP16 planes x 64 maps x 64 slots, 32 KiB,
+// stored contiguous and already swizzled) straight from global memory into LDS
+// (global_load_lds_dwordx4): 32 blocks of 1 KiB, wave w moving blocks w,
 // w + kWavesRS, ...  The two stage buffers are distinct LDS arrays, chosen at
 // compile time, so that the compiler can tell the K-steps' reads of one from
 // the loads in flight into the other (reads it cannot tell apart would wait
 // for every load in flight).
-__device__ __forceinline__ void vtile_dma(const _Float16 *__restrict__ Ag, int64_t ld, int64_t st,
-                                          _Float16 *sAbuf) {
+__device__ __forceinline__ void vtile_dma(const _Float16 *__restrict__ Ag, int64_t st, _Float16 *sAbuf) {
     constexpr int kBlocks = 4 * kMaps * (kStage / 8) / 64;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const _Float16 *tile = Ag + st * (4 * kMaps * kStage);  // stored in the LDS layout
 #pragma unroll
     for (int r = 0; r < (kBlocks + kWavesRS - 1) / kWavesRS; ++r) {
         const int blk = wave + r * kWavesRS;
-        if (blk < kBlocks) {
-            const int i = blk * 64 + lane;
-            const int plane = i / (kMaps * (kStage / 8));
-            const int rem = i % (kMaps * (kStage / 8));
-            const int row = rem / (kStage / 8), slot = rem % (kStage / 8);
-            const _Float16 *src = Ag + ((int64_t)plane * kMaps + row) * ld + st * kStage + a_slot(row, slot) * 8;
-            __builtin_amdgcn_global_load_lds(src, (lds_ptr_t)(sAbuf + blk * 512), 16, 0, 0);
-        }
+        if (blk < kBlocks)
+            __builtin_amdgcn_global_load_lds(tile + blk * 512 + lane * 8, (lds_ptr_t)(sAbuf + blk * 512), 16, 0, 0);
     }
 }
 
@@ -1444,7 +1444,7 @@ __global__ void __launch_bounds__(kThreadsRS) mfma_recon_sat_kernel(
         acc[1] = floatx16{};
         if (tpb < tpe) {
             // Both buffers are free: every wave passed the previous item's last barrier.
-            vtile_dma(Ag, n_slots, tpb * L::kStages, &sA0[0][0][0]);
+            vtile_dma(Ag, tpb * L::kStages, &sA0[0][0][0]);
             rows_store(&sRow[tpb & 1][0], rows_load(tpb));
             wait_vm();
             __syncthreads();
@@ -1488,7 +1488,7 @@ __global__ void __launch_bounds__(kThreadsRS) mfma_recon_sat_kernel(
                 // the stage), then the next stage's V tile into the other buffer
                 const bool rows_next = sq + 1 == L::kStages && tp + 1 < tpe;
                 const uint4 rn = rows_load(rows_next ? tp + 1 : tp);
-                if (more) vtile_dma(Ag, n_slots, tpb * L::kStages + st + 1, (sq & 1) ? &sA0[0][0][0] : &sA1[0][0][0]);
+                if (more) vtile_dma(Ag, tpb * L::kStages + st + 1, (sq & 1) ? &sA0[0][0][0] : &sA1[0][0][0]);
                 {
 #pragma unroll
                     for (int kq = 0; kq < 4; ++kq) {
