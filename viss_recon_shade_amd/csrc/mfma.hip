@@ -745,6 +745,21 @@ __host__ __device__ constexpr int star_y(int n, int j) {
 // Index of pair (x, y), x < y, in the lexicographic pair list (0,1), (0,2), ...
 __host__ __device__ constexpr int lex_pair(int n, int x, int y) { return x * n - x * (x + 1) / 2 + (y - x - 1); }
 
+// The star order of n satellites holds every pair exactly once, each with the
+// smaller satellite first, and every pair of star c holds satellite c.
+constexpr bool star_order_ok(int n) {
+    uint64_t seen[4] = {0, 0, 0, 0};  // bit lex_pair of the pairs met (n <= 16)
+    for (int j = 0; j < n * (n - 1) / 2; ++j) {
+        const int x = star_x(n, j), y = star_y(n, j), c = star_center(n, j);
+        if (x < 0 || x >= y || y >= n || (x != c && y != c)) return false;
+        const int b = lex_pair(n, x, y);
+        if ((seen[b >> 6] >> (b & 63)) & 1u) return false;
+        seen[b >> 6] |= uint64_t(1) << (b & 63);
+    }
+    return star_x(n, n * (n - 1) / 2) < 0;
+}
+static_assert(star_order_ok(16) && star_order_ok(8) && star_order_ok(4), "star order of the pairs");
+
 // A buffer index as a type, for stage loops unrolled by two.
 template <int B>
 struct BufC {
@@ -758,7 +773,6 @@ struct SatRows {
 };
 constexpr int kSatLd = kStage + 2;      // float2 per LDS row: 64 pixels + 16 bytes (rows on shifted banks)
 constexpr int kSatSlots = (kWaves - 1) * kNTV;  // visibilities per block: 15 waves x 16
-constexpr int kGramLd = 33;             // floats per row of a staged 32 x 32 Gram matrix
 
 // x = hi + lo in FP16: hi rounded toward zero, lo = x - hi rounded to nearest.
 // neg1 is -1 hidden from the optimiser (opaque_neg1): fma(hi, neg1, x) is then
