@@ -945,6 +945,10 @@ __global__ void __launch_bounds__(kThreads) mfma_visgen_sat_kernel(
     __shared__ _Float16 sA[2][2][kMaps][kStage];  // [buffer][hi/lo][map][pixel]
     __shared__ float2 sE[2][TOCC ? 2 : 1][R][kSatLd];  // [buffer][masked, full][row][pixel]
     __shared__ float sG[TOCC ? TB : 1][2][16][kGramLd16];
+    // the Gram wave's running totals: sum and compensation (each lane owns its
+    // entries, [q][P1, P2][row][column] as in sG)
+    __shared__ float sGt[TOCC && TB == 2 ? TB : 1][2][16][kGramLd16];
+    __shared__ float sGc[TOCC && TB == 2 ? TB : 1][2][16][kGramLd16];
     __shared__ SatRow sRow[R];
     __shared__ PixStage sPix[2];  // [stage & 1]
     const int lane = threadIdx.x & 63;
@@ -1155,8 +1159,32 @@ __global__ void __launch_bounds__(kThreads) mfma_visgen_sat_kernel(
         }
     } else if (gram) {
         // this lane: satellite gs = lane % 16 (its row and column of the Gram
-        // blocks), pixels 8 kg + i of each 32-pixel K-step
+        // blocks), pixels 8 kg + i of each 32-pixel K-step.  The products of
+        // kGramFold stages accumulate in registers, then fold into compensated
+        // totals in LDS (two-sum): a full-sky sum of ~1e5 terms would otherwise
+        // collect one FP32 rounding per product, at the magnitude of the sum.
         const int gs = lane & 15;
+        constexpr int kGramFold = 16;
+        constexpr bool kFold = TB == 2;  // four steps' blocks would spill
+        auto fold = [&](floatx4 (&p1)[TB], floatx4 (&p2)[TB]) {
+#pragma unroll
+            for (int q = 0; q < TB; ++q)
+#pragma unroll
+                for (int k = 0; k < 2; ++k) {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const float p = k ? p2[q][r] : p1[q][r];
+                        float &t = sGt[q][k][4 * kg + r][gs];
+                        float &c = sGc[q][k][4 * kg + r][gs];
+                        const float t0_ = t, sum = t0_ + p, bp = sum - t0_;
+                        c += (t0_ - (sum - bp)) + (p - bp);
+                        t = sum;
+                    }
+                    __builtin_amdgcn_sched_barrier(0);  // one (q, P1 / P2) block at a time
+                }
+#pragma unroll
+            for (int q = 0; q < TB; ++q) p1[q] = p2[q] = floatx4{};
+        };
         for (int64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
             int64_t grp;
             int t0;
@@ -1166,7 +1194,15 @@ __global__ void __launch_bounds__(kThreads) mfma_visgen_sat_kernel(
             (void)tile_begin(t0, live, Ag);
             floatx4 p1[TB], p2[TB];
 #pragma unroll
-            for (int q = 0; q < TB; ++q) p1[q] = p2[q] = floatx4{};
+            for (int q = 0; q < TB; ++q) {
+                p1[q] = p2[q] = floatx4{};
+                if (kFold) {
+#pragma unroll
+                    for (int k = 0; k < 2; ++k)
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) sGt[q][k][4 * kg + r][gs] = sGc[q][k][4 * kg + r][gs] = 0.0f;
+                }
+            }
             auto gstage = [&](int64_t st, auto bufc) {
                 constexpr int buf = decltype(bufc)::value;
                 const uint4 tnext = tile_load(Ag, g.n_pad, st + 1 < n_stages ? st + 1 : st);
@@ -1206,6 +1242,7 @@ __global__ void __launch_bounds__(kThreads) mfma_visgen_sat_kernel(
                         p2[q] = mfma16(lc.v, hs.v, p2[q]);
                     }
                 }
+                if (kFold && (st % kGramFold) == kGramFold - 1) fold(p1, p2);  // wave-uniform
                 if (st + 1 < n_stages)
                     (void)form_rows(&sPix[buf ^ 1], live, sE[buf ^ 1][0], sE[buf ^ 1][TOCC ? 1 : 0]);
                 stage_end(st, tnext, pnext, bufc);
@@ -1216,14 +1253,15 @@ __global__ void __launch_bounds__(kThreads) mfma_visgen_sat_kernel(
                 gstage(st + 1, BufC<1>{});
             }
             if (st < n_stages) gstage(st, BufC<0>{});
+            if (kFold) fold(p1, p2);
             // D of 16x16x32: lane holds rows 4 (lane / 16) + r of column lane % 16
 #pragma unroll
             for (int q = 0; q < TB; ++q)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int a = 4 * kg + r;
-                    sG[q][0][a][gs] = p1[q][r];
-                    sG[q][1][a][gs] = p2[q][r];
+                    sG[q][0][a][gs] = kFold ? sGt[q][0][a][gs] + sGc[q][0][a][gs] : p1[q][r];
+                    sG[q][1][a][gs] = kFold ? sGt[q][1][a][gs] + sGc[q][1][a][gs] : p2[q][r];
                 }
             __syncthreads();  // sG whole
         }
