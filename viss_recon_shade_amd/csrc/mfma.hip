@@ -16,6 +16,14 @@
 // block.  Instructions: vis-gen v_mfma_f32_16x16x32_f16 (M = 16 maps, N = 16
 // visibilities, K = 32 pixels), recon v_mfma_f32_32x32x16_f16 (M = 32 maps,
 // N = 32 pixels, K = 16 visibilities).
+//
+// On orbit geometries the second family of kernels (the "per-satellite
+// factor" kernels further down) forms E from factors of single satellites:
+// u v w = (r_y - r_x) / lambda, so E_xy(p) = E_y(p) conj(E_x(p)) with E_s(p) =
+// exp(-2 pi i s (r_s / lambda) . (l, m, n - 1)), one FP64 phase and one sin/cos
+// per (satellite, step, pixel) instead of per (baseline, step, pixel), and the
+// occultation of a (visibility, pixel) pair is that of either satellite, so
+// zeroing each satellite's occulted factors zeroes exactly the shaded entries.
 #include <hip/hip_runtime.h>
 
 #include "kernels.hpp"
