@@ -98,14 +98,15 @@ def test_sat_factors_reversed_pairs_and_windows(ctx, report):
     g.close()
 
 
-@pytest.mark.parametrize("nside", [4, 2])
-def test_sat_factors_star_layout_with_padding(ctx, report, nside):
+@pytest.mark.parametrize("nside,n_t", [(4, 40), (2, 40), (8, 41)])
+def test_sat_factors_star_layout_with_padding(ctx, report, nside, n_t):
     """16 satellites (star-ordered waves and their K-step skips) on skies smaller
-    than one 256-pixel chunk, so the stages hold padding pixels."""
-    print(f"[sat] 16 satellites, Nside {nside} (padding pixels), 40 steps", flush=True)
+    than one 256-pixel chunk, so the stages hold padding pixels, and with an odd
+    step count, so the last block holds one step (its skip masks from one row)."""
+    print(f"[sat] 16 satellites, Nside {nside}, {n_t} steps", flush=True)
     rng = np.random.default_rng(15 + nside)
-    g = ctx.geometry(16, 40, 1e7, nside)
+    g = ctx.geometry(16, n_t, 1e7, nside)
     I = rng.lognormal(0.0, 1.0, (24, g.n_pix)).astype(np.float32)
     for t_occ in (240.0, 0.0):
-        check(g, I, report, f"16 satellites, Nside {nside}, t_occ={t_occ:g}", t_occ=t_occ)
+        check(g, I, report, f"16 satellites, Nside {nside}, {n_t} steps, t_occ={t_occ:g}", t_occ=t_occ)
     g.close()
