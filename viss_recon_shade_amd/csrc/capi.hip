@@ -1108,6 +1108,28 @@ static vrs_status recon_common(vrs_ctx *ctx, const vrs_geom *g, const vrs_knobs 
     if (vrs_status s = sV.prepare()) return s;
     if (vrs_status s = sw.prepare()) return s;
     if (vrs_status s = sF.prepare()) return s;
+    if (k->precision == VRS_MFMA_SPLIT16 && n_maps == 1 && sat_factor_tb(g, k) > 0 &&
+        g->n_pad % 64 == 0) {
+        // one map: the pairs of a step as a Hermitian form in the satellite factors
+        const int64_t n_seg = quad_recon_segments(g->n_pad, g->n_t);
+        const size_t small_b = 256;
+        void *wop, *small, *part;
+        if (vrs_status s = ws_get(ctx, 2, quad_recon_wop_bytes(g->n_t), &wop)) return s;
+        if (vrs_status s = ws_get(ctx, 3, small_b + quad_recon_rows_bytes(g->n_t), &small)) return s;
+        if (vrs_status s = ws_get(ctx, 1, sizeof(float) * n_seg * g->n_pad, &part)) return s;
+        const GeomDev gd = g->dev();
+        {
+            Timer tm(ctx, "mfma_recon");
+            VRS_HIP(ctx, launch_quad_recon(gd, phase_knobs(k), g->orb.inv_lambda_km,
+                                           (const float2 *)sV.dev, (const float *)sw.dev,
+                                           (float)kappa, (unsigned int *)small, (float *)small + 1,
+                                           wop, (char *)small + small_b, (int32_t)n_seg,
+                                           (float *)part, ctx->stream));
+        }
+        VRS_HIP(ctx, launch_recon_finalize(gd, 1, (int32_t)n_seg, g->n_pix, (const float *)part,
+                                           (float *)sF.dev, ctx->stream));
+        return sF.finish();
+    }
     if (k->precision == VRS_MFMA_SPLIT16 && sat_recon_ns(g, k) > 0) {
         const int64_t n_slots = mfma_recon_sat_slots(g->n_sat, g->n_t);
         const int64_t groups = (n_maps + 63) / 64;

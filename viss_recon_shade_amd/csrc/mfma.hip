@@ -1611,7 +1611,224 @@ __global__ void __launch_bounds__(kThreadsRS) mfma_recon_sat_kernel(
     }
 }
 
+// ------------------------------------------------------------------ one-map recon as a quadratic form
+//
+// With one map the recon sum over the pairs of a step is a Hermitian form in
+// the step's per-satellite factors e_s = (cos, sin) of 2 pi phi_s (masked:
+// zero where satellite s cannot see the pixel):
+//   F(p) = sum_t Re sum_b V'_b e_yb conj(e_xb) = sum_t sum_x c_x Zr_x + s_x Zi_x,
+//   Z_x = sum_y W_xy e_y,  W_xy = V'_b for pair b = (x, y), V' = V w kappa,
+// so each (step, pixel) costs the 16 factors and one 16 x 32 real matrix
+// product (Z from [c; s]) on the matrix cores, instead of a product per pair.
+// The real form of W: Zr = [Wr, -Wi] [c; s], Zi = [Wi, Wr] [c; s].  K order of
+// the product (the B operand a lane forms): positions 8 q .. 8 q + 7 hold c then
+// s of satellites 4 q .. 4 q + 3, so lane group q (lanes 16 q .. 16 q + 15) forms
+// the factors of satellites 4 q .. 4 q + 3 for pixel lane % 16, and the result
+// rows it receives (4 q + r, r < 4) are Z of those same satellites.
+
+// Satellite row of one step for the quadratic-form recon (48 bytes): position
+// scaled by s / lambda, and the cap fields the FP32 test reads.
+struct QRow {
+    double sx, sy, sz;
+    float dx, dy, dz, lo, hi, unused;
+};
+static_assert(sizeof(QRow) == 48, "quadratic-form row layout");
+constexpr int kQGroups = 4;  // 16-pixel groups per wave (the W operand is read once per 64 pixels)
+
+// Per step t (one 64-thread block): W from V' (duplicate pairs add), split into
+// hi + lo FP16 and stored as the A operands of the 16x16x32 instruction,
+// [t][Zr hi, Zr lo, Zi hi, Zi lo][lane] 16 bytes each (lane: row lane % 16, K
+// positions 8 (lane / 16) ..), scaled by the power of two that keeps max |V' |
+// under 2^14; and the step's 16 satellite rows (absent satellites: a cap that
+// covers every pixel, so their factors are zero).
+__global__ void __launch_bounds__(64) quad_prep_kernel(GeomDev g, double sc, const float2 *__restrict__ V,
+                                                       const float *__restrict__ w, float kappa,
+                                                       const unsigned int *__restrict__ amax,
+                                                       uint4 *__restrict__ wop, QRow *__restrict__ rows,
+                                                       float *__restrict__ inv_scale) {
+    __shared__ float2 sW[16][17];
+    const int t = blockIdx.x, l = threadIdx.x;
+    const float scl = row_scale(amax[0], fabsf(kappa));
+    for (int i = l; i < 16 * 17; i += 64) (&sW[0][0])[i] = make_float2(0.0f, 0.0f);
+    if (l < 16) {
+        QRow q = {0.0, 0.0, 0.0, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        if (l < g.n_sat) {
+            const int64_t r = (int64_t)l * g.n_t + t;
+            const double4 p = g.pos[r];
+            const CapRec c = g.cap[r];
+            q = {p.x * sc, p.y * sc, p.z * sc, c.dx, c.dy, c.dz, c.lo, c.hi, 0.f};
+        }
+        rows[(int64_t)t * 16 + l] = q;
+    }
+    __syncthreads();
+    for (int b = l; b < g.n_base; b += 64) {
+        const int2 pr = g.pairs[b];
+        const int64_t v = (int64_t)b * g.n_t + t;
+        const float2 x = V[v];
+        const float ww = (w ? w[v] : 1.0f) * kappa * scl;
+        atomicAdd(&sW[pr.x][pr.y].x, x.x * ww);
+        atomicAdd(&sW[pr.x][pr.y].y, x.y * ww);
+    }
+    __syncthreads();
+    const int row = l & 15, q = l >> 4;
+    H8 o[4];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const float2 x = sW[row][4 * q + (j & 3)];
+        const float ar = j < 4 ? x.x : -x.y, ai = j < 4 ? x.y : x.x;
+        const _Float16 rh = (_Float16)ar, ih = (_Float16)ai;
+        o[0].v[j] = rh;
+        o[1].v[j] = (_Float16)(ar - (float)rh);
+        o[2].v[j] = ih;
+        o[3].v[j] = (_Float16)(ai - (float)ih);
+    }
+#pragma unroll
+    for (int pl = 0; pl < 4; ++pl)
+        wop[((int64_t)t * 4 + pl) * 64 + l] = make_uint4(o[pl].u[0], o[pl].u[1], o[pl].u[2], o[pl].u[3]);
+    if (t == 0 && l == 0) inv_scale[0] = 1.0f / scl;
+}
+
+// Quadratic-form recon: each wave holds 64 pixels (4 groups of 16) over one
+// segment of steps and needs no LDS and no barrier: per step it reads the W
+// operand (4 KiB, shared through the caches by every wave at that step) and
+// the rows of its lane groups' satellites, forms each lane's 16 factors
+// (4 satellites x 4 pixels; FP64 phase and fract, FP32 sin/cos, the FP32 cap
+// test with the exact FP64 predicate inside the guard band), and issues per
+// group three products for Zr and three for Zi (hi*hi + hi*lo + lo*hi).  The
+// next step's operand and rows load while the current step computes.  part:
+// [n_seg][n_pad] partial images (unscaled by inv_scale).
+__global__ void __launch_bounds__(256, 2) quad_recon_kernel(GeomDev g, PhaseKnobs k,
+                                                            const uint4 *__restrict__ wop,
+                                                            const QRow *__restrict__ rows,
+                                                            const float *__restrict__ inv_scale,
+                                                            int32_t n_seg, float *__restrict__ part) {
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int col = lane & 15, q = lane >> 4;
+    const int64_t pt = blockIdx.x / n_seg, seg = blockIdx.x % n_seg;
+    const int64_t base = (pt * 4 + wave) * (16 * kQGroups);
+    if (base >= g.n_pad) return;  // wave-uniform; the kernel has no barrier
+    const int64_t per = ((int64_t)g.n_t + n_seg - 1) / n_seg;
+    const int64_t t0 = seg * per;
+    const int64_t t1 = t0 + per < g.n_t ? t0 + per : g.n_t;
+    const double dzo = k.wfac ? 0.0 : 1.0;
+    const float neg1 = opaque_neg1();
+    double px[kQGroups], py[kQGroups], pz[kQGroups], pzd[kQGroups];
+    float fx[kQGroups], fy[kQGroups], fz[kQGroups], F[kQGroups];
+#pragma unroll
+    for (int gi = 0; gi < kQGroups; ++gi) {
+        const int64_t p = base + 16 * gi + col;
+        const double4 s = g.pix64[p];
+        const float4 s32 = g.pix32[p];
+        px[gi] = s.x;
+        py[gi] = s.y;
+        pz[gi] = s.z;
+        pzd[gi] = s.z - dzo;
+        fx[gi] = s32.x;
+        fy[gi] = s32.y;
+        fz[gi] = s32.z;
+        F[gi] = 0.0f;
+    }
+    struct StepIn {
+        uint4 a[4];
+        QRow r[4];
+    };
+    auto load_step = [&](int64_t t) {
+        StepIn s;
+        const uint4 *wt = wop + t * 4 * 64 + lane;
+#pragma unroll
+        for (int pl = 0; pl < 4; ++pl) s.a[pl] = wt[pl * 64];
+        const QRow *rr = rows + t * 16 + 4 * q;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) s.r[r] = rr[r];
+        return s;
+    };
+    StepIn cur = load_step(t0 < t1 ? t0 : 0);
+    for (int64_t t = t0; t < t1; ++t) {
+        const StepIn nxt = load_step(t + 1 < t1 ? t + 1 : t);
+        float mc[kQGroups][4], ms[kQGroups][4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const QRow &R = cur.r[r];
+#pragma unroll
+            for (int gi = 0; gi < kQGroups; ++gi) {
+                const double ph = fma(R.sx, px[gi], fma(R.sy, py[gi], R.sz * pzd[gi]));
+                const float rv = (float)__builtin_amdgcn_fract(ph);
+                const float x = fmaf(fz[gi], R.dz, fmaf(fy[gi], R.dy, fx[gi] * R.dx));
+                bool occ = x >= R.hi;
+                if (x >= R.lo && !occ)  // inside the guard band: the exact predicate
+                    occ = occ64m(g.pos[(int64_t)(4 * q + r) * g.n_t + t], px[gi], py[gi], pz[gi], g.limb);
+                mc[gi][r] = occ ? 0.0f : __builtin_amdgcn_cosf(rv);
+                ms[gi][r] = occ ? 0.0f : __builtin_amdgcn_sinf(rv);
+            }
+        }
+        H8 arh, arl, aih, ail;
+        arh.v = __builtin_bit_cast(half8, cur.a[0]);
+        arl.v = __builtin_bit_cast(half8, cur.a[1]);
+        aih.v = __builtin_bit_cast(half8, cur.a[2]);
+        ail.v = __builtin_bit_cast(half8, cur.a[3]);
+#pragma unroll
+        for (int gi = 0; gi < kQGroups; ++gi) {
+            H8 bh, bl;
+            split2m(mc[gi][0], mc[gi][1], neg1, bh.u[0], bl.u[0]);
+            split2m(mc[gi][2], mc[gi][3], neg1, bh.u[1], bl.u[1]);
+            split2m(ms[gi][0], ms[gi][1], neg1, bh.u[2], bl.u[2]);
+            split2m(ms[gi][2], ms[gi][3], neg1, bh.u[3], bl.u[3]);
+            floatx4 zr = mfma16(arh.v, bh.v, floatx4{});
+            zr = mfma16(arh.v, bl.v, zr);
+            zr = mfma16(arl.v, bh.v, zr);
+            floatx4 zi = mfma16(aih.v, bh.v, floatx4{});
+            zi = mfma16(aih.v, bl.v, zi);
+            zi = mfma16(ail.v, bh.v, zi);
+            float f = F[gi];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) f = fmaf(mc[gi][r], zr[r], fmaf(ms[gi][r], zi[r], f));
+            F[gi] = f;
+        }
+        cur = nxt;
+    }
+    // each pixel's sum over the four lane groups; lane group gi writes group gi
+    float out = 0.0f;
+#pragma unroll
+    for (int gi = 0; gi < kQGroups; ++gi) {
+        float v = F[gi];
+        v += __shfl_xor(v, 16, 64);
+        v += __shfl_xor(v, 32, 64);
+        if (gi == q) out = v;
+    }
+    part[seg * g.n_pad + base + 16 * q + col] = out * inv_scale[0];
+}
+
 }  // namespace
+
+int64_t quad_recon_segments(int64_t n_pad, int32_t n_t) {
+    const int64_t waves = n_pad / (16 * kQGroups);
+    int64_t n_seg = (12288 + waves - 1) / waves;
+    return n_seg < 1 ? 1 : (n_seg > n_t ? n_t : n_seg);
+}
+size_t quad_recon_wop_bytes(int32_t n_t) { return (size_t)n_t * 4 * 64 * sizeof(uint4); }
+size_t quad_recon_rows_bytes(int32_t n_t) { return (size_t)n_t * 16 * sizeof(QRow); }
+
+hipError_t launch_quad_recon(const GeomDev &g, const PhaseKnobs &k, double inv_lambda_km,
+                             const float2 *V, const float *w, float kappa, unsigned int *amax,
+                             float *inv_scale, void *wop, void *rows, int32_t n_seg, float *part,
+                             hipStream_t st) {
+    if (g.n_sat > 16 || g.n_pad % (16 * kQGroups) != 0) return hipErrorInvalidValue;
+    const double sc = k.s * inv_lambda_km;
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(256), 0, st, amax, 1);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(row_absmax_kernel, dim3(64, 1), dim3(256), 0, st, (const float *)V, g.n_vis,
+                       2 * g.n_vis, 1, w, amax, 0.0f);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(quad_prep_kernel, dim3(g.n_t), dim3(64), 0, st, g, sc, V, w, kappa, amax,
+                       (uint4 *)wop, (QRow *)rows, inv_scale);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    const int64_t ptiles = (g.n_pad / (16 * kQGroups) + 3) / 4;
+    hipLaunchKernelGGL(quad_recon_kernel, dim3((unsigned)(ptiles * n_seg)), dim3(256), 0, st, g, k,
+                       (const uint4 *)wop, (const QRow *)rows, (const float *)inv_scale, n_seg, part);
+    return hipGetLastError();
+}
 
 int mfma_vis_pad(int64_t n_vis, int64_t *n_vis_pad) {
     *n_vis_pad = (n_vis + kStage - 1) / kStage * kStage;

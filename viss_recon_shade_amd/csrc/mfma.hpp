@@ -49,6 +49,16 @@ hipError_t launch_mfma_recon_sat(const GeomDev &g, const PhaseKnobs &k, double i
                                  int64_t n_slots, unsigned int *amax, float *inv_scale, void *a16,
                                  void *spos, int32_t n_seg, float *part, int grid, hipStream_t st);
 const void *mfma_recon_sat_kernel_ptr(int ns);
+// One-map recon on per-satellite factors (at most 16 satellites, any pair list):
+// step segments, workspace sizes (W operand, satellite rows) and the launch;
+// part: [n_seg][n_pad] partial images, summed by launch_recon_finalize.
+int64_t quad_recon_segments(int64_t n_pad, int32_t n_t);
+size_t quad_recon_wop_bytes(int32_t n_t);
+size_t quad_recon_rows_bytes(int32_t n_t);
+hipError_t launch_quad_recon(const GeomDev &g, const PhaseKnobs &k, double inv_lambda_km,
+                             const float2 *V, const float *w, float kappa, unsigned int *amax,
+                             float *inv_scale, void *wop, void *rows, int32_t n_seg, float *part,
+                             hipStream_t st);
 int mfma_recon_sat_threads();
 int mfma_recon_sat_pix_per_block();
 
