@@ -133,7 +133,9 @@ typedef struct vrs_knobs {
                             * each satellite's own occultation zeroing its factor; with
                             * t_occ != 0 the shaded term is t_occ times each pair's full-sky
                             * sum, the maps entering as I - t_occ); 0: every (visibility,
-                            * pixel) entry forms its own FP64 phase and sin/cos. */
+                            * pixel) entry forms its own FP64 phase and sin/cos.  With one
+                            * map, recon on such a geometry treats each step's pairs as a
+                            * Hermitian form in the satellite factors. */
 } vrs_knobs;
 
 typedef struct vrs_shade_stats {
