@@ -1,8 +1,13 @@
-"""One-map matrix-core recon on per-satellite factors (precision "mfma", one map,
-an orbit geometry of at most 16 satellites): each step's pairs as a Hermitian
-form in the satellite factors, F = sum_x c_x Zr_x + s_x Zi_x with Z = W e.
+"""One-map matrix-core vis-gen and recon on per-satellite factors (precision
+"mfma", one map, an orbit geometry of at most 16 satellites): vis-gen as Gram
+matrices of the step's satellite factors over the pixels (masked and weighted by
+I - t_occ, plus the full-sky sums), recon as a Hermitian form in them, F =
+sum_x c_x Zr_x + s_x Zi_x with Z = W e.
 
-Every output against the FP64-phase VALU recon (which the oracle pins) and
+Vis-gen: every visibility against the FP64-phase VALU kernel and against the
+multi-map matrix-core kernels (map 0 of two), relative L2 2e-5, worst 5e-5.
+
+Recon: every output against the FP64-phase VALU recon (which the oracle pins) and
 against the multi-map matrix-core kernels run on the same V as map 0 of two:
 relative L2 over every pixel 5e-5, worst max-abs / max 1e-4 (the bounds of the
 other matrix-core recon tests).  Geometries: c1 (8 satellites), 16 satellites
@@ -42,6 +47,19 @@ def check(g, report, name, seed, preset="canonical", weights=True):
     print(f"[quad {name}] F {e:.2e} (worst {wo:.2e}, vs multi-map {em:.2e})", flush=True)
     assert e <= 5e-5 and wo <= 1e-4, (e, wo)
     assert em <= 5e-5, em
+    for t_occ in (240.0, 0.0):
+        km = vrs.knobs(preset, precision="mfma", t_occ=t_occ)
+        kx = vrs.knobs(preset, precision="mixed", t_occ=t_occ)
+        I = rng.lognormal(0.0, 1.0, (1, g.n_pix)).astype(np.float32)
+        Vg, Vx = g.visscal(I, km)[0].astype(np.float64), g.visscal(I, kx)[0].astype(np.float64)
+        I2 = np.concatenate([I, rng.lognormal(0.0, 1.0, (1, g.n_pix)).astype(np.float32)])
+        Vm = g.visscal(I2, km)[0].astype(np.float64)
+        ev, wv, evm = rel(Vg, Vx), worst(Vg, Vx), rel(Vg, Vm)
+        report(f"one-map Gram vis-gen {name} t_occ={t_occ:g}", V_rel=ev, V_worst=wv, V_vs_multimap=evm)
+        print(f"[gram {name} t_occ={t_occ:g}] V {ev:.2e} (worst {wv:.2e}, vs multi-map {evm:.2e})",
+              flush=True)
+        assert ev <= 2e-5 and wv <= 5e-5, (ev, wv)
+        assert evm <= 2e-5, evm
 
 
 @pytest.mark.parametrize("preset", ["canonical", "reference"])

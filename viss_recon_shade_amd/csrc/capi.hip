@@ -990,6 +990,21 @@ vrs_status vrs_visgen(vrs_ctx *ctx, const vrs_geom *g, const vrs_knobs *k, int32
     Staged sV(ctx, V, sizeof(float) * 2 * n_maps * g->n_vis, true);
     if (vrs_status s = sI.prepare()) return s;
     if (vrs_status s = sV.prepare()) return s;
+    if (k->precision == VRS_MFMA_SPLIT16 && n_maps == 1 && sat_factor_tb(g, k) > 0 &&
+        g->n_pad % 32 == 0) {
+        // one map: each step's visibilities as entries of Gram matrices of the satellite factors
+        const int64_t n_seg = gram_visgen_segments(g->n_pad, g->n_t);
+        void *part, *small;
+        if (vrs_status s = ws_get(ctx, 0, gram_visgen_part_bytes(g->n_t, n_seg), &part)) return s;
+        if (vrs_status s = ws_get(ctx, 3, 256 + quad_recon_rows_bytes(g->n_t), &small)) return s;
+        Timer tm(ctx, "mfma_visgen");
+        VRS_HIP(ctx, launch_gram_visgen(g->dev(), phase_knobs(k), g->orb.inv_lambda_km, g->n_pix,
+                                        (const float *)sI.dev, (unsigned int *)small,
+                                        (char *)small + 256, (int32_t)n_seg, (float *)part,
+                                        (float2 *)sV.dev, ctx->stream));
+        tm.stop();
+        return sV.finish();
+    }
     if (k->precision == VRS_MFMA_SPLIT16) {
         void *a16, *small;
         if (vrs_status s = ws_get(ctx, 0, mfma_a16_bytes_visgen(n_maps, g->n_pad), &a16)) return s;

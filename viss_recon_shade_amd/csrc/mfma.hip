@@ -1799,7 +1799,216 @@ __global__ void __launch_bounds__(256, 2) quad_recon_kernel(GeomDev g, PhaseKnob
     part[seg * g.n_pad + base + 16 * q + col] = out * inv_scale[0];
 }
 
+// ------------------------------------------------------------------ one-map vis-gen as Gram matrices
+//
+// With one map, the visibilities of a step are entries of 16 x 16 Gram
+// matrices of the step's satellite factors over the pixels (K = pixels):
+//   V_xy = sum_clear (I - t_occ) e_y conj(e_x) + t_occ U_xy, conjugated as
+// everywhere here (E = cos - i sin).  With A = w m (c, s) (w = (I - t_occ) x
+// scale, m the satellite's visibility of the pixel) and B = m (c, s):
+//   R = A_c B_c^T + A_s B_s^T,  Q = A_c B_s^T:  Re = R[x][y], Im = Q[x][y] - Q[y][x];
+// and the full-sky sums U as in mfma_visgen_sat_kernel's Gram wave (K + K^T and
+// P[x][y] - P[y][x]).  Each product is split hi + lo FP16 (three products).
+// The factor lane layout of the 16x16x32 instruction is the same for A and B
+// (row or column = satellite lane % 16, K = pixels 8 (lane / 16) ..), so every
+// lane forms one satellite's factors for 8 pixels of a K-step and holds both
+// operands.
+constexpr int kGTB = 2;  // steps per block
+
+__global__ void __launch_bounds__(256) qrows_kernel(GeomDev g, double sc, QRow *__restrict__ rows) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)g.n_t * 16) return;
+    const int sat = (int)(i % 16);
+    const int64_t t = i / 16;
+    QRow q = {0.0, 0.0, 0.0, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if (sat < g.n_sat) {
+        const int64_t r = (int64_t)sat * g.n_t + t;
+        const double4 p = g.pos[r];
+        const CapRec c = g.cap[r];
+        q = {p.x * sc, p.y * sc, p.z * sc, c.dx, c.dy, c.dz, c.lo, c.hi, 0.f};
+    }
+    rows[i] = q;
+}
+
+// Block: kGTB steps x one segment of the K-steps (32 pixels each), which the 4
+// waves take in turn; the waves' Gram blocks are summed in LDS and stored as
+// part[t][seg][R, Q, K, P][16][16].
+__global__ void __launch_bounds__(256, 2) gram_visgen_kernel(GeomDev g, PhaseKnobs k,
+                                                             const float *__restrict__ I,
+                                                             const unsigned int *__restrict__ amax,
+                                                             const QRow *__restrict__ rows,
+                                                             int32_t n_seg, float *__restrict__ part) {
+    __shared__ float sP[4][kGTB * 4 * 256];
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int gs = lane & 15, kg = lane >> 4;
+    const int64_t tg = blockIdx.x / n_seg, seg = blockIdx.x % n_seg;
+    const int64_t t0 = tg * kGTB;
+    const int64_t n_ks = g.n_pad / kKSV;
+    const int64_t ks0 = seg * n_ks / n_seg, ks1 = (seg + 1) * n_ks / n_seg;
+    const double dzo = k.wfac ? 0.0 : 1.0;
+    const float t_occ = k.t_occ;
+    const float scl = row_scale(amax[0], 1.0f);
+    const float neg1 = opaque_neg1();
+    QRow R[kGTB];
+    int64_t tq[kGTB];
+#pragma unroll
+    for (int q = 0; q < kGTB; ++q) {
+        tq[q] = t0 + q < g.n_t ? t0 + q : g.n_t - 1;  // a step past the end: its sums are not stored
+        R[q] = rows[tq[q] * 16 + gs];
+    }
+    floatx4 acc[kGTB][4];
+#pragma unroll
+    for (int q = 0; q < kGTB; ++q)
+#pragma unroll
+        for (int m = 0; m < 4; ++m) acc[q][m] = floatx4{};
+    for (int64_t ks = ks0 + wave; ks < ks1; ks += 4) {
+        double px[8], py[8], pz[8];
+        float fx[8], fy[8], fz[8], w[8];
+        uint32_t padm = 0;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int64_t j = ks * kKSV + 8 * kg + i;
+            const double4 s = g.pix64[j];
+            const float4 s32 = g.pix32[j];
+            const int32_t rg = g.ring[j];
+            px[i] = s.x;
+            py[i] = s.y;
+            pz[i] = s.z;
+            fx[i] = s32.x;
+            fy[i] = s32.y;
+            fz[i] = s32.z;
+            w[i] = rg >= 0 ? (I[rg] - t_occ) * scl : 0.0f;
+            padm |= (uint32_t)(rg < 0) << i;
+        }
+#pragma unroll
+        for (int q = 0; q < kGTB; ++q) {
+            float c[8], sn[8], mc[8], ms[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const double ph = fma(R[q].sx, px[i], fma(R[q].sy, py[i], R[q].sz * (pz[i] - dzo)));
+                const float rv = (float)__builtin_amdgcn_fract(ph);
+                const float x = fmaf(fz[i], R[q].dz, fmaf(fy[i], R[q].dy, fx[i] * R[q].dx));
+                bool occ = x >= R[q].hi;
+                if (x >= R[q].lo && !occ)  // inside the guard band: the exact predicate
+                    occ = occ64m(g.pos[(int64_t)gs * g.n_t + tq[q]], px[i], py[i], pz[i], g.limb);
+                const bool pad = (padm >> i) & 1u;
+                c[i] = pad ? 0.0f : __builtin_amdgcn_cosf(rv);
+                sn[i] = pad ? 0.0f : __builtin_amdgcn_sinf(rv);
+                mc[i] = occ ? 0.0f : c[i];
+                ms[i] = occ ? 0.0f : sn[i];
+            }
+            H8 ahc, alc, ahs, als, bhc, blc, bhs, bls, hc, lc, hs, ls, hc2, hs2;
+#pragma unroll
+            for (int p = 0; p < 4; ++p) {
+                split2m(w[2 * p] * mc[2 * p], w[2 * p + 1] * mc[2 * p + 1], neg1, ahc.u[p], alc.u[p]);
+                split2m(w[2 * p] * ms[2 * p], w[2 * p + 1] * ms[2 * p + 1], neg1, ahs.u[p], als.u[p]);
+                split2m(mc[2 * p], mc[2 * p + 1], neg1, bhc.u[p], blc.u[p]);
+                split2m(ms[2 * p], ms[2 * p + 1], neg1, bhs.u[p], bls.u[p]);
+                split2m(c[2 * p], c[2 * p + 1], neg1, hc.u[p], lc.u[p]);
+                split2m(sn[2 * p], sn[2 * p + 1], neg1, hs.u[p], ls.u[p]);
+            }
+            hc2.v = hc.v * (_Float16)0.5f;
+            hs2.v = hs.v * (_Float16)0.5f;
+            floatx4 a = acc[q][0];
+            a = mfma16(ahc.v, bhc.v, a);
+            a = mfma16(ahc.v, blc.v, a);
+            a = mfma16(alc.v, bhc.v, a);
+            a = mfma16(ahs.v, bhs.v, a);
+            a = mfma16(ahs.v, bls.v, a);
+            a = mfma16(als.v, bhs.v, a);
+            acc[q][0] = a;
+            a = acc[q][1];
+            a = mfma16(ahc.v, bhs.v, a);
+            a = mfma16(ahc.v, bls.v, a);
+            a = mfma16(alc.v, bhs.v, a);
+            acc[q][1] = a;
+            a = acc[q][2];
+            a = mfma16(hc2.v, hc.v, a);
+            a = mfma16(hc.v, lc.v, a);
+            a = mfma16(hs2.v, hs.v, a);
+            a = mfma16(hs.v, ls.v, a);
+            acc[q][2] = a;
+            a = acc[q][3];
+            a = mfma16(hc.v, hs.v, a);
+            a = mfma16(hc.v, ls.v, a);
+            a = mfma16(lc.v, hs.v, a);
+            acc[q][3] = a;
+        }
+    }
+    // D of 16x16x32: lane holds rows 4 (lane / 16) + r of column lane % 16
+#pragma unroll
+    for (int q = 0; q < kGTB; ++q)
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) sP[wave][(q * 4 + m) * 256 + (4 * kg + r) * 16 + gs] = acc[q][m][r];
+    __syncthreads();
+    for (int i = threadIdx.x; i < kGTB * 4 * 256; i += 256) {
+        const int q = i / 1024;
+        if (t0 + q < g.n_t)
+            part[((t0 + q) * n_seg + seg) * 1024 + (i % 1024)] = (sP[0][i] + sP[1][i]) + (sP[2][i] + sP[3][i]);
+    }
+}
+
+// V of each (pair, step) from the segments' Gram blocks (summed in FP64).
+__global__ void __launch_bounds__(256) gram_visgen_finish_kernel(GeomDev g, PhaseKnobs k, int32_t n_seg,
+                                                                 const unsigned int *__restrict__ amax,
+                                                                 const float *__restrict__ part,
+                                                                 float2 *__restrict__ V) {
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= g.n_vis) return;
+    const int64_t b = v / g.n_t, t = v % g.n_t;
+    const int2 pr = g.pairs[b];
+    const int x = pr.x, y = pr.y;
+    double r = 0.0, qa = 0.0, qb = 0.0, ka = 0.0, kb = 0.0, pa = 0.0, pb = 0.0;
+    for (int s = 0; s < n_seg; ++s) {
+        const float *G = part + (t * n_seg + s) * 1024;
+        r += G[x * 16 + y];
+        qa += G[256 + x * 16 + y];
+        qb += G[256 + y * 16 + x];
+        ka += G[512 + x * 16 + y];
+        kb += G[512 + y * 16 + x];
+        pa += G[768 + x * 16 + y];
+        pb += G[768 + y * 16 + x];
+    }
+    const double inv = 1.0 / (double)row_scale(amax[0], 1.0f);
+    const double to = k.t_occ;
+    V[v] = make_float2((float)(r * inv + to * (ka + kb)), (float)(-((qa - qb) * inv) - to * (pa - pb)));
+}
+
 }  // namespace
+
+int64_t gram_visgen_segments(int64_t n_pad, int32_t n_t) {
+    const int64_t n_tg = ((int64_t)n_t + kGTB - 1) / kGTB;
+    const int64_t most = n_pad / kKSV / 4 > 1 ? n_pad / kKSV / 4 : 1;  // at least one K-step per wave
+    const int64_t s = (8192 + n_tg - 1) / n_tg;
+    return s < 1 ? 1 : (s > most ? most : s);
+}
+size_t gram_visgen_part_bytes(int32_t n_t, int64_t n_seg) { return (size_t)n_t * n_seg * 1024 * sizeof(float); }
+
+hipError_t launch_gram_visgen(const GeomDev &g, const PhaseKnobs &k, double inv_lambda_km, int64_t n_pix,
+                              const float *I, unsigned int *amax, void *rows, int32_t n_seg, float *part,
+                              float2 *V, hipStream_t st) {
+    if (g.n_sat > 16 || g.n_pad % kKSV != 0) return hipErrorInvalidValue;
+    const double sc = k.s * inv_lambda_km;
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(256), 0, st, amax, 1);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(row_absmax_kernel, dim3(64, 1), dim3(256), 0, st, I, n_pix, n_pix, 0, nullptr,
+                       amax, k.t_occ);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    const int64_t nr = (int64_t)g.n_t * 16;
+    hipLaunchKernelGGL(qrows_kernel, dim3((unsigned)((nr + 255) / 256)), dim3(256), 0, st, g, sc, (QRow *)rows);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    const int64_t n_tg = ((int64_t)g.n_t + kGTB - 1) / kGTB;
+    hipLaunchKernelGGL(gram_visgen_kernel, dim3((unsigned)(n_tg * n_seg)), dim3(256), 0, st, g, k, I,
+                       (const unsigned int *)amax, (const QRow *)rows, n_seg, part);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(gram_visgen_finish_kernel, dim3((unsigned)((g.n_vis + 255) / 256)), dim3(256), 0, st,
+                       g, k, n_seg, (const unsigned int *)amax, (const float *)part, V);
+    return hipGetLastError();
+}
 
 int64_t quad_recon_segments(int64_t n_pad, int32_t n_t) {
     const int64_t waves = n_pad / (16 * kQGroups);

@@ -49,6 +49,14 @@ hipError_t launch_mfma_recon_sat(const GeomDev &g, const PhaseKnobs &k, double i
                                  int64_t n_slots, unsigned int *amax, float *inv_scale, void *a16,
                                  void *spos, int32_t n_seg, float *part, int grid, hipStream_t st);
 const void *mfma_recon_sat_kernel_ptr(int ns);
+// One-map vis-gen on per-satellite factors (at most 16 satellites, any pair
+// list): pixel segments, the Gram-block workspace, and the launch (rows:
+// quad_recon_rows_bytes; amax: one word).
+int64_t gram_visgen_segments(int64_t n_pad, int32_t n_t);
+size_t gram_visgen_part_bytes(int32_t n_t, int64_t n_seg);
+hipError_t launch_gram_visgen(const GeomDev &g, const PhaseKnobs &k, double inv_lambda_km, int64_t n_pix,
+                              const float *I, unsigned int *amax, void *rows, int32_t n_seg, float *part,
+                              float2 *V, hipStream_t st);
 // One-map recon on per-satellite factors (at most 16 satellites, any pair list):
 // step segments, workspace sizes (W operand, satellite rows) and the launch;
 // part: [n_seg][n_pad] partial images, summed by launch_recon_finalize.
