@@ -451,6 +451,10 @@ def main():
     # the same step with the plain FP32 recon kernel
     ms_allplain = timed_steps(torch, step_with(kp), max(2, args.steps // 2), 1)
     kt_allplain = kernel_avg_ms(ctx, step_with(kp), ["visgen", "recon"], 2)
+    # the whole step on the matrix cores: one-map Gram vis-gen + quadratic-form recon
+    ms_onemap = timed_steps(torch, step_with(kq), max(2, args.steps // 2), 1)
+    kt_onemap = kernel_avg_ms(ctx, step_with(kq), ["mfma_visgen", "mfma_recon"], 2)
+    Vq, Fq = state["V"].cpu().numpy(), state["F"].cpu().numpy()
     ms_mixed = timed_steps(torch, step_with(km), max(2, args.steps // 2), 1)
     kt_mixed = kernel_avg_ms(ctx, step_with(km), ["visgen", "recon"], 2)
     Vm, Fm = state["V"].cpu().numpy(), state["F"].cpu().numpy()
@@ -548,6 +552,13 @@ def main():
                         "kernels_ms": {k: round(v, 3) for k, v in kt_allplain.items()},
                         "note": "the headline step with the plain FP32 recon kernel (every "
                                 "(visibility, pixel) term forms its own phase, as the reference)"},
+        "one_map_split16": {"value": round(terms / (ms_onemap * 1e-3) / 1e9, 2),
+                            "ms_per_step": round(ms_onemap, 3),
+                            "kernels_ms": {k: round(v, 3) for k, v in kt_onemap.items()},
+                            "precision": "VRS_MFMA_SPLIT16",
+                            "note": "one map on the matrix cores: vis-gen as Gram matrices of the "
+                                    "per-satellite factors (gram_visgen_kernel), recon as a "
+                                    "Hermitian form in them (quad_recon_kernel)"},
         "mixed_f64phase": {"value": round(terms / (ms_mixed * 1e-3) / 1e9, 2),
                            "ms_per_step": round(ms_mixed, 3),
                            "kernels_ms": {k: round(v, 3) for k, v in kt_mixed.items()}},
@@ -571,6 +582,7 @@ def main():
     if not args.no_accuracy:
         out["accuracy"] = {"plain": accuracy(g, wl, pos, uvw, lmn, I, Vp, Fp, refs=refs),
                            "mixed": accuracy(g, wl, pos, uvw, lmn, I, Vm, Fm),
+                           "one_map_split16": accuracy(g, wl, pos, uvw, lmn, I, Vq, Fq),
                            "mask": mask_check(g, wl, pos, lmn, args.mask_stride)}
     if not args.no_cpu:
         Vc = Vp[0, :, 0].astype(np.float64) + 1j * Vp[0, :, 1]

@@ -134,8 +134,9 @@ typedef struct vrs_knobs {
                             * t_occ != 0 the shaded term is t_occ times each pair's full-sky
                             * sum, the maps entering as I - t_occ); 0: every (visibility,
                             * pixel) entry forms its own FP64 phase and sin/cos.  With one
-                            * map, recon on such a geometry treats each step's pairs as a
-                            * Hermitian form in the satellite factors. */
+                            * map, vis-gen on such a geometry forms each step's visibilities
+                            * as Gram matrices of the satellite factors and recon treats
+                            * them as a Hermitian form in the factors. */
 } vrs_knobs;
 
 typedef struct vrs_shade_stats {
